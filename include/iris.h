@@ -1,0 +1,153 @@
+/*
+ * iris.h -- C ABI of the MI355X FDTD engine for IrisFDTD-Academic's time loop.
+ *
+ * The reference runs its whole time-stepping loop inside one Fortran
+ * subroutine, `driver(e,h,eps_inv,bbox,fft_data,fft_data_o,store_pts,fdata)`
+ * (reference code/driver.f90:25-58 interface, :66 definition, loop
+ * :338-500, sole caller code/main.f90:179).  Its arguments are Fortran
+ * pointer arrays and every scalar travels through module globals
+ * (code/parameters.f90:82-257), so it has no C-callable form.  This header
+ * replaces that one call with a handle-based API:
+ *
+ *   iris_create   <- driver setup (code/driver.f90:174-335): UPML tables,
+ *                    Fourier boxes, TF/SF ranges, field allocation
+ *   iris_step     <- loop body (code/driver.f90:355-450): E/UPML/max wraps,
+ *                    H/UPML/min wraps, TF/SF, on-the-fly DFT
+ *   iris_flux     <- scattering_fourier_out numbers (code/dataprocess.f90:213-277);
+ *                    the host writes the <box>_scatt<j>.dat files itself
+ *   iris_get_dft  <- the emg(j)%fdata accumulators (code/dataprocess.f90:143-147)
+ *   iris_get_field<- e_cur / h_cur arrays (code/main.f90:166-167)
+ *   iris_destroy  <- driver teardown (code/driver.f90:503-552)
+ *
+ * Conventions: every entry point returns 0 on success and a negative
+ * IRIS_E* code on failure; nothing exits the process.  Host buffers are
+ * borrowed for the duration of the call; device memory belongs to the handle.
+ * A handle owns one HIP stream; calls on one handle must come from one host
+ * thread at a time; distinct handles are independent.  Complex values are
+ * interleaved (re, im) doubles in the Fortran storage order of the
+ * reference's arrays.
+ */
+#ifndef IRIS_H
+#define IRIS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IRIS_ABI_VERSION 1
+
+/* error codes */
+#define IRIS_OK 0
+#define IRIS_EINVAL -1   /* bad argument or inconsistent parameters      */
+#define IRIS_ENOMEM -2   /* device allocation failed                     */
+#define IRIS_EDEVICE -3  /* HIP runtime error (no device, launch failure) */
+#define IRIS_ESTATE -4   /* call out of order (e.g. step past nt)         */
+#define IRIS_EUNSUP -5   /* parameter combination not supported           */
+
+/*
+ * POD mirror of the reference's module globals after setparam
+ * (code/files.f90:39-344), set_emconstants (code/mesh.f90:75-109) and the
+ * driver's TF/SF setup (code/driver.f90:310-335).  All indices use the
+ * reference's 1-based Fortran convention.  Layout: 34 int32 then doubles, no
+ * padding, so Fortran can mirror it with a bind(C) derived type.
+ */
+typedef struct iris_params {
+  int32_t abi_version;    /* IRIS_ABI_VERSION                                   */
+  int32_t precision;      /* 8: FP64 fields; 4: FP32 fields, FP64 DFT           */
+  int32_t nx, ny, nz;     /* ixmax, iymax, izmax                                */
+  int32_t pmlzi, pmlzf;   /* z-UPML: [1,pmlzi] and [pmlzf,nz]                   */
+  int32_t hi, hf;         /* structure z-range [hi,hf]                          */
+  int32_t i_pol;          /* 1: E//y, 2: E//x                                   */
+  int32_t em_source;      /* 1: TF/SF plane-wave source on                      */
+  int32_t plane_wave;     /* 1: Gaussian pulse (the only working source)        */
+  int32_t cosine;         /* 0: complex analytic source, 1: real source         */
+  int32_t iz_min, iz_max; /* TF/SF planes tfsf_zi, tfsf_zf                      */
+  int32_t ixo;            /* pulse x-origin, pmlzi-50 (code/tfsf.f90:264)       */
+  int32_t nt;             /* texit                                              */
+  int32_t to_fourier;     /* first DFT step                                     */
+  int32_t sample_fourier; /* DFT sampling period                                */
+  int32_t no_downloads;   /* output cadence in steps                            */
+  int32_t nof;            /* number of wavelengths                              */
+  int32_t nbox;           /* 1 if the two Fourier boxes coincide, else 2        */
+  int32_t box[2][6];      /* [box][sxi,sxf,syi,syf,szi,szf]                     */
+  /* doubles */
+  double q;               /* Q: cell size in atomic units (f_abohr * q_nm)      */
+  double dt;              /* time step, atomic units                            */
+  double c0, eps0, mu0;   /* code/physconst.f90:31-33 as FP64 values            */
+  double pi, f_abohr;     /* float32-rounded pi and nm->a.u. factor             */
+  double eps[3];          /* eps(1) cover, eps(2) structure (real part), eps(3) */
+  double eps_in[3];       /* 1/eps(i)                                           */
+  double epsa_inv[2][3][3]; /* [region I/III][row][col] (code/mesh.f90:90-95)    */
+  double eamp;            /* Epump in atomic units                              */
+  double wlpump;          /* carrier wavelength in nm                           */
+  double gpump;           /* pulse width in mesh units                          */
+  double zopump;          /* pulse z-origin in mesh units                       */
+  double ratio_angle;     /* incidence parameter, 1 = normal                    */
+  double ux, uz;          /* cos/sin(ratio_angle*pi/2) as the host evaluated    */
+  double win;             /* source window factor (FTwindow=0 -> 1.0)           */
+  double ak[3][2];        /* Bloch phases akx,aky,akz as (re,im)                */
+} iris_params;
+
+typedef struct iris_handle iris_handle;
+
+/*
+ * Create a simulation.  Arrays follow the reference's shapes:
+ *   eps_inv : complex, (nx+2)*(ny+2)*(hf-hi+3), x fastest, z from hi-1 to hf+1
+ *             (eps_inv(1,1,0:nx+1,0:ny+1,hi-1:hf+1), code/mesh.f90:98)
+ *   fpzmn, fmzmn : 3*pmlzi doubles, (1:3,1:pmlzi)         (code/driver.f90:189)
+ *   fpzmx, fmzmx : 3*(nz-pmlzf+1) doubles, (1:3,pmlzf:nz)  (code/driver.f90:190)
+ *   fw   : nof doubles, angular frequency times dt        (code/files.f90:217)
+ * Fields start at zero (code/main.f90:169-170).
+ */
+int iris_create(const iris_params* p, const double* eps_inv, const double* fpzmn,
+                const double* fmzmn, const double* fpzmx, const double* fmzmx,
+                const double* fw, iris_handle** out);
+
+/* Run time steps it_first..it_last inclusive (1-based, it_first must be the
+ * next step); steps 2-11 of the reference loop body for each. Asynchronous
+ * with respect to the host: it returns once the work is queued. */
+int iris_step(iris_handle* h, int32_t it_first, int32_t it_last);
+
+/* Wait until all queued work on the handle's stream is done. */
+int iris_sync(iris_handle* h);
+
+/* Face fluxes Jabs(6,nof) of box nobox (1 or 2) at the current step, scaled
+ * by eps0*Q/(2*dt); jabs[iw*6 + face], face 0..5 = J1..J6.  Synchronous. */
+int iris_flux(iris_handle* h, int32_t nobox, double* jabs);
+
+/* Number of complex points npts of face pair `pair` (1: x, 2: y, 3: z) of box
+ * nobox, as the reference allocates emg(pair)%fdata (1 for an absent pair). */
+int iris_dft_points(const iris_handle* h, int32_t nobox, int32_t pair, int64_t* npts);
+
+/* Copy emg(pair)%fdata of box nobox: npts*nof complex, points fastest.
+ * Synchronous. */
+int iris_get_dft(iris_handle* h, int32_t nobox, int32_t pair, double* out);
+
+/* Copy the E (which=0) or H (which=1) field as complex
+ * (3,0:nx+1,0:ny+1,0:nz+1), component fastest.  Synchronous. */
+int iris_get_field(iris_handle* h, int32_t which, double* out);
+
+/* Overwrite the E or H field (same layout as iris_get_field); used to start
+ * from a given state.  Synchronous. */
+int iris_set_field(iris_handle* h, int32_t which, const double* in);
+
+/* The step most recently completed (0 before the first step). */
+int iris_current_step(const iris_handle* h, int32_t* it);
+
+/* Release device memory and the stream. NULL is accepted. */
+int iris_destroy(iris_handle* h);
+
+/* Message for the last failure on this handle (or of the last failed
+ * iris_create when h is NULL).  Never NULL. */
+const char* iris_last_error(const iris_handle* h);
+
+/* Library version string. */
+const char* iris_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* IRIS_H */

@@ -1,0 +1,75 @@
+// iris_device.h -- device-side helpers and the kernel argument block of the
+// MI355X FDTD engine.  Complex values are T2 (float2/double2) with the
+// reference's evaluation order for every operation.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace iris {
+
+template <typename T> struct vec2;
+template <> struct vec2<float> { using type = float2; };
+template <> struct vec2<double> { using type = double2; };
+template <typename T> using cplx = typename vec2<T>::type;
+
+template <typename T> __host__ __device__ inline cplx<T> mk(T re, T im) { cplx<T> r; r.x = re; r.y = im; return r; }
+template <typename C> __host__ __device__ inline C cadd(C a, C b) { C r; r.x = a.x + b.x; r.y = a.y + b.y; return r; }
+template <typename C> __host__ __device__ inline C csub(C a, C b) { C r; r.x = a.x - b.x; r.y = a.y - b.y; return r; }
+// real * complex: the reference promotes the real to (r,0); for finite
+// operands that equals (r*re, r*im)
+template <typename C, typename T> __host__ __device__ inline C cscale(T s, C a) { C r; r.x = s * a.x; r.y = s * a.y; return r; }
+// complex * complex as (ac-bd, ad+bc)
+template <typename C> __host__ __device__ inline C cmul(C a, C b) {
+  C r; r.x = a.x * b.x - a.y * b.y; r.y = a.x * b.y + a.y * b.x; return r;
+}
+// complex / complex with logb/scalbn scaling (C99 Annex G / compiler-rt
+// __divdc3 algorithm), which is what the reference build uses for
+// complex/real division
+__device__ inline double2 cdiv_scaled(double2 a, double2 b) {
+  double c = b.x, d = b.y;
+  int k = 0;
+  double lw = logb(fmax(fabs(c), fabs(d)));
+  if (isfinite(lw)) { k = (int)lw; c = scalbn(c, -k); d = scalbn(d, -k); }
+  double den = c * c + d * d;
+  double2 r;
+  r.x = scalbn((a.x * c + a.y * d) / den, -k);
+  r.y = scalbn((a.y * c - a.x * d) / den, -k);
+  return r;
+}
+
+// Per-plane UPML coefficients, each one the FP64 value of a subexpression
+// of reference code/boundary_II.f90:86-99 and :170-183 evaluated on the host.
+// Index 0..2 follow the field component (1..3 in the reference).
+template <typename T> struct PmlPlane {
+  T fp[3], fm[3];   // fp_i, fm_i at this plane
+  T cd[3];          // D/B decay: fm2/fp2, fm3/fp3, fm1/fp1
+  T gd[3];          // D gain:    (2eps)/fp2, (2eps)/fp3, (2eps)/fp1
+  T gb[3];          // B gain:    ((2eps)/fp2)*dtq2, ...
+  T ce[3];          // E/H decay: fm3/fp3, fm1/fp1, fm2/fp2
+  T ge[3];          // E gain:    1/(eps0*eps*fp3), 1/(eps0*eps*fp1), 1/(eps0*eps*fp2)
+  T gh[3];          // H gain:    1/(mu0*fp3), 1/(mu0*fp1), 1/(mu0*fp2)
+};
+
+// Everything the E/H passes need, passed by value.
+template <typename T> struct StepArgs {
+  using C = cplx<T>;
+  C* e[3];
+  C* h[3];
+  C* d[2][3];       // UPML D, [bottom/top][component]
+  C* b[2][3];       // UPML B
+  const T* eps_re;  // structure planes hi..hf (xy plane layout)
+  const T* eps_im;  // nullptr when eps_inv is real
+  const PmlPlane<T>* pml;  // indexed by iz (0..nz+1)
+  int nx, ny, nz;
+  int64_t sx, sxy;  // x pitch and plane pitch of the field arrays
+  int xoff;         // storage offset of ix = 0 within a row
+  int pmlzi, pmlzf, hi, hf;
+  T a1[3], a3[3];   // diagonal of epsa_inv for regions I and III
+  T e02;            // eps0**2
+  T dtcq2;
+  C fxp, fyp, fzp;  // exp(+i ak) Bloch factors
+  C fxm, fym, fzm;  // exp(-i ak)
+  int zchunk;       // planes per thread in the z-marching passes
+};
+
+}  // namespace iris

@@ -1,0 +1,576 @@
+// iris_engine.hip -- C ABI (include/iris.h) of the MI355X FDTD engine.
+//
+// Replaces the time loop of IrisFDTD-Academic's driver()
+// (reference code/driver.f90:338-500).  Device memory layout:
+//   fields   E and H, one complex array per component, (nx+2)(ny+2)(nz+2)
+//            cells with explicit ghost planes; x fastest with a padded pitch
+//            so that ix = 1 starts on a 128-byte boundary; updated in place;
+//   UPML     D (E side) and B (H side) per component for the bottom
+//            [1,pmlzi] and top [pmlzf,nz] slabs, same xy pitch;
+//   eps_inv  real (and, only if non-zero, imaginary) part on the structure
+//            planes [hi,hf];
+//   DFT      per box, the reference's emg(j)%fdata blocks back to back in
+//            FP64 complex, plus a K-slot ring of face samples.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/iris.h"
+#include "iris_kernels.h"
+
+using namespace iris;
+
+namespace {
+
+constexpr int kDftSlots = 16;  // samples buffered before folding into emg
+
+thread_local std::string g_create_error = "no error";
+
+struct DeviceBox {
+  BoxGeom g{};
+  int active_mask = 0;
+  int64_t npts_ref[3] = {1, 1, 1};  // what the reference allocates
+  double2* emg = nullptr;
+  void* samp = nullptr;
+};
+
+}  // namespace
+
+struct iris_handle {
+  iris_params p{};
+  int prec = 8;
+  size_t csize = 16;  // bytes per complex value of the fields
+  hipStream_t stream = nullptr;
+  std::string err = "no error";
+  int device = 0;
+  // layout
+  int xoff = 0;
+  int64_t sx = 0, sy = 0, sxy = 0, ncell = 0;
+  // device buffers
+  void* f[6] = {};          // E0..E2, H0..H2
+  void* d[2][3] = {};
+  void* b[2][3] = {};
+  void* eps_re = nullptr;
+  void* eps_im = nullptr;
+  void* pml = nullptr;
+  double* fw = nullptr;
+  double2* ph = nullptr;
+  double* flux = nullptr;
+  DeviceBox box[2];
+  int nsamp = 0;            // samples waiting in the ring
+  int samp_n0 = 0;          // DFT phase index of the first waiting sample
+  int it = 0;
+  int zchunk = 1;
+  dim3 blk_yee, grd_yee, blk_tf, grd_tf;
+  double dtcq2 = 0;
+  std::vector<void*> allocs;
+};
+
+namespace {
+
+int fail(iris_handle* h, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (h) h->err = buf; else g_create_error = buf;
+  return code;
+}
+
+#define HIP_TRY(h, call)                                                                     \
+  do {                                                                                       \
+    hipError_t e_ = (call);                                                                  \
+    if (e_ != hipSuccess) return fail((h), IRIS_EDEVICE, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+int dalloc(iris_handle* h, void** p, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(h, IRIS_ENOMEM, "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+  }
+  h->allocs.push_back(*p);
+  e = hipMemsetAsync(*p, 0, bytes, h->stream);
+  if (e != hipSuccess) return fail(h, IRIS_EDEVICE, "hipMemsetAsync: %s", hipGetErrorString(e));
+  return 0;
+}
+
+int pow2ceil(int v) { int p = 1; while (p < v) p <<= 1; return p; }
+
+// Bloch factor exp(+-i*ak) as the reference evaluates cdexp(+-ci*ak)
+void bloch(const double ak[2], int sign, double out[2]) {
+  // ci*ak = (0*a - 1*b, 0*b + 1*a)
+  double re = 0.0 * ak[0] - 1.0 * ak[1], im = 0.0 * ak[1] + 1.0 * ak[0];
+  if (sign < 0) { re = -re; im = -im; }
+  double m = std::exp(re);
+  out[0] = m * std::cos(im);
+  out[1] = m * std::sin(im);
+}
+
+template <typename T>
+StepArgs<T> step_args(const iris_handle* h) {
+  using C = cplx<T>;
+  StepArgs<T> a{};
+  for (int c = 0; c < 3; ++c) {
+    a.e[c] = (C*)h->f[c];
+    a.h[c] = (C*)h->f[3 + c];
+    for (int k = 0; k < 2; ++k) { a.d[k][c] = (C*)h->d[k][c]; a.b[k][c] = (C*)h->b[k][c]; }
+  }
+  a.eps_re = (const T*)h->eps_re;
+  a.eps_im = (const T*)h->eps_im;
+  a.pml = (const PmlPlane<T>*)h->pml;
+  a.nx = h->p.nx; a.ny = h->p.ny; a.nz = h->p.nz;
+  a.sx = h->sx; a.sxy = h->sxy; a.xoff = h->xoff;
+  a.pmlzi = h->p.pmlzi; a.pmlzf = h->p.pmlzf; a.hi = h->p.hi; a.hf = h->p.hf;
+  for (int i = 0; i < 3; ++i) { a.a1[i] = (T)h->p.epsa_inv[0][i][i]; a.a3[i] = (T)h->p.epsa_inv[1][i][i]; }
+  a.e02 = (T)(h->p.eps0 * h->p.eps0);
+  a.dtcq2 = (T)h->dtcq2;
+  double f[2];
+  bloch(h->p.ak[0], +1, f); a.fxp = mk<T>((T)f[0], (T)f[1]);
+  bloch(h->p.ak[1], +1, f); a.fyp = mk<T>((T)f[0], (T)f[1]);
+  bloch(h->p.ak[2], +1, f); a.fzp = mk<T>((T)f[0], (T)f[1]);
+  bloch(h->p.ak[0], -1, f); a.fxm = mk<T>((T)f[0], (T)f[1]);
+  bloch(h->p.ak[1], -1, f); a.fym = mk<T>((T)f[0], (T)f[1]);
+  bloch(h->p.ak[2], -1, f); a.fzm = mk<T>((T)f[0], (T)f[1]);
+  a.zchunk = h->zchunk;
+  return a;
+}
+
+TfsfArgs tfsf_args(const iris_handle* h, int it) {
+  const iris_params& p = h->p;
+  TfsfArgs t{};
+  t.q = p.q; t.ux = p.ux; t.uz = p.uz; t.zopump = p.zopump;
+  t.ko = (2.0 * p.pi) / (p.wlpump * p.f_abohr);
+  t.dk = p.gpump * p.q;
+  double eps_ms;
+  if (p.iz_min > p.hi) { eps_ms = p.eps[2]; t.sign_kz = -1.0; } else { eps_ms = p.eps[0]; t.sign_kz = 1.0; }
+  t.sq = std::sqrt(eps_ms);
+  t.eps_ms = eps_ms;
+  t.c0dt_it = (p.c0 * p.dt) * (double)it;
+  t.amp = p.eamp * p.q;
+  t.hamp = t.amp * ((p.dt * p.c0) / p.q);
+  t.win = p.win;
+  t.ein = p.eps_in[0];
+  t.dtcq2 = h->dtcq2 * 1.0;
+  t.ixo = p.ixo;
+  t.iz_min = p.iz_min;
+  t.i_pol = p.i_pol;
+  t.cosine = p.cosine;
+  t.mode2d = (p.nx == 1 || p.ny == 1) ? 1 : 0;
+  return t;
+}
+
+template <typename T>
+int launch_step(iris_handle* h, int it) {
+  StepArgs<T> a = step_args<T>(h);
+  hipLaunchKernelGGL(k_epass<T>, h->grd_yee, h->blk_yee, 0, h->stream, a);
+  hipLaunchKernelGGL(k_hpass<T>, h->grd_yee, h->blk_yee, 0, h->stream, a);
+  if (h->p.em_source) {
+    TfsfArgs t = tfsf_args(h, it);
+    hipLaunchKernelGGL(k_tfsf<T>, h->grd_tf, h->blk_tf, 0, h->stream, a, t);
+  }
+  HIP_TRY(h, hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+int dft_flush(iris_handle* h) {
+  if (h->nsamp == 0) return 0;
+  const int nof = h->p.nof;
+  const int nk = h->nsamp;
+  hipLaunchKernelGGL(k_dft_phase, dim3((nof * nk + 255) / 256), dim3(256), 0, h->stream, h->fw, nof, nk,
+                     h->samp_n0, h->p.sample_fourier, h->ph);
+  for (int b = 0; b < h->p.nbox; ++b) {
+    DeviceBox& B = h->box[b];
+    if (B.g.ntot == 0) continue;
+    const int gy = std::min(nof, 64);
+    dim3 grid((unsigned)((B.g.ntot + 255) / 256), (unsigned)gy);
+    hipLaunchKernelGGL((k_dft_accum<T, kDftSlots>), grid, dim3(256), 0, h->stream, B.g, (const cplx<T>*)B.samp, nk,
+                       (const double2*)h->ph, nof, h->p.dt, B.emg);
+  }
+  HIP_TRY(h, hipGetLastError());
+  h->nsamp = 0;
+  return 0;
+}
+
+template <typename T>
+int dft_sample(iris_handle* h, int n) {
+  if (h->nsamp == 0) h->samp_n0 = n;
+  StepArgs<T> a = step_args<T>(h);
+  for (int b = 0; b < h->p.nbox; ++b) {
+    DeviceBox& B = h->box[b];
+    if (B.g.ntot == 0) continue;
+    hipLaunchKernelGGL(k_dft_sample<T>, dim3((unsigned)((B.g.ntot + 255) / 256)), dim3(256), 0, h->stream, a, B.g,
+                       (cplx<T>*)B.samp, h->nsamp);
+  }
+  HIP_TRY(h, hipGetLastError());
+  h->nsamp++;
+  if (h->nsamp == kDftSlots) return dft_flush<T>(h);
+  return 0;
+}
+
+template <typename T>
+int run_steps(iris_handle* h, int it_first, int it_last) {
+  for (int it = it_first; it <= it_last; ++it) {
+    int rc = launch_step<T>(h, it);
+    if (rc) return rc;
+    if (it >= h->p.to_fourier && (it % h->p.sample_fourier) == 0) {
+      rc = dft_sample<T>(h, it + 1 - h->p.to_fourier);
+      if (rc) return rc;
+    }
+    h->it = it;
+  }
+  return 0;
+}
+
+int flush(iris_handle* h) { return h->prec == 8 ? dft_flush<double>(h) : dft_flush<float>(h); }
+
+int check_params(const iris_params* p) {
+  if (!p) return fail(nullptr, IRIS_EINVAL, "params is NULL");
+  if (p->abi_version != IRIS_ABI_VERSION)
+    return fail(nullptr, IRIS_EINVAL, "abi_version %d, library expects %d", p->abi_version, IRIS_ABI_VERSION);
+  if (p->precision != 4 && p->precision != 8) return fail(nullptr, IRIS_EINVAL, "precision must be 4 or 8");
+  if (p->nx < 1 || p->ny < 1 || p->nz < 3) return fail(nullptr, IRIS_EINVAL, "grid %d x %d x %d too small", p->nx, p->ny, p->nz);
+  if ((int64_t)p->nx * p->ny * p->nz > (int64_t)1 << 33) return fail(nullptr, IRIS_EINVAL, "grid too large");
+  if (p->pmlzi < 2 || p->pmlzf <= p->pmlzi || p->pmlzf >= p->nz)
+    return fail(nullptr, IRIS_EINVAL, "need 2 <= pmlzi < pmlzf < nz (got %d, %d, nz=%d)", p->pmlzi, p->pmlzf, p->nz);
+  if (p->hi < 1 || p->hf < p->hi || p->hf > p->nz) return fail(nullptr, IRIS_EINVAL, "structure [%d,%d] outside [1,nz]", p->hi, p->hf);
+  if (p->em_source) {
+    if (p->plane_wave != 1) return fail(nullptr, IRIS_EUNSUP, "only the Gaussian TF/SF source (plane_wave=1) is supported");
+    if (p->i_pol != 1 && p->i_pol != 2) return fail(nullptr, IRIS_EUNSUP, "i_pol must be 1 or 2");
+    if (p->iz_min < 2 || p->iz_min > p->nz) return fail(nullptr, IRIS_EINVAL, "iz_min %d outside [2,nz]", p->iz_min);
+  }
+  if (p->nof < 1) return fail(nullptr, IRIS_EINVAL, "nof must be >= 1");
+  if (p->sample_fourier < 1) return fail(nullptr, IRIS_EINVAL, "sample_fourier must be >= 1");
+  if (p->nbox != 1 && p->nbox != 2) return fail(nullptr, IRIS_EINVAL, "nbox must be 1 or 2");
+  for (int b = 0; b < p->nbox; ++b) {
+    const int32_t* x = p->box[b];
+    if (x[0] < 1 || x[0] > x[1] || x[1] > p->nx || x[2] < 1 || x[2] > x[3] || x[3] > p->ny || x[4] < 1 ||
+        x[4] > x[5] || x[5] > p->nz)
+      return fail(nullptr, IRIS_EINVAL, "Fourier box %d lies outside the interior", b + 1);
+  }
+  return 0;
+}
+
+template <typename T>
+void fill_pml(const iris_params& p, const double* fpmn, const double* fmmn, const double* fpmx, const double* fmmx,
+              std::vector<PmlPlane<T>>& tab) {
+  tab.assign(p.nz + 2, PmlPlane<T>{});
+  const double dtq2 = (p.dt / p.q) * (p.dt / p.q);
+  for (int iz = 1; iz <= p.nz; ++iz) {
+    const double *fp, *fm;
+    double ep;
+    if (iz <= p.pmlzi) { fp = fpmn + 3 * (iz - 1); fm = fmmn + 3 * (iz - 1); ep = p.eps[0]; }
+    else if (iz >= p.pmlzf) { fp = fpmx + 3 * (iz - p.pmlzf); fm = fmmx + 3 * (iz - p.pmlzf); ep = p.eps[2]; }
+    else continue;
+    PmlPlane<T>& P = tab[iz];
+    // component c uses profiles (c+1)%3 for D/B and (c+2)%3 for the E/H decay
+    // (reference code/boundary_II.f90:86-99: 1->2,3 ; 2->3,1 ; 3->1,2)
+    for (int c = 0; c < 3; ++c) {
+      const int pd = (c + 1) % 3, pe = (c + 2) % 3;
+      P.fp[c] = (T)fp[c];
+      P.fm[c] = (T)fm[c];
+      P.cd[c] = (T)(fm[pd] / fp[pd]);
+      P.gd[c] = (T)((2.0 * ep) / fp[pd]);
+      P.gb[c] = (T)(((2.0 * ep) / fp[pd]) * dtq2);
+      P.ce[c] = (T)(fm[pe] / fp[pe]);
+      P.ge[c] = (T)(1.0 / (p.eps0 * ep * fp[pe]));
+      P.gh[c] = (T)(1.0 / (p.mu0 * fp[pe]));
+    }
+  }
+}
+
+template <typename T>
+int upload_tables(iris_handle* h, const double* eps_inv, const double* fpmn, const double* fmmn, const double* fpmx,
+                  const double* fmmx) {
+  const iris_params& p = h->p;
+  // UPML planes
+  std::vector<PmlPlane<T>> tab;
+  fill_pml<T>(p, fpmn, fmmn, fpmx, fmmx, tab);
+  int rc = dalloc(h, &h->pml, sizeof(PmlPlane<T>) * tab.size());
+  if (rc) return rc;
+  HIP_TRY(h, hipMemcpyAsync(h->pml, tab.data(), sizeof(PmlPlane<T>) * tab.size(), hipMemcpyHostToDevice, h->stream));
+  // eps_inv on planes hi..hf (input has planes hi-1..hf+1, x in 0..nx+1)
+  const int nzs = p.hf - p.hi + 1;
+  const size_t n = (size_t)h->sxy * nzs;
+  std::vector<T> re(n, (T)0), im(n, (T)0);
+  bool any_im = false;
+  const int64_t isx = p.nx + 2, isxy = (int64_t)(p.nx + 2) * (p.ny + 2);
+  for (int z = 0; z < nzs; ++z)
+    for (int y = 0; y <= p.ny + 1; ++y)
+      for (int x = 0; x <= p.nx + 1; ++x) {
+        const double* v = eps_inv + 2 * (x + isx * y + isxy * (z + 1));
+        const size_t o = (size_t)(x + h->xoff) + (size_t)h->sx * y + (size_t)h->sxy * z;
+        re[o] = (T)v[0];
+        im[o] = (T)v[1];
+        if (v[1] != 0.0) any_im = true;
+      }
+  rc = dalloc(h, &h->eps_re, sizeof(T) * n);
+  if (rc) return rc;
+  HIP_TRY(h, hipMemcpyAsync(h->eps_re, re.data(), sizeof(T) * n, hipMemcpyHostToDevice, h->stream));
+  if (any_im) {
+    rc = dalloc(h, &h->eps_im, sizeof(T) * n);
+    if (rc) return rc;
+    HIP_TRY(h, hipMemcpyAsync(h->eps_im, im.data(), sizeof(T) * n, hipMemcpyHostToDevice, h->stream));
+  }
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+void box_geometry(const iris_params& p, int b, DeviceBox& B) {
+  const int32_t* x = p.box[b];
+  BoxGeom& g = B.g;
+  g.bxi = x[0]; g.bxf = x[1]; g.byi = x[2]; g.byf = x[3]; g.bzi = x[4]; g.bzf = x[5];
+  g.nxb = g.bxf + 1 - g.bxi; g.nyb = g.byf + 1 - g.byi; g.nzb = g.bzf + 1 - g.bzi;
+  // pairs used by the DFT (reference code/dataprocess.f90:136-207)
+  int use[3];
+  if (p.nx != 1 && p.ny != 1) { use[0] = use[1] = use[2] = 1; }
+  else if (p.ny == 1 && p.nx != 1) { use[0] = 1; use[1] = 0; use[2] = 1; }
+  else if (p.nx == 1 && p.ny != 1) { use[0] = 0; use[1] = 1; use[2] = 1; }
+  else { use[0] = 0; use[1] = 0; use[2] = 1; }
+  const int64_t full[3] = {2LL * 6 * g.nyb * g.nzb, 2LL * 6 * g.nxb * g.nzb, 2LL * 6 * g.nxb * g.nyb};
+  B.npts_ref[0] = (p.nx != 1) ? full[0] : 1;
+  B.npts_ref[1] = (p.ny != 1) ? full[1] : 1;
+  B.npts_ref[2] = full[2];
+  int64_t s = 0;
+  B.active_mask = 0;
+  for (int j = 0; j < 3; ++j) {
+    g.npts[j] = use[j] ? full[j] : 0;
+    g.start[j] = s;
+    s += g.npts[j];
+    if (use[j]) B.active_mask |= 1 << j;
+  }
+  g.ntot = s;
+}
+
+template <typename T>
+int create_impl(iris_handle* h, const double* eps_inv, const double* fpmn, const double* fmmn, const double* fpmx,
+                const double* fmmx, const double* fw) {
+  const iris_params& p = h->p;
+  h->csize = sizeof(cplx<T>);
+  // x pitch: ix=1 on a 128-byte boundary
+  const int align = (int)(128 / sizeof(cplx<T>));
+  h->xoff = align - 1;
+  h->sx = ((h->xoff + p.nx + 2 + align - 1) / align) * align;
+  h->sy = p.ny + 2;
+  h->sxy = h->sx * h->sy;
+  h->ncell = h->sxy * (p.nz + 2);
+  int rc;
+  for (int i = 0; i < 6; ++i)
+    if ((rc = dalloc(h, &h->f[i], h->csize * (size_t)h->ncell))) return rc;
+  const int zl[2] = {p.pmlzi, p.nz - p.pmlzf + 1};
+  for (int k = 0; k < 2; ++k)
+    for (int c = 0; c < 3; ++c) {
+      if ((rc = dalloc(h, &h->d[k][c], h->csize * (size_t)h->sxy * zl[k]))) return rc;
+      if ((rc = dalloc(h, &h->b[k][c], h->csize * (size_t)h->sxy * zl[k]))) return rc;
+    }
+  if ((rc = upload_tables<T>(h, eps_inv, fpmn, fmmn, fpmx, fmmx))) return rc;
+  // DFT
+  if ((rc = dalloc(h, (void**)&h->fw, sizeof(double) * p.nof))) return rc;
+  HIP_TRY(h, hipMemcpyAsync(h->fw, fw, sizeof(double) * p.nof, hipMemcpyHostToDevice, h->stream));
+  if ((rc = dalloc(h, (void**)&h->ph, sizeof(double2) * (size_t)p.nof * kDftSlots))) return rc;
+  if ((rc = dalloc(h, (void**)&h->flux, sizeof(double) * 6 * (size_t)p.nof))) return rc;
+  for (int b = 0; b < p.nbox; ++b) {
+    DeviceBox& B = h->box[b];
+    box_geometry(p, b, B);
+    if ((rc = dalloc(h, (void**)&B.emg, sizeof(double2) * (size_t)std::max<int64_t>(B.g.ntot, 1) * p.nof))) return rc;
+    if ((rc = dalloc(h, &B.samp, h->csize * (size_t)std::max<int64_t>(B.g.ntot, 1) * kDftSlots))) return rc;
+  }
+  // launch shapes: x-rows of up to 64 (one wavefront), 256 threads per block
+  const int bx = std::min(64, pow2ceil(p.nx));
+  const int by = std::max(1, std::min(256 / bx, pow2ceil(p.ny)));
+  h->blk_yee = dim3(bx, by, 1);
+  const int gx = (p.nx + bx - 1) / bx, gy = (p.ny + by - 1) / by;
+  const int64_t nbxy = (int64_t)gx * gy;
+  int zc = (int)std::min<int64_t>(64, std::max<int64_t>(1, nbxy * p.nz / 4096));
+  h->zchunk = zc;
+  h->grd_yee = dim3(gx, gy, (p.nz + zc - 1) / zc);
+  const int tbx = std::min(64, pow2ceil(p.nx + 1));
+  const int tby = std::max(1, std::min(256 / tbx, pow2ceil(p.ny + 1)));
+  h->blk_tf = dim3(tbx, tby, 1);
+  h->grd_tf = dim3((p.nx + 1 + tbx - 1) / tbx, (p.ny + 1 + tby - 1) / tby, 1);
+  h->dtcq2 = ((p.dt * p.c0) / p.q) * ((p.dt * p.c0) / p.q);
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+void release(iris_handle* h) {
+  if (!h) return;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (void* p : h->allocs) (void)hipFree(p);
+  h->allocs.clear();
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  h->stream = nullptr;
+}
+
+// field index of (ix,iy,iz) in device layout
+inline int64_t dev_index(const iris_handle* h, int ix, int iy, int iz) {
+  return (int64_t)(ix + h->xoff) + h->sx * iy + h->sxy * iz;
+}
+
+}  // namespace
+
+extern "C" {
+
+int iris_create(const iris_params* p, const double* eps_inv, const double* fpzmn, const double* fmzmn,
+                const double* fpzmx, const double* fmzmx, const double* fw, iris_handle** out) {
+  if (!out) return fail(nullptr, IRIS_EINVAL, "out is NULL");
+  *out = nullptr;
+  int rc = check_params(p);
+  if (rc) return rc;
+  if (!eps_inv || !fpzmn || !fmzmn || !fpzmx || !fmzmx || !fw) return fail(nullptr, IRIS_EINVAL, "NULL array argument");
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev < 1) return fail(nullptr, IRIS_EDEVICE, "no HIP device available");
+  iris_handle* h = new iris_handle();
+  h->p = *p;
+  h->prec = p->precision;
+  (void)hipGetDevice(&h->device);
+  e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete h;
+    return fail(nullptr, IRIS_EDEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
+  }
+  rc = (h->prec == 8) ? create_impl<double>(h, eps_inv, fpzmn, fmzmn, fpzmx, fmzmx, fw)
+                      : create_impl<float>(h, eps_inv, fpzmn, fmzmn, fpzmx, fmzmx, fw);
+  if (rc) {
+    g_create_error = h->err;
+    release(h);
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return 0;
+}
+
+int iris_step(iris_handle* h, int32_t it_first, int32_t it_last) {
+  if (!h) return fail(nullptr, IRIS_EINVAL, "handle is NULL");
+  if (it_first != h->it + 1) return fail(h, IRIS_ESTATE, "it_first=%d but the next step is %d", it_first, h->it + 1);
+  if (it_last < it_first) return 0;
+  (void)hipSetDevice(h->device);
+  return h->prec == 8 ? run_steps<double>(h, it_first, it_last) : run_steps<float>(h, it_first, it_last);
+}
+
+int iris_sync(iris_handle* h) {
+  if (!h) return fail(nullptr, IRIS_EINVAL, "handle is NULL");
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int iris_flux(iris_handle* h, int32_t nobox, double* jabs) {
+  if (!h || !jabs) return fail(h, IRIS_EINVAL, "NULL argument");
+  if (nobox < 1 || nobox > h->p.nbox) return fail(h, IRIS_EINVAL, "box %d does not exist", nobox);
+  int rc = flush(h);
+  if (rc) return rc;
+  DeviceBox& B = h->box[nobox - 1];
+  const int nof = h->p.nof;
+  HIP_TRY(h, hipMemsetAsync(h->flux, 0, sizeof(double) * 6 * nof, h->stream));
+  hipLaunchKernelGGL(k_flux, dim3(6, nof), dim3(256), 0, h->stream, B.g, (const double2*)B.emg, nof, B.active_mask,
+                     h->flux);
+  HIP_TRY(h, hipGetLastError());
+  HIP_TRY(h, hipMemcpyAsync(jabs, h->flux, sizeof(double) * 6 * nof, hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  // Jabs = (eps0*q/(2.0*dt))*Jabs, reference code/dataprocess.f90:277
+  const double s = (h->p.eps0 * h->p.q) / (2.0 * h->p.dt);
+  for (int i = 0; i < 6 * nof; ++i) jabs[i] = s * jabs[i];
+  return 0;
+}
+
+int iris_dft_points(const iris_handle* h, int32_t nobox, int32_t pair, int64_t* npts) {
+  if (!h || !npts) return fail(nullptr, IRIS_EINVAL, "NULL argument");
+  if (nobox < 1 || nobox > h->p.nbox || pair < 1 || pair > 3) return fail(nullptr, IRIS_EINVAL, "bad box or pair");
+  *npts = h->box[nobox - 1].npts_ref[pair - 1];
+  return 0;
+}
+
+int iris_get_dft(iris_handle* h, int32_t nobox, int32_t pair, double* out) {
+  if (!h || !out) return fail(h, IRIS_EINVAL, "NULL argument");
+  if (nobox < 1 || nobox > h->p.nbox || pair < 1 || pair > 3) return fail(h, IRIS_EINVAL, "bad box or pair");
+  int rc = flush(h);
+  if (rc) return rc;
+  DeviceBox& B = h->box[nobox - 1];
+  const int j = pair - 1;
+  const size_t n = (size_t)B.npts_ref[j] * h->p.nof;
+  if (B.g.npts[j] == 0) {
+    std::memset(out, 0, sizeof(double2) * n);
+    return 0;
+  }
+  HIP_TRY(h, hipMemcpyAsync(out, B.emg + B.g.start[j] * h->p.nof, sizeof(double2) * n, hipMemcpyDeviceToHost,
+                            h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int iris_get_field(iris_handle* h, int32_t which, double* out) {
+  if (!h || !out || which < 0 || which > 1) return fail(h, IRIS_EINVAL, "bad argument");
+  const iris_params& p = h->p;
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  std::vector<char> buf(h->csize * (size_t)h->ncell);
+  const int64_t rsx = p.nx + 2, rsxy = rsx * (p.ny + 2);
+  for (int c = 0; c < 3; ++c) {
+    HIP_TRY(h, hipMemcpyAsync(buf.data(), h->f[3 * which + c], buf.size(), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    for (int z = 0; z <= p.nz + 1; ++z)
+      for (int y = 0; y <= p.ny + 1; ++y)
+        for (int x = 0; x <= p.nx + 1; ++x) {
+          const int64_t di = dev_index(h, x, y, z);
+          double* o = out + 2 * (c + 3 * (x + rsx * y + rsxy * z));
+          if (h->prec == 8) {
+            const double2 v = ((const double2*)buf.data())[di];
+            o[0] = v.x; o[1] = v.y;
+          } else {
+            const float2 v = ((const float2*)buf.data())[di];
+            o[0] = v.x; o[1] = v.y;
+          }
+        }
+  }
+  return 0;
+}
+
+int iris_set_field(iris_handle* h, int32_t which, const double* in) {
+  if (!h || !in || which < 0 || which > 1) return fail(h, IRIS_EINVAL, "bad argument");
+  const iris_params& p = h->p;
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  std::vector<char> buf(h->csize * (size_t)h->ncell, 0);
+  const int64_t rsx = p.nx + 2, rsxy = rsx * (p.ny + 2);
+  for (int c = 0; c < 3; ++c) {
+    for (int z = 0; z <= p.nz + 1; ++z)
+      for (int y = 0; y <= p.ny + 1; ++y)
+        for (int x = 0; x <= p.nx + 1; ++x) {
+          const int64_t di = dev_index(h, x, y, z);
+          const double* v = in + 2 * (c + 3 * (x + rsx * y + rsxy * z));
+          if (h->prec == 8) ((double2*)buf.data())[di] = make_double2(v[0], v[1]);
+          else ((float2*)buf.data())[di] = make_float2((float)v[0], (float)v[1]);
+        }
+    HIP_TRY(h, hipMemcpyAsync(h->f[3 * which + c], buf.data(), buf.size(), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+  }
+  return 0;
+}
+
+int iris_current_step(const iris_handle* h, int32_t* it) {
+  if (!h || !it) return IRIS_EINVAL;
+  *it = h->it;
+  return 0;
+}
+
+int iris_destroy(iris_handle* h) {
+  if (!h) return 0;
+  release(h);
+  delete h;
+  return 0;
+}
+
+const char* iris_last_error(const iris_handle* h) { return h ? h->err.c_str() : g_create_error.c_str(); }
+
+const char* iris_version(void) { return "iris-mi355x 0.1 (gfx950)"; }
+
+}  // extern "C"

@@ -1,0 +1,192 @@
+"""ctypes binding of the engine's C ABI (include/iris.h).
+
+The engine library ``lib/libiris.so`` must have been built (``build.py``);
+loading fails loudly otherwise -- there is no CPU fallback.
+"""
+import ctypes as ct
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_DIR = os.path.join(PKG_DIR, "lib")
+
+IRIS_ABI_VERSION = 1
+
+
+class IrisParams(ct.Structure):
+    """Mirror of ``iris_params`` (include/iris.h)."""
+    _fields_ = [
+        ("abi_version", ct.c_int32), ("precision", ct.c_int32),
+        ("nx", ct.c_int32), ("ny", ct.c_int32), ("nz", ct.c_int32),
+        ("pmlzi", ct.c_int32), ("pmlzf", ct.c_int32),
+        ("hi", ct.c_int32), ("hf", ct.c_int32),
+        ("i_pol", ct.c_int32), ("em_source", ct.c_int32), ("plane_wave", ct.c_int32),
+        ("cosine", ct.c_int32), ("iz_min", ct.c_int32), ("iz_max", ct.c_int32),
+        ("ixo", ct.c_int32), ("nt", ct.c_int32), ("to_fourier", ct.c_int32),
+        ("sample_fourier", ct.c_int32), ("no_downloads", ct.c_int32), ("nof", ct.c_int32),
+        ("nbox", ct.c_int32), ("box", (ct.c_int32 * 6) * 2),
+        ("q", ct.c_double), ("dt", ct.c_double), ("c0", ct.c_double), ("eps0", ct.c_double),
+        ("mu0", ct.c_double), ("pi", ct.c_double), ("f_abohr", ct.c_double),
+        ("eps", ct.c_double * 3), ("eps_in", ct.c_double * 3),
+        ("epsa_inv", ((ct.c_double * 3) * 3) * 2),
+        ("eamp", ct.c_double), ("wlpump", ct.c_double), ("gpump", ct.c_double), ("zopump", ct.c_double),
+        ("ratio_angle", ct.c_double), ("ux", ct.c_double), ("uz", ct.c_double), ("win", ct.c_double),
+        ("ak", (ct.c_double * 2) * 3),
+    ]
+
+    def copy(self):
+        c = IrisParams()
+        ct.pointer(c)[0] = self
+        return c
+
+    def as_dict(self):
+        out = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            if hasattr(v, "__len__"):
+                v = np.array(v).tolist()
+            out[name] = v
+        return out
+
+
+class IrisError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def _p(a):
+    return a.ctypes.data_as(ct.c_void_p)
+
+
+def load_library(path=None):
+    """Load libiris.so once; raises if it is missing (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = path or os.path.join(LIB_DIR, "libiris.so")
+    if not os.path.exists(path):
+        raise IrisError(f"HIP engine library not built: {path} (run irisfdtd-academic_amd/build.py)")
+    lib = ct.CDLL(path)
+    P = ct.POINTER(IrisParams)
+    H = ct.c_void_p
+    lib.iris_create.argtypes = [P, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p,
+                                ct.POINTER(H)]
+    lib.iris_create.restype = ct.c_int
+    lib.iris_step.argtypes = [H, ct.c_int32, ct.c_int32]
+    lib.iris_step.restype = ct.c_int
+    lib.iris_sync.argtypes = [H]
+    lib.iris_sync.restype = ct.c_int
+    lib.iris_flux.argtypes = [H, ct.c_int32, ct.c_void_p]
+    lib.iris_flux.restype = ct.c_int
+    lib.iris_dft_points.argtypes = [H, ct.c_int32, ct.c_int32, ct.POINTER(ct.c_int64)]
+    lib.iris_dft_points.restype = ct.c_int
+    lib.iris_get_dft.argtypes = [H, ct.c_int32, ct.c_int32, ct.c_void_p]
+    lib.iris_get_dft.restype = ct.c_int
+    lib.iris_get_field.argtypes = [H, ct.c_int32, ct.c_void_p]
+    lib.iris_get_field.restype = ct.c_int
+    lib.iris_set_field.argtypes = [H, ct.c_int32, ct.c_void_p]
+    lib.iris_set_field.restype = ct.c_int
+    lib.iris_current_step.argtypes = [H, ct.POINTER(ct.c_int32)]
+    lib.iris_current_step.restype = ct.c_int
+    lib.iris_destroy.argtypes = [H]
+    lib.iris_destroy.restype = ct.c_int
+    lib.iris_last_error.argtypes = [H]
+    lib.iris_last_error.restype = ct.c_char_p
+    lib.iris_version.argtypes = []
+    lib.iris_version.restype = ct.c_char_p
+    _lib = lib
+    return lib
+
+
+EXPORTED = ["iris_create", "iris_step", "iris_sync", "iris_flux", "iris_dft_points", "iris_get_dft",
+            "iris_get_field", "iris_set_field", "iris_current_step", "iris_destroy", "iris_last_error",
+            "iris_version"]
+
+
+class Engine:
+    """One simulation on the GPU, driven through the C ABI.
+
+    ``setup`` is a :class:`pyiris.host.Setup` (parameters and arrays derived
+    from an inputFDTD.dat by the Fortran host library).
+    """
+
+    def __init__(self, setup, precision=None):
+        self.lib = load_library()
+        p = setup.params.copy()
+        if precision is not None:
+            p.precision = int(precision)
+        self.params = p
+        self.setup = setup
+        self._h = ct.c_void_p()
+        rc = self.lib.iris_create(ct.byref(p), _p(setup.eps_inv), _p(setup.fpzmn), _p(setup.fmzmn),
+                                  _p(setup.fpzmx), _p(setup.fmzmx), _p(setup.fw), ct.byref(self._h))
+        if rc != 0:
+            raise IrisError(f"iris_create failed ({rc}): {self.lib.iris_last_error(None).decode()}")
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise IrisError(f"{what} failed ({rc}): {self.lib.iris_last_error(self._h).decode()}")
+
+    @property
+    def it(self):
+        v = ct.c_int32()
+        self._check(self.lib.iris_current_step(self._h, ct.byref(v)), "iris_current_step")
+        return v.value
+
+    def step(self, it_first, it_last):
+        self._check(self.lib.iris_step(self._h, int(it_first), int(it_last)), "iris_step")
+
+    def run_to(self, it_last):
+        self.step(self.it + 1, it_last)
+
+    def sync(self):
+        self._check(self.lib.iris_sync(self._h), "iris_sync")
+
+    def flux(self, nobox=1):
+        """Jabs as (nof, 6): column j-1 is J_j (reference Jabs(j,iw))."""
+        out = np.zeros(6 * self.params.nof, dtype=np.float64)
+        self._check(self.lib.iris_flux(self._h, int(nobox), _p(out)), "iris_flux")
+        return out.reshape(self.params.nof, 6)
+
+    def dft_points(self, nobox, pair):
+        n = ct.c_int64()
+        self._check(self.lib.iris_dft_points(self._h, int(nobox), int(pair), ct.byref(n)), "iris_dft_points")
+        return n.value
+
+    def dft(self, nobox, pair):
+        """emg(pair)%fdata as complex (nof, npts)."""
+        n = self.dft_points(nobox, pair)
+        out = np.zeros(2 * n * self.params.nof, dtype=np.float64)
+        self._check(self.lib.iris_get_dft(self._h, int(nobox), int(pair), _p(out)), "iris_get_dft")
+        return out.view(np.complex128).reshape(self.params.nof, n)
+
+    def field(self, which):
+        """E (0) or H (1) as complex (nz+2, ny+2, nx+2, 3) (C order of the Fortran (3,0:nx+1,0:ny+1,0:nz+1))."""
+        p = self.params
+        out = np.zeros(2 * 3 * (p.nx + 2) * (p.ny + 2) * (p.nz + 2), dtype=np.float64)
+        self._check(self.lib.iris_get_field(self._h, int(which), _p(out)), "iris_get_field")
+        return out.view(np.complex128).reshape(p.nz + 2, p.ny + 2, p.nx + 2, 3)
+
+    def set_field(self, which, arr):
+        a = np.ascontiguousarray(arr, dtype=np.complex128)
+        self._check(self.lib.iris_set_field(self._h, int(which), _p(a.view(np.float64))), "iris_set_field")
+
+    def close(self):
+        if self._h:
+            self.lib.iris_destroy(self._h)
+            self._h = ct.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
