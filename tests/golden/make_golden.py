@@ -1,0 +1,94 @@
+#!/usr/bin/env python3
+"""Write the golden fixtures of tests/golden/ by running the reference binary.
+
+The reference (IrisFDTD-Academic's Fortran, built unmodified in numerics by
+oracle/patch_ref.py into oracle/_ref/irisfdtd_ref) is run on each input below;
+its output files (<box>_scatt<j>.dat blocks) and the opt-in dump of its DFT
+accumulators are stored as <case>.npz next to the input text <case>.dat.
+
+Inputs: example_1 and example_2 are the reference's own example inputs
+(example_*/inputFDTD.dat), example_2 shortened to 1000 steps; the others are
+small grids chosen to cover the paths of the loop (3D with both
+polarisations, real source, two Fourier boxes, 2D slit arrays along x and y,
+a substrate different from the cover, config-1 geometry).
+
+Usage: python tests/golden/make_golden.py [case ...]   (needs the built
+reference binary; run `make -C oracle ref` first)
+"""
+import os
+import re
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, os.path.join(ROOT, "irisfdtd-academic_amd"))
+
+import pyoracle  # noqa: E402
+from pyiris import inputs  # noqa: E402
+
+REF_ROOT = "/root/reference"
+
+
+def example(name, itmax_fs=None):
+    with open(os.path.join(REF_ROOT, name, "inputFDTD.dat"), encoding="latin-1") as f:
+        text = f.read()
+    if itmax_fs is not None:
+        text = re.sub(r"^200\.0(\s+itmax)", f"{itmax_fs!r}\\1", text, count=1, flags=re.M)
+    return text
+
+
+CASES = {
+    # name: (input text factory, pairs whose emg is stored)
+    "example_1": (lambda: example("example_1"), [3]),
+    "example_2_short": (lambda: example("example_2", itmax_fs=7.7033), []),
+    "cfg1_short": (lambda: inputs.render(nt=400, nof=16, dlambda=40.0, no_downloads=4), [3]),
+    "small3d_ipol2": (lambda: inputs.render(
+        ixmax=8, iymax=6, izmax=96, pmlzi=10, pmlzf=86, hi=40, dh=12, dx=3, eps_bbox=2.25, tfsf_o=20,
+        box=(2, 7, 1, 6, 15, 80), nof=4, nt=300, no_downloads=3, lno=500.0, dlambda=100.0, eps3=1.7), [2, 3]),
+    "small3d_ipol1_cos_2box": (lambda: inputs.render(
+        i_pol=1, ixmax=7, iymax=5, izmax=64, pmlzi=8, pmlzf=56, hi=30, dh=5, dx=2, eps_bbox=4.0, tfsf_o=14,
+        box=(1, 7, 2, 4, 10, 50), mbox=(2, 5, 1, 5, 12, 45), nof=3, nt=200, no_downloads=2, cosine=1), [1, 3]),
+    "slit2d_x": (lambda: inputs.render(
+        ixmax=20, iymax=1, izmax=80, pmlzi=10, pmlzf=70, hi=35, dh=6, dx=4, eps_bbox=9.0, tfsf_o=16,
+        box=(1, 1, 1, 1, 12, 60), nof=5, nt=250, no_downloads=2), [1, 3]),
+    "slit2d_y": (lambda: inputs.render(
+        ixmax=1, iymax=12, izmax=70, pmlzi=10, pmlzf=60, hi=30, dh=6, dx=0, eps_bbox=6.0, eps1=2.25, tfsf_o=16,
+        box=(1, 1, 2, 11, 12, 55), nof=5, nt=250, no_downloads=2), [2, 3]),
+    "substrate3d": (lambda: inputs.render(
+        ixmax=6, iymax=6, izmax=72, pmlzi=10, pmlzf=62, hi=30, dh=8, dx=2, eps_bbox=2.5, tfsf_o=16, eps3=2.0,
+        box=(1, 6, 1, 6, 12, 55), nof=6, nt=240, no_downloads=4, lno=450.0, dlambda=90.0), [3]),
+}
+
+
+def make(name):
+    factory, pairs = CASES[name]
+    text = factory()
+    with open(os.path.join(HERE, name + ".dat"), "w", encoding="latin-1") as f:
+        f.write(text)
+    with tempfile.TemporaryDirectory(prefix="iris_golden_") as wd:
+        pyoracle.run_reference(text, os.path.join(wd, "run"), dump_emg=True)
+        run = os.path.join(wd, "run")
+        out = {}
+        for b in (1, 2):
+            if not os.path.exists(os.path.join(run, f"{b}_scatt1.dat")):
+                continue
+            blocks = [pyoracle.read_scatt(os.path.join(run, f"{b}_scatt{j}.dat")) for j in range(1, 7)]
+            out[f"it_b{b}"] = np.array([it for it, _ in blocks[0]], dtype=np.int32)
+            out[f"jabs_b{b}"] = np.stack([np.stack([v for _, v in blk], axis=0) for blk in blocks], axis=-1)
+            tot = pyoracle.read_scatt(os.path.join(run, f"{b}_scatt.dat"))
+            out[f"jtot_b{b}"] = np.stack([v for _, v in tot], axis=0)
+            for pair in pairs:
+                out[f"emg_b{b}_p{pair}"] = pyoracle.read_emg_dump(os.path.join(run, f"emg{b}_{pair}.bin"))
+        spec = np.loadtxt(os.path.join(run, "dielectric_layer_spectra.dat"))
+        out["film_spectrum"] = spec
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+    print(name, {k: v.shape for k, v in out.items()})
+
+
+if __name__ == "__main__":
+    for n in (sys.argv[1:] or list(CASES)):
+        make(n)

@@ -1,0 +1,167 @@
+"""GPU parity: the HIP engine through its C ABI against the oracle.
+
+Tolerances (stated per north-star):
+  FP64 fields path  vs reference golden outputs: Jabs normwise relative per
+                    face <= 1e-10, DFT fields per wavelength <= 1e-10;
+                    vs the oracle's fields after N steps: <= 1e-12.
+  FP32 fields path  (FP64 DFT accumulators): normalised T and R within 1e-5
+                    of the FP64 oracle.
+Differences at FP64 come only from ulp-level differences of exp/sincos
+(device vs libm) in the TF/SF source and DFT phases, and from the order of
+the flux reduction.
+"""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import (GOLDEN_CASES, PKG, dft_rel_err, face_rel_err, golden_path, load_golden,
+                      run_with_cadence, write_input)
+import pyoracle
+from pyiris import Engine, IrisError, inputs, load_input
+
+pytestmark = pytest.mark.gpu
+
+
+def _announce(msg):
+    print(f"[gpu-test] {msg}", flush=True)
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_fp64_matches_reference_golden(name):
+    _announce(f"golden {name} fp64")
+    g = load_golden(name)
+    s = load_input(golden_path(name, "dat"))
+    with Engine(s, precision=8) as eng:
+        res = run_with_cadence(eng, s.params, s.params.nbox)
+        for b in range(1, s.params.nbox + 1):
+            assert [it for it, _ in res[b]] == g[f"it_b{b}"].tolist()
+            jab = np.stack([j for _, j in res[b]])
+            err = face_rel_err(jab, g[f"jabs_b{b}"])
+            assert err <= 1e-10, f"{name} box {b}: Jabs rel err {err:.3e}"
+            for key in g:
+                if key.startswith(f"emg_b{b}_p"):
+                    pair = int(key[-1])
+                    e = dft_rel_err(eng.dft(b, pair), g[key])
+                    assert e <= 1e-10, f"{name} {key}: rel err {e:.3e}"
+
+
+def _small_setup(tmp_path, **kw):
+    base = dict(ixmax=9, iymax=7, izmax=80, pmlzi=10, pmlzf=70, hi=35, dh=8, dx=3, eps_bbox=3.0, tfsf_o=20,
+                eps3=1.5, box=(2, 8, 1, 7, 14, 64), nof=5, nt=150, no_downloads=50, lno=450.0, dlambda=100.0)
+    base.update(kw)
+    return load_input(write_input(tmp_path, inputs.render(**base)))
+
+
+@pytest.mark.parametrize("i_pol", [1, 2])
+def test_fp64_fields_match_oracle(tmp_path, i_pol):
+    """Interior fields, the live ghost planes and the DFT after 97 steps."""
+    _announce(f"fields vs oracle i_pol={i_pol}")
+    s = _small_setup(tmp_path, i_pol=i_pol)
+    o = pyoracle.Oracle(s)
+    o.run_to(97)
+    with Engine(s, precision=8) as eng:
+        eng.run_to(97)
+        for w in (0, 1):
+            a, b = eng.field(w), o.field(w)
+            inner = (slice(1, -1), slice(1, -1), slice(1, -1))
+            rel = np.linalg.norm(a[inner] - b[inner]) / np.linalg.norm(b[inner])
+            assert rel <= 1e-12, (w, rel)
+        # H ghost planes read by the next E update (x=0, y=0, z=0)
+        a, b = eng.field(1), o.field(1)
+        for sl in [(slice(1, -1), slice(1, -1), 0), (slice(1, -1), 0, slice(1, -1)), (0, slice(1, -1), slice(1, -1))]:
+            rel = np.linalg.norm(a[sl] - b[sl]) / max(np.linalg.norm(b[sl]), 1e-300)
+            assert rel <= 1e-12, (sl, rel)
+        for pair in (1, 2, 3):
+            assert dft_rel_err(eng.dft(1, pair), o.dft(1, pair)) <= 1e-12
+        np.testing.assert_allclose(eng.flux(1), o.flux(1), rtol=1e-10, atol=1e-12 * np.abs(o.flux(1)).max())
+
+
+def test_fp64_step_chunking_is_invisible(tmp_path):
+    """Stepping 1..n in one call or in uneven chunks gives identical state
+    (the DFT sample ring is flushed at arbitrary points)."""
+    _announce("chunking")
+    s = _small_setup(tmp_path, nt=60)
+    with Engine(s, precision=8) as a, Engine(s, precision=8) as b:
+        a.run_to(60)
+        for stop in (1, 2, 17, 18, 35, 59, 60):
+            b.run_to(stop)
+            b.flux(1)
+        for w in (0, 1):
+            assert np.array_equal(a.field(w), b.field(w))
+        for pair in (1, 2, 3):
+            assert np.array_equal(a.dft(1, pair), b.dft(1, pair))
+
+
+def test_fp32_tr_within_1e5_of_fp64_oracle(tmp_path):
+    """FP32 fields with FP64 DFT accumulators: normalised T(lambda), R(lambda)
+    against the FP64 oracle on a periodic slit array with a full box."""
+    _announce("fp32 T/R")
+    s = load_input(write_input(tmp_path, inputs.render(
+        ixmax=24, iymax=8, izmax=400, pmlzi=20, pmlzf=380, hi=190, dh=20, dx=6, eps_bbox=4.0, tfsf_o=30,
+        box=(1, 24, 1, 8, 25, 370), nof=10, nt=3000, no_downloads=3000, lno=450.0, dlambda=25.0)))
+    o = pyoracle.Oracle(s)
+    o.run_to(s.params.nt)
+    Jo = o.flux(1)
+    with Engine(s, precision=4) as eng:
+        eng.run_to(s.params.nt)
+        Jg = eng.flux(1)
+    den_o = Jo[:, 4] + Jo[:, 5]
+    den_g = Jg[:, 4] + Jg[:, 5]
+    dR = np.abs(Jg[:, 4] / den_g - Jo[:, 4] / den_o).max()
+    dT = np.abs(Jg[:, 5] / den_g - Jo[:, 5] / den_o).max()
+    _announce(f"fp32 max |dR|={dR:.3e} |dT|={dT:.3e}")
+    assert dR <= 1e-5 and dT <= 1e-5
+
+
+def test_linearity_exact_at_full_size():
+    """2x the source amplitude gives exactly 2x the fields, at the
+    512x512x1024 roofline grid (FP32), after a few steps."""
+    _announce("full-size linearity fp32")
+    kw = dict(ixmax=512, iymax=512, izmax=1024, pmlzi=20, pmlzf=1005, hi=492, dh=40, dx=20, eps_bbox=4.0,
+              tfsf_o=30, box=(1, 512, 1, 512, 25, 1000), nof=4, nt=20, no_downloads=20)
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        s1 = load_input(write_input(d, inputs.render(epump=1.0, **kw), "a.dat"))
+        s2 = load_input(write_input(d, inputs.render(epump=2.0, **kw), "b.dat"))
+    with Engine(s1, precision=4) as a:
+        a.run_to(20)
+        fa = a.field(0)[20:60]
+        ja = a.flux(1)
+    with Engine(s2, precision=4) as b:
+        b.run_to(20)
+        fb = b.field(0)[20:60]
+        jb = b.flux(1)
+    assert np.abs(fa).max() > 0
+    assert np.array_equal(fb, 2.0 * fa)
+    assert np.all(np.isfinite(ja))
+    np.testing.assert_allclose(jb, 4.0 * ja, rtol=1e-12, atol=1e-30)
+
+
+def test_step_out_of_order_is_an_error(tmp_path):
+    s = _small_setup(tmp_path, nt=10)
+    with Engine(s, precision=8) as eng:
+        eng.run_to(3)
+        with pytest.raises(IrisError, match="next step is 4"):
+            eng.step(7, 9)
+
+
+def test_fortran_program_end_to_end(tmp_path):
+    """bin/irisfdtd (Fortran host + HIP engine) on the reference's
+    example_1 input writes the reference's output files."""
+    _announce("fortran program example_1")
+    exe = os.path.join(PKG, "bin", "irisfdtd")
+    assert os.path.exists(exe), "bin/irisfdtd not built"
+    shutil.copy(golden_path("example_1", "dat"), tmp_path / "inputFDTD.dat")
+    subprocess.run([exe], cwd=tmp_path, check=True, timeout=300, stdout=subprocess.DEVNULL)
+    g = load_golden("example_1")
+    blocks = [pyoracle.read_scatt(str(tmp_path / f"1_scatt{j}.dat")) for j in range(1, 7)]
+    assert [it for it, _ in blocks[0]] == g["it_b1"].tolist()
+    jab = np.stack([np.stack([v for _, v in blk]) for blk in blocks], axis=-1)
+    assert face_rel_err(jab, g["jabs_b1"]) <= 1e-10
+    tot = pyoracle.read_scatt(str(tmp_path / "1_scatt.dat"))
+    np.testing.assert_allclose(np.stack([v for _, v in tot]), g["jtot_b1"], rtol=1e-9, atol=1e-20)
+    spec = np.loadtxt(tmp_path / "dielectric_layer_spectra.dat")
+    np.testing.assert_allclose(spec, g["film_spectrum"], rtol=1e-14)
