@@ -1,0 +1,238 @@
+#!/usr/bin/env python3
+"""Throughput of the MI355X FDTD engine on the roofline-scale workload.
+
+Workload: the slit-array film on a 512 x 512 x 1024 grid (dx = 5 nm),
+periodic x/y, 20-cell z-UPML at each end, eps_r = 4.0 film 200 nm (40 cells)
+thick cut by a 100 nm (20-cell) slit along y, normal-incidence TF/SF
+Gaussian pulse covering 400-1000 nm, on-the-fly DFT at 100 wavelengths on a
+full-lateral box (z = 25..1000), output cadence every nt/10 steps (nt =
+20000).  A step is one pass of the time loop (E, H, TF/SF, DFT sample, DFT
+fold every 16 samples, face fluxes at the cadence) over the whole grid.
+
+Prints one JSON line.  value = FP32-field Gcell-updates/s over the timed
+steps (inputs resident in HBM); the FP64-field rate, per-kernel HIP-event
+timings, the E-pass HBM roofline and a CPU baseline (the reference binary on
+a bounded sample) ride along.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-fp64] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import platform
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "irisfdtd-academic_amd")
+sys.path.insert(0, PKG)
+
+METRIC = "Gcell-updates/s & % HBM roofline, FP32 vs FP64 plain; adjoint/forward time"
+HBM_PEAK_GBS = 8000.0
+NX = NY = 512
+NZ = 1024
+NT_CONFIG = 20000
+WORKLOAD = dict(description="slit-array film 512x512x1024 (roofline workload)",
+                ixmax=NX, iymax=NY, izmax=NZ, pmlzi=20, pmlzf=1005, hi=492, dh=40, dx=20,
+                eps1=1.0, eps_bbox=4.0, eps3=1.0, tfsf_o=30, wlpump=700.0, pump_dt=4.0, def_pulse=4,
+                box=(1, NX, 1, NY, 25, 1000), nof=100, dlambda=600.0 / 99.0, lno=400.0,
+                nt=NT_CONFIG, no_downloads=10, to_fourier=0, sample_fourier=1)
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_setup(precision, **over):
+    from pyiris import inputs, load_input
+    kw = dict(WORKLOAD)
+    kw.update(over)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "inputFDTD.dat")
+        with open(path, "w") as f:
+            f.write(inputs.render(**kw))
+        return load_input(path, precision=precision)
+
+
+def algorithmic_bytes(p, precision):
+    """HBM bytes per launch that the E and H passes must move (compulsory
+    traffic): per cell read E, read H, write own field (3 complex each);
+    per UPML cell read+write the 3 auxiliary complex values; per structure
+    cell read eps_inv (real).  c = bytes per complex value."""
+    c = 8 if precision == 4 else 16
+    r = c // 2
+    ncell = p.nx * p.ny * p.nz
+    npml = p.pmlzi + (p.nz - p.pmlzf + 1)
+    nxy = p.nx * p.ny
+    e_pass = 9 * c * ncell + 6 * c * nxy * npml + r * nxy * (p.hf - p.hi + 1)
+    h_pass = 9 * c * ncell + 6 * c * nxy * npml
+    return e_pass, h_pass
+
+
+def run_gpu(precision, steps, warmup):
+    from pyiris import Engine
+    s = make_setup(precision)
+    p = s.params
+    log(f"precision {precision}: creating engine for {p.nx}x{p.ny}x{p.nz}, nof={p.nof}")
+    eng = Engine(s, precision=precision)
+    nos = 1
+
+    def advance(it_from, it_to, nos):
+        it = it_from
+        while it <= it_to:
+            cand = p.to_fourier + p.no_downloads * nos
+            stop = min(cand, it_to) if cand >= it else it_to
+            eng.step(it, stop)
+            if stop >= p.to_fourier and (stop == cand or stop == p.nt):
+                eng.flux(1)
+                nos += 1
+            it = stop + 1
+        return nos
+
+    nos = advance(1, warmup, nos)
+    eng.sync()
+    eng.profile(True)
+    eng.profile_read(reset=True)
+    log(f"precision {precision}: timing {steps} steps")
+    t0 = time.perf_counter()
+    nos = advance(warmup + 1, warmup + steps, nos)
+    eng.sync()
+    t1 = time.perf_counter()
+    prof = eng.profile_read(reset=True)
+    eng.profile(False)
+    eng.close()
+    secs = t1 - t0
+    gcells = p.nx * p.ny * p.nz * steps / secs / 1e9
+    eb, hb = algorithmic_bytes(p, precision)
+    kern = {}
+    for k, (ms, n) in prof.items():
+        if n:
+            kern[k] = {"avg_us": 1e3 * ms / n, "launches": n, "total_ms": ms}
+    e_avg = kern["epass"]["avg_us"] * 1e-6
+    h_avg = kern["hpass"]["avg_us"] * 1e-6
+    return dict(gcells=gcells, ms_per_step=1e3 * secs / steps, secs=secs, kernels=kern,
+                epass_gbs=eb / e_avg / 1e9, hpass_gbs=hb / h_avg / 1e9, epass_bytes=eb, hpass_bytes=hb)
+
+
+def measured_traffic(kernel, precision):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary, if any."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_fp{32 if precision == 4 else 64}_pmc.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d.get("hbm_bytes_per_launch")
+
+
+def cpu_baseline():
+    """The reference binary (oracle/_ref/irisfdtd_ref, serial Fortran) on a
+    bounded sample of the workload: same z-structure, source and slit, 128 x
+    128 lateral cells, stepping only (nof=1 on a 1x1 box, as the reference's
+    stepping-only rate is defined), two runs of S1 and S2 steps differenced to
+    remove set-up.  Pinned to one core."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    from pyiris import inputs
+    if not pyoracle.ref_available():
+        return None
+    lat = 128
+    s1, s2 = 3, 15
+    kw = dict(WORKLOAD)
+    kw.update(ixmax=lat, iymax=lat, box=(1, 1, 1, 1, 25, 1000), nof=1, no_downloads=1)
+    times = {}
+    base = tempfile.mkdtemp(prefix="iris_cpu_")
+    try:
+        for n in (s1, s2):
+            kw["nt"] = n
+            text = inputs.render(**kw)
+            t0 = time.perf_counter()
+            pyoracle.run_reference(text, os.path.join(base, f"r{n}"), timeout=600, pin_core=0)
+            times[n] = time.perf_counter() - t0
+    finally:
+        shutil.rmtree(base, ignore_errors=True)
+    dt = times[s2] - times[s1]
+    rate = lat * lat * NZ * (s2 - s1) / dt / 1e9
+    cpu = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": rate, "unit": "Gcell-updates/s", "cores": 1, "kind": "reference",
+            "sample": f"reference Fortran (amdflang -O2) on {lat}x{lat}x{NZ} of the same z-structure, stepping "
+                      f"only (nof=1, 1x1 box), ({s2}-{s1}) steps by differencing two runs "
+                      f"({times[s2]:.1f}s - {times[s1]:.1f}s), taskset core 0; host {cpu}, "
+                      f"{os.cpu_count()} cpus visible"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--fp64-steps", type=int, default=None)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this build "
+              "(z-slab sharding is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    r32 = run_gpu(4, a.steps, a.warmup)
+    log(f"fp32: {r32['gcells']:.2f} Gcell/s, {r32['ms_per_step']:.3f} ms/step, E-pass {r32['epass_gbs']:.0f} GB/s")
+    r64 = None
+    if not a.no_fp64:
+        r64 = run_gpu(8, a.fp64_steps or max(a.steps // 2, 1), a.warmup)
+        log(f"fp64: {r64['gcells']:.2f} Gcell/s, {r64['ms_per_step']:.3f} ms/step, E-pass {r64['epass_gbs']:.0f} GB/s")
+    cpu = None if a.no_cpu else cpu_baseline()
+    if cpu:
+        log(f"cpu baseline: {cpu['value'] * 1e3:.2f} Mcell/s")
+
+    k = r32["kernels"]
+    dom = "epass" if k["epass"]["total_ms"] >= k["hpass"]["total_ms"] else "hpass"
+    achieved = r32[f"{dom}_gbs"]
+    traffic = measured_traffic(dom, 4)
+    line = {
+        "metric": METRIC,
+        "value": round(r32["gcells"], 4),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(r32["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "slit-array film 512x512x1024, 20-cell z-UPML, TF/SF Gaussian 400-1000 nm, "
+                               "DFT at 100 wavelengths on a full-lateral box, cadence nt/10 (nt=20000)",
+                   "grid": [NX, NY, NZ], "nof": 100, "fields": "fp32", "dft_accumulators": "fp64"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "algorithmic_bytes_per_launch": r32[f"{dom}_bytes"],
+                     "avg_launch_us": round(k[dom]["avg_us"], 2)},
+        "cpu_baseline": cpu,
+        "kernels_fp32_us": {n: round(v["avg_us"], 2) for n, v in k.items()},
+        "hbm_frac_fp32_step": round((r32["epass_bytes"] + r32["hpass_bytes"]) / (r32["ms_per_step"] * 1e-3)
+                                    / 1e9 / HBM_PEAK_GBS, 4),
+    }
+    if r64:
+        line["fp64"] = {"value": round(r64["gcells"], 4), "ms_per_step": round(r64["ms_per_step"], 4),
+                        "epass_gbs": round(r64["epass_gbs"], 1), "hpass_gbs": round(r64["hpass_gbs"], 1),
+                        "steps": a.fp64_steps or max(a.steps // 2, 1),
+                        "kernels_us": {n: round(v["avg_us"], 2) for n, v in r64["kernels"].items()}}
+    if cpu:
+        line["speedup_vs_cpu"] = round(r32["gcells"] / cpu["value"], 1)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

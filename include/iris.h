@@ -143,6 +143,25 @@ int iris_destroy(iris_handle* h);
  * iris_create when h is NULL).  Never NULL. */
 const char* iris_last_error(const iris_handle* h);
 
+/* Kernel classes timed by the profiling hook. */
+#define IRIS_K_EPASS 0      /* E update + UPML + max-side wraps          */
+#define IRIS_K_HPASS 1      /* H update + UPML + min-side wraps          */
+#define IRIS_K_TFSF 2       /* TF/SF incident field and corrections      */
+#define IRIS_K_DFT_SAMPLE 3 /* copy of the box faces into the sample ring */
+#define IRIS_K_DFT_ACCUM 4  /* fold of buffered samples into emg         */
+#define IRIS_K_FLUX 5       /* face-flux reduction                       */
+#define IRIS_NKERNEL 6
+
+/* Enable (1) or disable (0) per-kernel timing with HIP events recorded
+ * around every launch on the handle's stream.  No counterpart in the
+ * reference (its only timing is a progress print, code/driver.f90:340-353). */
+int iris_profile(iris_handle* h, int32_t enable);
+
+/* Accumulated device time in ms and launch counts per kernel class since the
+ * last reset (arrays of IRIS_NKERNEL); waits for pending work.  reset != 0
+ * clears the totals after reading. */
+int iris_profile_read(iris_handle* h, double* ms, int64_t* launches, int32_t reset);
+
 /* Library version string. */
 const char* iris_version(void);
 

@@ -70,6 +70,13 @@ struct iris_handle {
   dim3 blk_yee, grd_yee, blk_tf, grd_tf;
   double dtcq2 = 0;
   std::vector<void*> allocs;
+  // profiling: event pairs around launches, drained in batches
+  struct Pending { int k; hipEvent_t a, b; };
+  bool prof = false;
+  std::vector<Pending> pend;
+  std::vector<hipEvent_t> pool;
+  double prof_ms[IRIS_NKERNEL] = {};
+  int64_t prof_n[IRIS_NKERNEL] = {};
 };
 
 namespace {
@@ -104,6 +111,49 @@ int dalloc(iris_handle* h, void** p, size_t bytes) {
 }
 
 int pow2ceil(int v) { int p = 1; while (p < v) p <<= 1; return p; }
+
+// ---- profiling -------------------------------------------------------------
+hipEvent_t take_event(iris_handle* h) {
+  if (!h->pool.empty()) {
+    hipEvent_t e = h->pool.back();
+    h->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+void prof_drain(iris_handle* h) {
+  for (auto& p : h->pend) {
+    float ms = 0.f;
+    if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      h->prof_ms[p.k] += ms;
+      h->prof_n[p.k] += 1;
+    }
+    h->pool.push_back(p.a);
+    h->pool.push_back(p.b);
+  }
+  h->pend.clear();
+}
+
+// Brackets one kernel launch with events when profiling is on.
+struct ProfScope {
+  iris_handle* h;
+  int k;
+  hipEvent_t a = nullptr;
+  ProfScope(iris_handle* h_, int k_) : h(h_), k(k_) {
+    if (h->prof && (a = take_event(h))) (void)hipEventRecord(a, h->stream);
+  }
+  ~ProfScope() {
+    if (!a) return;
+    hipEvent_t b = take_event(h);
+    if (!b) { h->pool.push_back(a); return; }
+    (void)hipEventRecord(b, h->stream);
+    h->pend.push_back({k, a, b});
+    if (h->pend.size() >= 1024) prof_drain(h);
+  }
+};
 
 // Bloch factor exp(+-i*ak) as the reference evaluates cdexp(+-ci*ak)
 void bloch(const double ak[2], int sign, double out[2]) {
@@ -171,10 +221,17 @@ TfsfArgs tfsf_args(const iris_handle* h, int it) {
 template <typename T>
 int launch_step(iris_handle* h, int it) {
   StepArgs<T> a = step_args<T>(h);
-  hipLaunchKernelGGL(k_epass<T>, h->grd_yee, h->blk_yee, 0, h->stream, a);
-  hipLaunchKernelGGL(k_hpass<T>, h->grd_yee, h->blk_yee, 0, h->stream, a);
+  {
+    ProfScope ps(h, IRIS_K_EPASS);
+    hipLaunchKernelGGL(k_epass<T>, h->grd_yee, h->blk_yee, 0, h->stream, a);
+  }
+  {
+    ProfScope ps(h, IRIS_K_HPASS);
+    hipLaunchKernelGGL(k_hpass<T>, h->grd_yee, h->blk_yee, 0, h->stream, a);
+  }
   if (h->p.em_source) {
     TfsfArgs t = tfsf_args(h, it);
+    ProfScope ps(h, IRIS_K_TFSF);
     hipLaunchKernelGGL(k_tfsf<T>, h->grd_tf, h->blk_tf, 0, h->stream, a, t);
   }
   HIP_TRY(h, hipGetLastError());
@@ -193,6 +250,7 @@ int dft_flush(iris_handle* h) {
     if (B.g.ntot == 0) continue;
     const int gy = std::min(nof, 64);
     dim3 grid((unsigned)((B.g.ntot + 255) / 256), (unsigned)gy);
+    ProfScope ps(h, IRIS_K_DFT_ACCUM);
     hipLaunchKernelGGL((k_dft_accum<T, kDftSlots>), grid, dim3(256), 0, h->stream, B.g, (const cplx<T>*)B.samp, nk,
                        (const double2*)h->ph, nof, h->p.dt, B.emg);
   }
@@ -208,6 +266,7 @@ int dft_sample(iris_handle* h, int n) {
   for (int b = 0; b < h->p.nbox; ++b) {
     DeviceBox& B = h->box[b];
     if (B.g.ntot == 0) continue;
+    ProfScope ps(h, IRIS_K_DFT_SAMPLE);
     hipLaunchKernelGGL(k_dft_sample<T>, dim3((unsigned)((B.g.ntot + 255) / 256)), dim3(256), 0, h->stream, a, B.g,
                        (cplx<T>*)B.samp, h->nsamp);
   }
@@ -405,6 +464,9 @@ int create_impl(iris_handle* h, const double* eps_inv, const double* fpmn, const
 void release(iris_handle* h) {
   if (!h) return;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  prof_drain(h);
+  for (hipEvent_t e : h->pool) (void)hipEventDestroy(e);
+  h->pool.clear();
   for (void* p : h->allocs) (void)hipFree(p);
   h->allocs.clear();
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -473,8 +535,11 @@ int iris_flux(iris_handle* h, int32_t nobox, double* jabs) {
   DeviceBox& B = h->box[nobox - 1];
   const int nof = h->p.nof;
   HIP_TRY(h, hipMemsetAsync(h->flux, 0, sizeof(double) * 6 * nof, h->stream));
-  hipLaunchKernelGGL(k_flux, dim3(6, nof), dim3(256), 0, h->stream, B.g, (const double2*)B.emg, nof, B.active_mask,
-                     h->flux);
+  {
+    ProfScope ps(h, IRIS_K_FLUX);
+    hipLaunchKernelGGL(k_flux, dim3(6, nof), dim3(256), 0, h->stream, B.g, (const double2*)B.emg, nof,
+                       B.active_mask, h->flux);
+  }
   HIP_TRY(h, hipGetLastError());
   HIP_TRY(h, hipMemcpyAsync(jabs, h->flux, sizeof(double) * 6 * nof, hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(h, hipStreamSynchronize(h->stream));
@@ -566,6 +631,25 @@ int iris_destroy(iris_handle* h) {
   if (!h) return 0;
   release(h);
   delete h;
+  return 0;
+}
+
+int iris_profile(iris_handle* h, int32_t enable) {
+  if (!h) return fail(nullptr, IRIS_EINVAL, "handle is NULL");
+  if (!enable) prof_drain(h);
+  h->prof = enable != 0;
+  return 0;
+}
+
+int iris_profile_read(iris_handle* h, double* ms, int64_t* launches, int32_t reset) {
+  if (!h || !ms || !launches) return fail(h, IRIS_EINVAL, "NULL argument");
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  prof_drain(h);
+  for (int k = 0; k < IRIS_NKERNEL; ++k) {
+    ms[k] = h->prof_ms[k];
+    launches[k] = h->prof_n[k];
+    if (reset) { h->prof_ms[k] = 0; h->prof_n[k] = 0; }
+  }
   return 0;
 }
 

@@ -95,6 +95,10 @@ def load_library(path=None):
     lib.iris_destroy.restype = ct.c_int
     lib.iris_last_error.argtypes = [H]
     lib.iris_last_error.restype = ct.c_char_p
+    lib.iris_profile.argtypes = [H, ct.c_int32]
+    lib.iris_profile.restype = ct.c_int
+    lib.iris_profile_read.argtypes = [H, ct.c_void_p, ct.c_void_p, ct.c_int32]
+    lib.iris_profile_read.restype = ct.c_int
     lib.iris_version.argtypes = []
     lib.iris_version.restype = ct.c_char_p
     _lib = lib
@@ -103,7 +107,9 @@ def load_library(path=None):
 
 EXPORTED = ["iris_create", "iris_step", "iris_sync", "iris_flux", "iris_dft_points", "iris_get_dft",
             "iris_get_field", "iris_set_field", "iris_current_step", "iris_destroy", "iris_last_error",
-            "iris_version"]
+            "iris_version", "iris_profile", "iris_profile_read"]
+
+KERNELS = ["epass", "hpass", "tfsf", "dft_sample", "dft_accum", "flux"]
 
 
 class Engine:
@@ -173,6 +179,16 @@ class Engine:
     def set_field(self, which, arr):
         a = np.ascontiguousarray(arr, dtype=np.complex128)
         self._check(self.lib.iris_set_field(self._h, int(which), _p(a.view(np.float64))), "iris_set_field")
+
+    def profile(self, enable=True):
+        self._check(self.lib.iris_profile(self._h, 1 if enable else 0), "iris_profile")
+
+    def profile_read(self, reset=False):
+        """{kernel: (total_ms, launches)} since the last reset."""
+        ms = np.zeros(len(KERNELS))
+        n = np.zeros(len(KERNELS), dtype=np.int64)
+        self._check(self.lib.iris_profile_read(self._h, _p(ms), _p(n), 1 if reset else 0), "iris_profile_read")
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(KERNELS)}
 
     def close(self):
         if self._h:
