@@ -150,7 +150,8 @@ const char* iris_last_error(const iris_handle* h);
 #define IRIS_K_DFT_SAMPLE 3 /* copy of the box faces into the sample ring */
 #define IRIS_K_DFT_ACCUM 4  /* fold of buffered samples into emg         */
 #define IRIS_K_FLUX 5       /* face-flux reduction                       */
-#define IRIS_NKERNEL 6
+#define IRIS_K_YEE 6        /* fused E+H pass (replaces EPASS/HPASS/TFSF) */
+#define IRIS_NKERNEL 7
 
 /* Enable (1) or disable (0) per-kernel timing with HIP events recorded
  * around every launch on the handle's stream.  No counterpart in the

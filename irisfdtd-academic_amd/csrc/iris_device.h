@@ -50,13 +50,19 @@ template <typename T> struct PmlPlane {
   T gh[3];          // H gain:    1/(mu0*fp3), 1/(mu0*fp1), 1/(mu0*fp2)
 };
 
-// Everything the E/H passes need, passed by value.
+// Everything the E/H passes need, passed by value.  e/h/d/b are the fields
+// of the previous step; en/hn/dn/bn receive the new values (the same arrays
+// for the in-place passes, the other buffer of each pair for the fused pass).
 template <typename T> struct StepArgs {
   using C = cplx<T>;
   C* e[3];
   C* h[3];
   C* d[2][3];       // UPML D, [bottom/top][component]
   C* b[2][3];       // UPML B
+  C* en[3];
+  C* hn[3];
+  C* dn[2][3];
+  C* bn[2][3];
   const T* eps_re;  // structure planes hi..hf (xy plane layout)
   const T* eps_im;  // nullptr when eps_inv is real
   const PmlPlane<T>* pml;  // indexed by iz (0..nz+1)

@@ -17,7 +17,9 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <utility>
 #include <string>
 #include <vector>
 
@@ -53,9 +55,14 @@ struct iris_handle {
   int xoff = 0;
   int64_t sx = 0, sy = 0, sxy = 0, ncell = 0;
   // device buffers
-  void* f[6] = {};          // E0..E2, H0..H2
+  void* f[6] = {};          // E0..E2, H0..H2 (current)
   void* d[2][3] = {};
   void* b[2][3] = {};
+  bool fused = true;        // fused E+H pass over double-buffered fields
+  void* g[6] = {};          // fused pass: the other buffer of each pair
+  void* dn[2][3] = {};
+  void* bn[2][3] = {};
+  dim3 blk_fused, grd_fused;
   void* eps_re = nullptr;
   void* eps_im = nullptr;
   void* pml = nullptr;
@@ -172,7 +179,14 @@ StepArgs<T> step_args(const iris_handle* h) {
   for (int c = 0; c < 3; ++c) {
     a.e[c] = (C*)h->f[c];
     a.h[c] = (C*)h->f[3 + c];
-    for (int k = 0; k < 2; ++k) { a.d[k][c] = (C*)h->d[k][c]; a.b[k][c] = (C*)h->b[k][c]; }
+    a.en[c] = (C*)(h->fused ? h->g[c] : h->f[c]);
+    a.hn[c] = (C*)(h->fused ? h->g[3 + c] : h->f[3 + c]);
+    for (int k = 0; k < 2; ++k) {
+      a.d[k][c] = (C*)h->d[k][c];
+      a.b[k][c] = (C*)h->b[k][c];
+      a.dn[k][c] = (C*)(h->fused ? h->dn[k][c] : h->d[k][c]);
+      a.bn[k][c] = (C*)(h->fused ? h->bn[k][c] : h->b[k][c]);
+    }
   }
   a.eps_re = (const T*)h->eps_re;
   a.eps_im = (const T*)h->eps_im;
@@ -221,6 +235,21 @@ TfsfArgs tfsf_args(const iris_handle* h, int it) {
 template <typename T>
 int launch_step(iris_handle* h, int it) {
   StepArgs<T> a = step_args<T>(h);
+  if (h->fused) {
+    TfsfArgs t = tfsf_args(h, it);
+    {
+      ProfScope ps(h, IRIS_K_YEE);
+      hipLaunchKernelGGL(k_yee_fused<T>, h->grd_fused, h->blk_fused, 0, h->stream, a, t, h->p.em_source);
+    }
+    HIP_TRY(h, hipGetLastError());
+    for (int i = 0; i < 6; ++i) std::swap(h->f[i], h->g[i]);
+    for (int k = 0; k < 2; ++k)
+      for (int c = 0; c < 3; ++c) {
+        std::swap(h->d[k][c], h->dn[k][c]);
+        std::swap(h->b[k][c], h->bn[k][c]);
+      }
+    return 0;
+  }
   {
     ProfScope ps(h, IRIS_K_EPASS);
     hipLaunchKernelGGL(k_epass<T>, h->grd_yee, h->blk_yee, 0, h->stream, a);
@@ -248,8 +277,7 @@ int dft_flush(iris_handle* h) {
   for (int b = 0; b < h->p.nbox; ++b) {
     DeviceBox& B = h->box[b];
     if (B.g.ntot == 0) continue;
-    const int gy = std::min(nof, 64);
-    dim3 grid((unsigned)((B.g.ntot + 255) / 256), (unsigned)gy);
+    dim3 grid((unsigned)((B.g.ntot + 255) / 256), 1);
     ProfScope ps(h, IRIS_K_DFT_ACCUM);
     hipLaunchKernelGGL((k_dft_accum<T, kDftSlots>), grid, dim3(256), 0, h->stream, B.g, (const cplx<T>*)B.samp, nk,
                        (const double2*)h->ph, nof, h->p.dt, B.emg);
@@ -423,13 +451,19 @@ int create_impl(iris_handle* h, const double* eps_inv, const double* fpmn, const
   h->sxy = h->sx * h->sy;
   h->ncell = h->sxy * (p.nz + 2);
   int rc;
-  for (int i = 0; i < 6; ++i)
+  for (int i = 0; i < 6; ++i) {
     if ((rc = dalloc(h, &h->f[i], h->csize * (size_t)h->ncell))) return rc;
+    if (h->fused && (rc = dalloc(h, &h->g[i], h->csize * (size_t)h->ncell))) return rc;
+  }
   const int zl[2] = {p.pmlzi, p.nz - p.pmlzf + 1};
   for (int k = 0; k < 2; ++k)
     for (int c = 0; c < 3; ++c) {
       if ((rc = dalloc(h, &h->d[k][c], h->csize * (size_t)h->sxy * zl[k]))) return rc;
       if ((rc = dalloc(h, &h->b[k][c], h->csize * (size_t)h->sxy * zl[k]))) return rc;
+      if (h->fused) {
+        if ((rc = dalloc(h, &h->dn[k][c], h->csize * (size_t)h->sxy * zl[k]))) return rc;
+        if ((rc = dalloc(h, &h->bn[k][c], h->csize * (size_t)h->sxy * zl[k]))) return rc;
+      }
     }
   if ((rc = upload_tables<T>(h, eps_inv, fpmn, fmmn, fpmx, fmmx))) return rc;
   // DFT
@@ -452,6 +486,13 @@ int create_impl(iris_handle* h, const double* eps_inv, const double* fpmn, const
   int zc = (int)std::min<int64_t>(64, std::max<int64_t>(1, nbxy * p.nz / 4096));
   h->zchunk = zc;
   h->grd_yee = dim3(gx, gy, (p.nz + zc - 1) / zc);
+  {
+    const int fgx = (p.nx + kFTX - 1) / kFTX, fgy = (p.ny + kFTY - 1) / kFTY;
+    const int fzc = (int)std::min<int64_t>(64, std::max<int64_t>(1, (int64_t)fgx * fgy * p.nz / 4096));
+    h->blk_fused = dim3(kFTX, kFTY + 1, 1);
+    h->grd_fused = dim3(fgx, fgy, (p.nz + fzc - 1) / fzc);
+    if (h->fused) h->zchunk = fzc;
+  }
   const int tbx = std::min(64, pow2ceil(p.nx + 1));
   const int tby = std::max(1, std::min(256 / tbx, pow2ceil(p.ny + 1)));
   h->blk_tf = dim3(tbx, tby, 1);
@@ -495,6 +536,10 @@ int iris_create(const iris_params* p, const double* eps_inv, const double* fpzmn
   iris_handle* h = new iris_handle();
   h->p = *p;
   h->prec = p->precision;
+  {
+    const char* env = std::getenv("IRIS_FUSED");
+    h->fused = !(env && env[0] == '0');
+  }
   (void)hipGetDevice(&h->device);
   e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {

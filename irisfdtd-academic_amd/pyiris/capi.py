@@ -109,7 +109,7 @@ EXPORTED = ["iris_create", "iris_step", "iris_sync", "iris_flux", "iris_dft_poin
             "iris_get_field", "iris_set_field", "iris_current_step", "iris_destroy", "iris_last_error",
             "iris_version", "iris_profile", "iris_profile_read"]
 
-KERNELS = ["epass", "hpass", "tfsf", "dft_sample", "dft_accum", "flux"]
+KERNELS = ["epass", "hpass", "tfsf", "dft_sample", "dft_accum", "flux", "yee"]
 
 
 class Engine:

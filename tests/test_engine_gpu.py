@@ -189,3 +189,90 @@ def test_fortran_program_end_to_end(tmp_path):
     np.testing.assert_allclose(np.stack([v for _, v in tot]), g["jtot_b1"], rtol=1e-9, atol=1e-20)
     spec = np.loadtxt(tmp_path / "dielectric_layer_spectra.dat")
     np.testing.assert_allclose(spec, g["film_spectrum"], rtol=1e-14)
+
+
+def _engine_with(setup, precision, fused, ak=None):
+    old = os.environ.get("IRIS_FUSED")
+    os.environ["IRIS_FUSED"] = "1" if fused else "0"
+    try:
+        s = setup
+        if ak is not None:
+            from pyiris.host import Setup
+            p = setup.params.copy()
+            for i, v in enumerate(ak):
+                p.ak[i][0] = v.real
+                p.ak[i][1] = v.imag
+            s = Setup(p, setup.eps_inv, setup.fpzmn, setup.fmzmn, setup.fpzmx, setup.fmzmx, setup.fw,
+                      setup.spectra, setup.extra)
+        return Engine(s, precision=precision)
+    finally:
+        if old is None:
+            del os.environ["IRIS_FUSED"]
+        else:
+            os.environ["IRIS_FUSED"] = old
+
+
+FUSED_CASES = {
+    "3d_partial_tiles_ipol2": dict(ixmax=70, iymax=9, izmax=150, pmlzi=10, pmlzf=140, hi=60, dh=9, dx=7,
+                                   eps_bbox=3.0, tfsf_o=20, box=(3, 68, 2, 9, 15, 130), nof=3, nt=80),
+    "3d_ipol1": dict(i_pol=1, ixmax=9, iymax=13, izmax=90, pmlzi=10, pmlzf=80, hi=40, dh=8, dx=3, eps_bbox=2.0,
+                     tfsf_o=20, box=(1, 9, 1, 13, 15, 70), nof=3, nt=80, cosine=1),
+    "1d": dict(ixmax=1, iymax=1, izmax=120, pmlzi=10, pmlzf=110, hi=50, dh=9, tfsf_o=20, box=(1, 1, 1, 1, 15, 100),
+               nof=3, nt=120),
+    "2d_x": dict(ixmax=20, iymax=1, izmax=80, pmlzi=10, pmlzf=70, hi=35, dh=6, dx=4, eps_bbox=9.0, tfsf_o=16,
+                 box=(1, 1, 1, 1, 12, 60), nof=3, nt=90),
+    "2d_y": dict(ixmax=1, iymax=12, izmax=70, pmlzi=10, pmlzf=60, hi=30, dh=6, eps_bbox=6.0, eps1=2.25,
+                 tfsf_o=16, box=(1, 1, 2, 11, 12, 55), nof=3, nt=90),
+}
+
+
+@pytest.mark.parametrize("name", sorted(FUSED_CASES))
+@pytest.mark.parametrize("precision", [8, 4])
+def test_fused_pass_is_bitwise_the_two_pass_result(tmp_path, name, precision):
+    """The fused E+H pass performs the same operations as the in-place E and
+    H passes (halo cells recomputed from identical inputs), so every live
+    value must agree bit for bit."""
+    _announce(f"fused vs two-pass {name} prec={precision}")
+    kw = dict(FUSED_CASES[name])
+    nt = kw.pop("nt")
+    s = load_input(write_input(tmp_path, inputs.render(nt=nt, no_downloads=nt, **kw)))
+    with _engine_with(s, precision, True) as f, _engine_with(s, precision, False) as t:
+        for stop in (nt // 3, nt):
+            f.run_to(stop)
+            t.run_to(stop)
+            ef, et = f.field(0), t.field(0)
+            hf, ht = f.field(1), t.field(1)
+            inn = slice(1, -1)
+            assert np.array_equal(ef[inn, inn, inn], et[inn, inn, inn])
+            assert np.array_equal(hf[inn, inn, inn], ht[inn, inn, inn])
+            for sl in [(inn, inn, 0, [1, 2]), (inn, 0, inn, [0, 2]), (0, inn, inn, [0, 1])]:
+                assert np.array_equal(hf[sl[:3]][..., sl[3]], ht[sl[:3]][..., sl[3]]), sl
+            for pair in (1, 2, 3):
+                assert np.array_equal(f.dft(1, pair), t.dft(1, pair))
+            assert np.array_equal(f.flux(1), t.flux(1))
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_bloch_phases_match_oracle(tmp_path, fused):
+    """Nonzero complex Bloch phases in x, y and z (the reference's akx, aky,
+    akz; zero in its own inputs) through both Yee paths against the oracle."""
+    _announce(f"bloch fused={fused}")
+    kw = dict(FUSED_CASES["3d_partial_tiles_ipol2"])
+    kw.pop("nt")
+    s = load_input(write_input(tmp_path, inputs.render(nt=60, no_downloads=60, **kw)))
+    ak = (0.3 + 0.0j, -0.17 + 0.0j, 0.05 + 0.0j)
+    from pyiris.host import Setup
+    p = s.params.copy()
+    for i, v in enumerate(ak):
+        p.ak[i][0], p.ak[i][1] = v.real, v.imag
+    so = Setup(p, s.eps_inv, s.fpzmn, s.fmzmn, s.fpzmx, s.fmzmx, s.fw, s.spectra, s.extra)
+    o = pyoracle.Oracle(so)
+    o.run_to(60)
+    with _engine_with(s, 8, fused, ak=ak) as eng:
+        eng.run_to(60)
+        inn = slice(1, -1)
+        for w in (0, 1):
+            a, b = eng.field(w)[inn, inn, inn], o.field(w)[inn, inn, inn]
+            assert np.linalg.norm(a - b) / np.linalg.norm(b) <= 1e-12, w
+        for pair in (1, 2, 3):
+            assert dft_rel_err(eng.dft(1, pair), o.dft(1, pair)) <= 1e-12
