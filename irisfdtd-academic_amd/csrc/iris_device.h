@@ -50,19 +50,26 @@ template <typename T> struct PmlPlane {
   T gh[3];          // H gain:    1/(mu0*fp3), 1/(mu0*fp1), 1/(mu0*fp2)
 };
 
-// Everything the E/H passes need, passed by value.  e/h/d/b are the fields
-// of the previous step; en/hn/dn/bn receive the new values (the same arrays
-// for the in-place passes, the other buffer of each pair for the fused pass).
+// Everything the Yee passes need, passed by value.  Each field is one
+// allocation holding its three components cs elements apart; the UPML aux
+// fields D (E side) and B (H side) hold the bottom slab [1,pmlzi] and then
+// the top slab [pmlzf,nz], three components each (component stride ps0/ps1,
+// top slab at offset pb1).  e/h/d/b are the previous step's values, en/hn/
+// dn/bn receive the new ones (the same arrays for the in-place passes, the
+// other buffer of each pair for the fused pass).
 template <typename T> struct StepArgs {
   using C = cplx<T>;
-  C* e[3];
-  C* h[3];
-  C* d[2][3];       // UPML D, [bottom/top][component]
-  C* b[2][3];       // UPML B
-  C* en[3];
-  C* hn[3];
-  C* dn[2][3];
-  C* bn[2][3];
+  C* e;
+  C* h;
+  C* en;
+  C* hn;
+  C* d;
+  C* b;
+  C* dn;
+  C* bn;
+  int64_t cs;       // component stride of E and H
+  int64_t ps0, ps1; // component stride of the bottom / top UPML slab
+  int64_t pb1;      // offset of the top slab in D and B
   const T* eps_re;  // structure planes hi..hf (xy plane layout)
   const T* eps_im;  // nullptr when eps_inv is real
   const PmlPlane<T>* pml;  // indexed by iz (0..nz+1)
@@ -76,6 +83,18 @@ template <typename T> struct StepArgs {
   C fxp, fyp, fzp;  // exp(+i ak) Bloch factors
   C fxm, fym, fzm;  // exp(-i ak)
   int zchunk;       // planes per thread in the z-marching passes
+  // offset of component c of UPML slab k at in-slab index pidx
+  __device__ int64_t aux(int k, int c, int64_t pidx) const {
+    return (k ? pb1 + c * ps1 : c * ps0) + pidx;
+  }
+};
+
+// TF/SF source constants (reference code/tfsf.f90:231-290), in device
+// memory; the step index enters as c0dt*it.
+struct TfsfConst {
+  double q, ux, uz, zopump, ko, dk, sq, sign_kz, c0dt, amp, hamp, win;
+  double ein, dtcq2, eps_ms;
+  int ixo, iz_min, i_pol, cosine, mode2d, on;
 };
 
 }  // namespace iris

@@ -30,7 +30,10 @@ using namespace iris;
 
 namespace {
 
-constexpr int kDftSlots = 16;  // samples buffered before folding into emg
+// samples buffered before folding into emg: 16 with FP64 fields (exact
+// reference order), 32 with FP32 fields (half the accumulator traffic)
+constexpr int kDftSlots64 = 16, kDftSlots32 = 32, kDftSlotsMax = 32;
+template <typename T> constexpr int dft_slots() { return sizeof(T) == 8 ? kDftSlots64 : kDftSlots32; }
 
 thread_local std::string g_create_error = "no error";
 
@@ -55,14 +58,23 @@ struct iris_handle {
   int xoff = 0;
   int64_t sx = 0, sy = 0, sxy = 0, ncell = 0;
   // device buffers
-  void* f[6] = {};          // E0..E2, H0..H2 (current)
-  void* d[2][3] = {};
-  void* b[2][3] = {};
+  // fields: one allocation per field, components cs apart
+  void* fe = nullptr;       // E (current)
+  void* fh = nullptr;       // H (current)
+  void* ge = nullptr;       // fused pass: the other E and H buffers
+  void* gh = nullptr;
+  int64_t cs = 0;
+  // UPML aux: bottom slab then top slab, components ps0/ps1 apart
+  void* d = nullptr;
+  void* b = nullptr;
+  void* dn = nullptr;
+  void* bn = nullptr;
+  int64_t ps0 = 0, ps1 = 0, pb1 = 0;
   bool fused = true;        // fused E+H pass over double-buffered fields
-  void* g[6] = {};          // fused pass: the other buffer of each pair
-  void* dn[2][3] = {};
-  void* bn[2][3] = {};
   dim3 blk_fused, grd_fused;
+  TfsfConst* tfc = nullptr; // device copy of the TF/SF constants
+  int* chunks = nullptr;    // first plane of each fused z-chunk: bulk ones, then general ones
+  int nbulk = 0, ngen = 0;
   void* eps_re = nullptr;
   void* eps_im = nullptr;
   void* pml = nullptr;
@@ -176,18 +188,18 @@ template <typename T>
 StepArgs<T> step_args(const iris_handle* h) {
   using C = cplx<T>;
   StepArgs<T> a{};
-  for (int c = 0; c < 3; ++c) {
-    a.e[c] = (C*)h->f[c];
-    a.h[c] = (C*)h->f[3 + c];
-    a.en[c] = (C*)(h->fused ? h->g[c] : h->f[c]);
-    a.hn[c] = (C*)(h->fused ? h->g[3 + c] : h->f[3 + c]);
-    for (int k = 0; k < 2; ++k) {
-      a.d[k][c] = (C*)h->d[k][c];
-      a.b[k][c] = (C*)h->b[k][c];
-      a.dn[k][c] = (C*)(h->fused ? h->dn[k][c] : h->d[k][c]);
-      a.bn[k][c] = (C*)(h->fused ? h->bn[k][c] : h->b[k][c]);
-    }
-  }
+  a.e = (C*)h->fe;
+  a.h = (C*)h->fh;
+  a.en = (C*)(h->fused ? h->ge : h->fe);
+  a.hn = (C*)(h->fused ? h->gh : h->fh);
+  a.d = (C*)h->d;
+  a.b = (C*)h->b;
+  a.dn = (C*)(h->fused ? h->dn : h->d);
+  a.bn = (C*)(h->fused ? h->bn : h->b);
+  a.cs = h->cs;
+  a.ps0 = h->ps0;
+  a.ps1 = h->ps1;
+  a.pb1 = h->pb1;
   a.eps_re = (const T*)h->eps_re;
   a.eps_im = (const T*)h->eps_im;
   a.pml = (const PmlPlane<T>*)h->pml;
@@ -208,9 +220,10 @@ StepArgs<T> step_args(const iris_handle* h) {
   return a;
 }
 
-TfsfArgs tfsf_args(const iris_handle* h, int it) {
+// TF/SF constants as the reference evaluates them (code/tfsf.f90:262-290)
+TfsfConst tfsf_const(const iris_handle* h) {
   const iris_params& p = h->p;
-  TfsfArgs t{};
+  TfsfConst t{};
   t.q = p.q; t.ux = p.ux; t.uz = p.uz; t.zopump = p.zopump;
   t.ko = (2.0 * p.pi) / (p.wlpump * p.f_abohr);
   t.dk = p.gpump * p.q;
@@ -218,7 +231,7 @@ TfsfArgs tfsf_args(const iris_handle* h, int it) {
   if (p.iz_min > p.hi) { eps_ms = p.eps[2]; t.sign_kz = -1.0; } else { eps_ms = p.eps[0]; t.sign_kz = 1.0; }
   t.sq = std::sqrt(eps_ms);
   t.eps_ms = eps_ms;
-  t.c0dt_it = (p.c0 * p.dt) * (double)it;
+  t.c0dt = p.c0 * p.dt;
   t.amp = p.eamp * p.q;
   t.hamp = t.amp * ((p.dt * p.c0) / p.q);
   t.win = p.win;
@@ -229,6 +242,7 @@ TfsfArgs tfsf_args(const iris_handle* h, int it) {
   t.i_pol = p.i_pol;
   t.cosine = p.cosine;
   t.mode2d = (p.nx == 1 || p.ny == 1) ? 1 : 0;
+  t.on = p.em_source ? 1 : 0;
   return t;
 }
 
@@ -236,18 +250,25 @@ template <typename T>
 int launch_step(iris_handle* h, int it) {
   StepArgs<T> a = step_args<T>(h);
   if (h->fused) {
-    TfsfArgs t = tfsf_args(h, it);
     {
       ProfScope ps(h, IRIS_K_YEE);
-      hipLaunchKernelGGL(k_yee_fused<T>, h->grd_fused, h->blk_fused, 0, h->stream, a, t, h->p.em_source);
+      dim3 g = h->grd_fused;
+      if (h->nbulk) {
+        g.z = h->nbulk;
+        hipLaunchKernelGGL((k_yee_fused<T, false>), g, h->blk_fused, 0, h->stream, a, (const TfsfConst*)h->tfc, it,
+                           (const int*)h->chunks);
+      }
+      if (h->ngen) {
+        g.z = h->ngen;
+        hipLaunchKernelGGL((k_yee_fused<T, true>), g, h->blk_fused, 0, h->stream, a, (const TfsfConst*)h->tfc, it,
+                           (const int*)h->chunks + h->nbulk);
+      }
     }
     HIP_TRY(h, hipGetLastError());
-    for (int i = 0; i < 6; ++i) std::swap(h->f[i], h->g[i]);
-    for (int k = 0; k < 2; ++k)
-      for (int c = 0; c < 3; ++c) {
-        std::swap(h->d[k][c], h->dn[k][c]);
-        std::swap(h->b[k][c], h->bn[k][c]);
-      }
+    std::swap(h->fe, h->ge);
+    std::swap(h->fh, h->gh);
+    std::swap(h->d, h->dn);
+    std::swap(h->b, h->bn);
     return 0;
   }
   {
@@ -259,9 +280,8 @@ int launch_step(iris_handle* h, int it) {
     hipLaunchKernelGGL(k_hpass<T>, h->grd_yee, h->blk_yee, 0, h->stream, a);
   }
   if (h->p.em_source) {
-    TfsfArgs t = tfsf_args(h, it);
     ProfScope ps(h, IRIS_K_TFSF);
-    hipLaunchKernelGGL(k_tfsf<T>, h->grd_tf, h->blk_tf, 0, h->stream, a, t);
+    hipLaunchKernelGGL(k_tfsf<T>, h->grd_tf, h->blk_tf, 0, h->stream, a, (const TfsfConst*)h->tfc, it);
   }
   HIP_TRY(h, hipGetLastError());
   return 0;
@@ -272,15 +292,17 @@ int dft_flush(iris_handle* h) {
   if (h->nsamp == 0) return 0;
   const int nof = h->p.nof;
   const int nk = h->nsamp;
-  hipLaunchKernelGGL(k_dft_phase, dim3((nof * nk + 255) / 256), dim3(256), 0, h->stream, h->fw, nof, nk,
-                     h->samp_n0, h->p.sample_fourier, h->ph);
+  constexpr bool exact = sizeof(T) == 8;
+  const int nrows = dft_slots<T>();
+  hipLaunchKernelGGL(k_dft_phase, dim3((nof * nrows + 255) / 256), dim3(256), 0, h->stream, h->fw, nof, nk, nrows,
+                     h->samp_n0, h->p.sample_fourier, exact ? 1.0 : h->p.dt, h->ph);
   for (int b = 0; b < h->p.nbox; ++b) {
     DeviceBox& B = h->box[b];
     if (B.g.ntot == 0) continue;
     dim3 grid((unsigned)((B.g.ntot + 255) / 256), 1);
     ProfScope ps(h, IRIS_K_DFT_ACCUM);
-    hipLaunchKernelGGL((k_dft_accum<T, kDftSlots>), grid, dim3(256), 0, h->stream, B.g, (const cplx<T>*)B.samp, nk,
-                       (const double2*)h->ph, nof, h->p.dt, B.emg);
+    hipLaunchKernelGGL((k_dft_fold<cplx<T>, dft_slots<T>()>), grid, dim3(256), 0, h->stream, B.g,
+                       (const cplx<T>*)B.samp, nk, (const double2*)h->ph, nof, h->p.dt, B.emg);
   }
   HIP_TRY(h, hipGetLastError());
   h->nsamp = 0;
@@ -300,7 +322,7 @@ int dft_sample(iris_handle* h, int n) {
   }
   HIP_TRY(h, hipGetLastError());
   h->nsamp++;
-  if (h->nsamp == kDftSlots) return dft_flush<T>(h);
+  if (h->nsamp == dft_slots<T>()) return dft_flush<T>(h);
   return 0;
 }
 
@@ -451,31 +473,28 @@ int create_impl(iris_handle* h, const double* eps_inv, const double* fpmn, const
   h->sxy = h->sx * h->sy;
   h->ncell = h->sxy * (p.nz + 2);
   int rc;
-  for (int i = 0; i < 6; ++i) {
-    if ((rc = dalloc(h, &h->f[i], h->csize * (size_t)h->ncell))) return rc;
-    if (h->fused && (rc = dalloc(h, &h->g[i], h->csize * (size_t)h->ncell))) return rc;
-  }
-  const int zl[2] = {p.pmlzi, p.nz - p.pmlzf + 1};
-  for (int k = 0; k < 2; ++k)
-    for (int c = 0; c < 3; ++c) {
-      if ((rc = dalloc(h, &h->d[k][c], h->csize * (size_t)h->sxy * zl[k]))) return rc;
-      if ((rc = dalloc(h, &h->b[k][c], h->csize * (size_t)h->sxy * zl[k]))) return rc;
-      if (h->fused) {
-        if ((rc = dalloc(h, &h->dn[k][c], h->csize * (size_t)h->sxy * zl[k]))) return rc;
-        if ((rc = dalloc(h, &h->bn[k][c], h->csize * (size_t)h->sxy * zl[k]))) return rc;
-      }
-    }
+  // component stride rounded to 256 elements keeps every component aligned
+  h->cs = ((h->ncell + 255) / 256) * 256;
+  const size_t fbytes = h->csize * 3 * (size_t)h->cs;
+  if ((rc = dalloc(h, &h->fe, fbytes)) || (rc = dalloc(h, &h->fh, fbytes))) return rc;
+  if (h->fused && ((rc = dalloc(h, &h->ge, fbytes)) || (rc = dalloc(h, &h->gh, fbytes)))) return rc;
+  h->ps0 = h->sxy * p.pmlzi;
+  h->ps1 = h->sxy * (p.nz - p.pmlzf + 1);
+  h->pb1 = 3 * h->ps0;
+  const size_t abytes = h->csize * 3 * (size_t)(h->ps0 + h->ps1);
+  if ((rc = dalloc(h, &h->d, abytes)) || (rc = dalloc(h, &h->b, abytes))) return rc;
+  if (h->fused && ((rc = dalloc(h, &h->dn, abytes)) || (rc = dalloc(h, &h->bn, abytes)))) return rc;
   if ((rc = upload_tables<T>(h, eps_inv, fpmn, fmmn, fpmx, fmmx))) return rc;
   // DFT
   if ((rc = dalloc(h, (void**)&h->fw, sizeof(double) * p.nof))) return rc;
   HIP_TRY(h, hipMemcpyAsync(h->fw, fw, sizeof(double) * p.nof, hipMemcpyHostToDevice, h->stream));
-  if ((rc = dalloc(h, (void**)&h->ph, sizeof(double2) * (size_t)p.nof * kDftSlots))) return rc;
+  if ((rc = dalloc(h, (void**)&h->ph, sizeof(double2) * (size_t)p.nof * kDftSlotsMax))) return rc;
   if ((rc = dalloc(h, (void**)&h->flux, sizeof(double) * 6 * (size_t)p.nof))) return rc;
   for (int b = 0; b < p.nbox; ++b) {
     DeviceBox& B = h->box[b];
     box_geometry(p, b, B);
     if ((rc = dalloc(h, (void**)&B.emg, sizeof(double2) * (size_t)std::max<int64_t>(B.g.ntot, 1) * p.nof))) return rc;
-    if ((rc = dalloc(h, &B.samp, h->csize * (size_t)std::max<int64_t>(B.g.ntot, 1) * kDftSlots))) return rc;
+    if ((rc = dalloc(h, &B.samp, h->csize * (size_t)std::max<int64_t>(B.g.ntot, 1) * kDftSlotsMax))) return rc;
   }
   // launch shapes: x-rows of up to 64 (one wavefront), 256 threads per block
   const int bx = std::min(64, pow2ceil(p.nx));
@@ -488,7 +507,7 @@ int create_impl(iris_handle* h, const double* eps_inv, const double* fpmn, const
   h->grd_yee = dim3(gx, gy, (p.nz + zc - 1) / zc);
   {
     const int fgx = (p.nx + kFTX - 1) / kFTX, fgy = (p.ny + kFTY - 1) / kFTY;
-    const int fzc = (int)std::min<int64_t>(64, std::max<int64_t>(1, (int64_t)fgx * fgy * p.nz / 4096));
+    const int fzc = (int)std::min<int64_t>(32, std::max<int64_t>(1, (int64_t)fgx * fgy * p.nz / 4096));
     h->blk_fused = dim3(kFTX, kFTY + 1, 1);
     h->grd_fused = dim3(fgx, fgy, (p.nz + fzc - 1) / fzc);
     if (h->fused) h->zchunk = fzc;
@@ -498,6 +517,37 @@ int create_impl(iris_handle* h, const double* eps_inv, const double* fpmn, const
   h->blk_tf = dim3(tbx, tby, 1);
   h->grd_tf = dim3((p.nx + 1 + tbx - 1) / tbx, (p.ny + 1 + tby - 1) / tby, 1);
   h->dtcq2 = ((p.dt * p.c0) / p.q) * ((p.dt * p.c0) / p.q);
+  {
+    const TfsfConst t = tfsf_const(h);
+    if ((rc = dalloc(h, (void**)&h->tfc, sizeof(TfsfConst)))) return rc;
+    HIP_TRY(h, hipMemcpyAsync(h->tfc, &t, sizeof t, hipMemcpyHostToDevice, h->stream));
+  }
+  {
+    // fused z-chunks: a chunk is bulk when every plane it reads or writes,
+    // z0-1 .. z1+1, lies strictly between the UPML slabs, outside the
+    // structure and away from the TF/SF planes, and it is not the top chunk
+    std::vector<int> bulk, gen;
+    const int zc = h->zchunk;
+    auto is_bulk_plane = [&](int z) {
+      if (z <= p.pmlzi || z >= p.pmlzf) return false;
+      if (z >= p.hi && z <= p.hf) return false;
+      if (p.em_source && (z == p.iz_min - 1 || z == p.iz_min)) return false;
+      return true;
+    };
+    for (int z0 = 1; z0 <= p.nz; z0 += zc) {
+      const int z1 = std::min(z0 + zc - 1, p.nz);
+      bool ok = z1 < p.nz;
+      for (int z = z0 - 1; ok && z <= z1 + 1; ++z) ok = is_bulk_plane(z);
+      (ok ? bulk : gen).push_back(z0);
+    }
+    h->nbulk = (int)bulk.size();
+    h->ngen = (int)gen.size();
+    std::vector<int> all(bulk);
+    all.insert(all.end(), gen.begin(), gen.end());
+    if ((rc = dalloc(h, (void**)&h->chunks, sizeof(int) * all.size()))) return rc;
+    HIP_TRY(h, hipMemcpyAsync(h->chunks, all.data(), sizeof(int) * all.size(), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+  }
   HIP_TRY(h, hipStreamSynchronize(h->stream));
   return 0;
 }
@@ -626,7 +676,8 @@ int iris_get_field(iris_handle* h, int32_t which, double* out) {
   std::vector<char> buf(h->csize * (size_t)h->ncell);
   const int64_t rsx = p.nx + 2, rsxy = rsx * (p.ny + 2);
   for (int c = 0; c < 3; ++c) {
-    HIP_TRY(h, hipMemcpyAsync(buf.data(), h->f[3 * which + c], buf.size(), hipMemcpyDeviceToHost, h->stream));
+    const char* src = (const char*)(which ? h->fh : h->fe) + h->csize * (size_t)h->cs * c;
+    HIP_TRY(h, hipMemcpyAsync(buf.data(), src, buf.size(), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(h, hipStreamSynchronize(h->stream));
     for (int z = 0; z <= p.nz + 1; ++z)
       for (int y = 0; y <= p.ny + 1; ++y)
@@ -660,7 +711,8 @@ int iris_set_field(iris_handle* h, int32_t which, const double* in) {
           if (h->prec == 8) ((double2*)buf.data())[di] = make_double2(v[0], v[1]);
           else ((float2*)buf.data())[di] = make_float2((float)v[0], (float)v[1]);
         }
-    HIP_TRY(h, hipMemcpyAsync(h->f[3 * which + c], buf.data(), buf.size(), hipMemcpyHostToDevice, h->stream));
+    char* dst = (char*)(which ? h->fh : h->fe) + h->csize * (size_t)h->cs * c;
+    HIP_TRY(h, hipMemcpyAsync(dst, buf.data(), buf.size(), hipMemcpyHostToDevice, h->stream));
     HIP_TRY(h, hipStreamSynchronize(h->stream));
   }
   return 0;
