@@ -71,6 +71,7 @@ struct iris_handle {
   void* bn = nullptr;
   int64_t ps0 = 0, ps1 = 0, pb1 = 0;
   bool fused = true;        // fused E+H pass over double-buffered fields
+  int xcd_rows = 0;         // tile rows per XCD run of the fused pass (0: identity order)
   dim3 blk_fused, grd_fused;
   TfsfConst* tfc = nullptr; // device copy of the TF/SF constants
   int* chunks = nullptr;    // first plane of each fused z-chunk: bulk ones, then general ones
@@ -252,16 +253,16 @@ int launch_step(iris_handle* h, int it) {
   if (h->fused) {
     {
       ProfScope ps(h, IRIS_K_YEE);
-      dim3 g = h->grd_fused;
+      TileMap tm{(int)h->grd_fused.x, (int)h->grd_fused.y, 0, (int)h->grd_fused.x * h->xcd_rows};
       if (h->nbulk) {
-        g.z = h->nbulk;
-        hipLaunchKernelGGL((k_yee_fused<T, false>), g, h->blk_fused, 0, h->stream, a, (const TfsfConst*)h->tfc, it,
-                           (const int*)h->chunks);
+        tm.nblk = tm.tx * tm.ty * h->nbulk;
+        hipLaunchKernelGGL((k_yee_fused<T, false>), dim3(tm.nblk), h->blk_fused, 0, h->stream, a,
+                           (const TfsfConst*)h->tfc, it, (const int*)h->chunks, tm);
       }
       if (h->ngen) {
-        g.z = h->ngen;
-        hipLaunchKernelGGL((k_yee_fused<T, true>), g, h->blk_fused, 0, h->stream, a, (const TfsfConst*)h->tfc, it,
-                           (const int*)h->chunks + h->nbulk);
+        tm.nblk = tm.tx * tm.ty * h->ngen;
+        hipLaunchKernelGGL((k_yee_fused<T, true>), dim3(tm.nblk), h->blk_fused, 0, h->stream, a,
+                           (const TfsfConst*)h->tfc, it, (const int*)h->chunks + h->nbulk, tm);
       }
     }
     HIP_TRY(h, hipGetLastError());
@@ -589,6 +590,8 @@ int iris_create(const iris_params* p, const double* eps_inv, const double* fpzmn
   {
     const char* env = std::getenv("IRIS_FUSED");
     h->fused = !(env && env[0] == '0');
+    const char* xenv = std::getenv("IRIS_XCD_ROWS");
+    if (xenv && *xenv) h->xcd_rows = std::max(0, std::atoi(xenv));
   }
   (void)hipGetDevice(&h->device);
   e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);

@@ -379,6 +379,35 @@ __device__ inline void e_item(const StepArgs<T>& a, int xs, int ys, int zs, cplx
   hown[2] = h3;
 }
 
+// Tile order of a fused launch.  Workgroups are dealt to the 8 XCDs
+// round-robin by workgroup id, and each XCD has its own L2.  With run > 0
+// the tiles, in (x, y, chunk) order, are cut into runs of `run` tiles and the
+// runs are dealt to the XCDs round-robin: the workgroups of one XCD process
+// whole runs, so x- and y-neighbours inside a run, whose halo rows and
+// columns overlap, share an L2, while every chunk is still spread over all
+// XCDs.  Tiles past the last whole round of runs, and run = 0, keep the
+// identity order t = L.
+struct TileMap {
+  int tx, ty;  // tiles along x and y
+  int nblk;    // workgroups in the launch (tx * ty * chunks)
+  int run;     // tiles per run (0: identity order)
+};
+constexpr int kNumXcd = 8;
+
+__device__ inline void tile_of(const TileMap& m, int L, int* bx, int* by, int* bz) {
+  int t = L;
+  if (m.run > 0) {
+    const int round = kNumXcd * m.run;
+    if (L < (m.nblk / round) * round) {
+      const int x = L % kNumXcd, loc = L / kNumXcd;
+      t = ((loc / m.run) * kNumXcd + x) * m.run + loc % m.run;
+    }
+  }
+  *bx = t % m.tx;
+  *by = (t / m.tx) % m.ty;
+  *bz = t / (m.tx * m.ty);
+}
+
 template <typename C> __device__ inline void phase3(C ph, C v[3]) {
   v[0] = cmul(ph, v[0]);
   v[1] = cmul(ph, v[1]);
@@ -388,17 +417,20 @@ template <typename C> __device__ inline void phase3(C ph, C v[3]) {
 // GEN = false: the lean variant for z-chunks whose planes z0-1..z1+1 are all
 // bulk (between the UPML slabs, outside the structure and the TF/SF planes,
 // z1 < nz); GEN = true handles every other chunk.  chunks[] lists the first
-// plane of each chunk this launch processes.
+// plane of each chunk this launch processes; the grid is one-dimensional,
+// tm.nblk workgroups, and tm gives each its tile.
 template <typename T, bool GEN>
 __global__ __launch_bounds__(kFTX * (kFTY + 1), IRIS_FUSED_MINBLOCKS) void k_yee_fused(StepArgs<T> a, const TfsfConst* tp, int it,
-                                                                                      const int* chunks) {
+                                                                                      const int* chunks, TileMap tm) {
   using C = cplx<T>;
   __shared__ C se[3][kFTY + 1][kFTX + 1];
   const int lane = threadIdx.x, wy = threadIdx.y;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const int64_t cs = a.cs;
-  const int x0 = blockIdx.x * kFTX + 1, y0 = blockIdx.y * kFTY + 1;
-  const int z0 = chunks[blockIdx.z];
+  int tbx, tby, tbz;
+  tile_of(tm, blockIdx.x, &tbx, &tby, &tbz);
+  const int x0 = tbx * kFTX + 1, y0 = tby * kFTY + 1;
+  const int z0 = chunks[tbz];
   const int z1 = min(z0 + a.zchunk - 1, nz);
   const bool halo = wy == kFTY;
   // item A: own cell, or a Bloch image inside a partial tile, or (halo wave)
