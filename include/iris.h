@@ -17,6 +17,8 @@
  *                    the host writes the <box>_scatt<j>.dat files itself
  *   iris_get_dft  <- the emg(j)%fdata accumulators (code/dataprocess.f90:143-147)
  *   iris_get_field<- e_cur / h_cur arrays (code/main.f90:166-167)
+ *   iris_get_plane<- the field reads of the snapshot writer (code/sdl.f90:210-313,
+ *                    called with e_cur, h_prev at code/driver.f90:468-495)
  *   iris_destroy  <- driver teardown (code/driver.f90:503-552)
  *
  * Conventions: every entry point returns 0 on success and a negative
@@ -132,6 +134,23 @@ int iris_get_field(iris_handle* h, int32_t which, double* out);
 /* Overwrite the E or H field (same layout as iris_get_field); used to start
  * from a given state.  Synchronous. */
 int iris_set_field(iris_handle* h, int32_t which, const double* in);
+
+/* Field selectors of iris_get_plane: base + component (0: x, 1: y, 2: z). */
+#define IRIS_PLANE_E 0      /* E after the step most recently completed       */
+#define IRIS_PLANE_H_PREV 3 /* H of the step before it: what the reference's   */
+                            /* snapshots show (h_prev, code/driver.f90:471)   */
+#define IRIS_PLANE_H 6      /* H after the step most recently completed       */
+
+/* Copy one field component on one plane: axis 1 is the plane x = index with
+ * free coordinates (y, z); axis 2 is y = index with (x, z); axis 3 is
+ * z = index with (x, y) -- the reference's snapshot planes (code/sdl.f90:
+ * 92-110).  range = {a0, a1, b0, b1} are the inclusive ranges of the first
+ * and second free coordinate; index and ranges lie in 1..n of their axis.
+ * out receives (a1-a0+1)*(b1-b0+1) complex values, first free coordinate
+ * fastest.  IRIS_PLANE_H_PREV needs the fused pass (the in-place two-pass
+ * engine overwrites H; IRIS_EUNSUP).  Synchronous. */
+int iris_get_plane(iris_handle* h, int32_t what, int32_t axis, int32_t index, const int32_t range[4],
+                   double* out);
 
 /* The step most recently completed (0 before the first step). */
 int iris_current_step(const iris_handle* h, int32_t* it);

@@ -53,11 +53,10 @@ def build_host():
     os.makedirs(BIN, exist_ok=True)
     os.makedirs(OBJ, exist_ok=True)
     fdir = os.path.join(HERE, "fortran")
-    fflags = ["-O2", "-fPIC", "-module-dir", OBJ, "-J", OBJ]
     # amdflang accepts -J for the module directory
     fflags = ["-O2", "-fPIC", "-J", OBJ]
     objs = []
-    for name in ["iris_c_api", "iris_setup"]:
+    for name in ["iris_c_api", "iris_sdl", "iris_setup"]:
         o = os.path.join(OBJ, name + ".o")
         run([FC] + fflags + ["-c", os.path.join(fdir, name + ".f90"), "-o", o])
         objs.append(o)

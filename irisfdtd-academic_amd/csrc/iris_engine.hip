@@ -71,6 +71,8 @@ struct iris_handle {
   void* bn = nullptr;
   int64_t ps0 = 0, ps1 = 0, pb1 = 0;
   bool fused = true;        // fused E+H pass over double-buffered fields
+  void* plane_buf = nullptr; // staging of iris_get_plane
+  int64_t plane_cap = 0;      // its capacity in complex values
   int xcd_rows = 0;         // tile rows per XCD run of the fused pass (0: identity order)
   dim3 blk_fused, grd_fused;
   TfsfConst* tfc = nullptr; // device copy of the TF/SF constants
@@ -561,6 +563,9 @@ void release(iris_handle* h) {
   h->pool.clear();
   for (void* p : h->allocs) (void)hipFree(p);
   h->allocs.clear();
+  if (h->plane_buf) (void)hipFree(h->plane_buf);
+  h->plane_buf = nullptr;
+  h->plane_cap = 0;
   if (h->stream) (void)hipStreamDestroy(h->stream);
   h->stream = nullptr;
 }
@@ -717,6 +722,64 @@ int iris_set_field(iris_handle* h, int32_t which, const double* in) {
     char* dst = (char*)(which ? h->fh : h->fe) + h->csize * (size_t)h->cs * c;
     HIP_TRY(h, hipMemcpyAsync(dst, buf.data(), buf.size(), hipMemcpyHostToDevice, h->stream));
     HIP_TRY(h, hipStreamSynchronize(h->stream));
+  }
+  return 0;
+}
+
+int iris_get_plane(iris_handle* h, int32_t what, int32_t axis, int32_t index, const int32_t range[4],
+                   double* out) {
+  if (!h || !range || !out) return fail(h, IRIS_EINVAL, "bad argument");
+  const iris_params& p = h->p;
+  if (what < 0 || what > 8) return fail(h, IRIS_EINVAL, "iris_get_plane: what must be 0..8");
+  if (axis < 1 || axis > 3) return fail(h, IRIS_EINVAL, "iris_get_plane: axis must be 1, 2 or 3");
+  const int n[3] = {p.nx, p.ny, p.nz};
+  const int fa = axis == 1 ? 1 : 0, fb = axis == 3 ? 1 : 2;  // free axes
+  if (index < 1 || index > n[axis - 1] || range[0] < 1 || range[0] > range[1] || range[1] > n[fa] ||
+      range[2] < 1 || range[2] > range[3] || range[3] > n[fb])
+    return fail(h, IRIS_EINVAL, "iris_get_plane: index %d or range [%d,%d]x[%d,%d] outside 1..n of axis %d", index,
+                range[0], range[1], range[2], range[3], axis);
+  const int base = what / 3, comp = what % 3;
+  const void* f;
+  if (base == 0) f = h->fe;
+  else if (base == 2) f = h->fh;
+  else if (h->fused) f = h->gh;  // the fused pass read H(it-1) from here and wrote H(it) into fh
+  else return fail(h, IRIS_EUNSUP, "iris_get_plane: H of the previous step is kept only by the fused pass");
+  const int na = range[1] - range[0] + 1, nb = range[3] - range[2] + 1;
+  const int64_t np = (int64_t)na * nb;
+  if (np > h->plane_cap) {
+    if (h->plane_buf) {
+      HIP_TRY(h, hipStreamSynchronize(h->stream));
+      (void)hipFree(h->plane_buf);
+      h->plane_buf = nullptr;
+      h->plane_cap = 0;
+    }
+    if (hipMalloc(&h->plane_buf, h->csize * (size_t)np) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(h, IRIS_ENOMEM, "iris_get_plane: %lld complex values", (long long)np);
+    }
+    h->plane_cap = np;
+  }
+  const char* src = (const char*)f + h->csize * (size_t)h->cs * comp;
+  const dim3 grid((unsigned)((np + 255) / 256));
+  if (h->prec == 8)
+    hipLaunchKernelGGL(k_plane<double>, grid, dim3(256), 0, h->stream, (const double2*)src, axis, index, range[0], na,
+                       range[2], nb, h->sx, h->sxy, h->xoff, (double2*)h->plane_buf);
+  else
+    hipLaunchKernelGGL(k_plane<float>, grid, dim3(256), 0, h->stream, (const float2*)src, axis, index, range[0], na,
+                       range[2], nb, h->sx, h->sxy, h->xoff, (float2*)h->plane_buf);
+  HIP_TRY(h, hipGetLastError());
+  if (h->prec == 8) {
+    HIP_TRY(h, hipMemcpyAsync(out, h->plane_buf, sizeof(double2) * (size_t)np, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+  } else {
+    std::vector<float2> buf((size_t)np);
+    HIP_TRY(h, hipMemcpyAsync(buf.data(), h->plane_buf, sizeof(float2) * (size_t)np, hipMemcpyDeviceToHost,
+                              h->stream));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    for (int64_t i = 0; i < np; ++i) {
+      out[2 * i] = buf[(size_t)i].x;
+      out[2 * i + 1] = buf[(size_t)i].y;
+    }
   }
   return 0;
 }

@@ -167,6 +167,22 @@ __global__ __launch_bounds__(256) void k_dft_fold(BoxGeom g, const S* __restrict
   }
 }
 
+// One field component on one plane: out[a + na*b] = f(x, y, z) with the
+// fixed coordinate `index` on `axis` (1: x, 2: y, 3: z) and the two free
+// coordinates a0 + a, b0 + b in (x, y, z) order.
+template <typename T>
+__global__ void k_plane(const cplx<T>* __restrict__ f, int axis, int index, int a0, int na, int b0, int nb,
+                        int64_t sx, int64_t sxy, int xoff, cplx<T>* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)na * nb) return;
+  const int ia = a0 + (int)(t % na), ib = b0 + (int)(t / na);
+  int x, y, z;
+  if (axis == 1) { x = index; y = ia; z = ib; }
+  else if (axis == 2) { x = ia; y = index; z = ib; }
+  else { x = ia; y = ib; z = index; }
+  out[t] = f[(int64_t)(x + xoff) + sx * y + sxy * z];
+}
+
 // Face fluxes, reference code/dataprocess.f90:241-271: block = (pair, iw,
 // half); writes the unscaled sum with the reference's sign into
 // out[iw*6 + face].

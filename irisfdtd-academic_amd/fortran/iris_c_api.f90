@@ -9,6 +9,8 @@ module iris_c_api
   implicit none
 
   integer(c_int32_t), parameter :: IRIS_ABI_VERSION = 1
+  ! iris_get_plane selectors: base + component (0: x, 1: y, 2: z)
+  integer(c_int32_t), parameter :: IRIS_PLANE_E = 0, IRIS_PLANE_H_PREV = 3, IRIS_PLANE_H = 6
 
   type, bind(C) :: iris_params_t
     integer(c_int32_t) :: abi_version
@@ -64,6 +66,15 @@ module iris_c_api
       type(c_ptr), value :: h
       integer(c_int32_t), value :: nobox
       type(c_ptr), value :: jabs
+    end function
+
+    ! one field component on one plane (field snapshots, reference code/sdl.f90)
+    integer(c_int) function iris_get_plane(h, what, axis, index, range, out) bind(C, name="iris_get_plane")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: what, axis, index
+      integer(c_int32_t), intent(in) :: range(4)
+      type(c_ptr), value :: out
     end function
 
     integer(c_int) function iris_destroy(h) bind(C, name="iris_destroy")

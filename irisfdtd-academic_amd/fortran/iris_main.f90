@@ -13,17 +13,20 @@
 ! list-directed formatting as scattering_fourier_out
 ! (reference code/dataprocess.f90:279-299).
 !
-! Field snapshots (SDL planes, reference code/sdl.f90) are not produced yet;
-! a run that asks for them says so in log.dat and continues.
+! At the same output steps the field snapshots of the enabled SDL planes
+! (reference code/driver.f90:468-495, code/sdl.f90) are written by the
+! iris_sdl writer from planes fetched with iris_get_plane.
 program irisfdtd
   use iso_c_binding
   use iris_c_api
   use iris_setup
+  use iris_sdl
   implicit none
   character(len=100) :: first
   integer :: u, ios, io_main, if_main, i, status, prec_default
   real(8) :: dxl, dyl, dzl
   character(len=16) :: envp
+  integer, parameter :: SNAPSHOT_IO_ERROR = -20
 
   prec_default = 8
   call get_environment_variable('IRIS_PRECISION', envp, status=status)
@@ -101,7 +104,6 @@ contains
       return
     end if
     call iris_log_parameters(run, lg)
-    if (sum(run%sdl_plane_on) /= 0) write(lg, *) '...SDL field snapshots requested: not produced by this build'
     call write_spectra(run)
 
     rc = iris_create(run%p, c_loc(run%eps_inv), c_loc(run%fpzmn), c_loc(run%fmzmn), &
@@ -145,13 +147,22 @@ contains
           call write_flux(nb, stop_at, jabs)
         end do
         if (rc /= 0) exit
+        if (sdl_enabled(run%sdl)) then
+          call write_snapshots(h, run, nos + 1, rc, errmsg)
+          if (rc /= 0) exit
+        end if
         nos = nos + 1
       end if
       it = stop_at + 1
     end do
     if (rc /= 0) then
-      write(lg, *) '...Error -> ', iris_error_text(h)
-      write(*, '(A)') 'irisfdtd: '//iris_error_text(h)
+      if (rc == SNAPSHOT_IO_ERROR) then
+        write(lg, *) '...Error -> ', trim(errmsg)
+        write(*, '(A)') 'irisfdtd: '//trim(errmsg)
+      else
+        write(lg, *) '...Error -> ', iris_error_text(h)
+        write(*, '(A)') 'irisfdtd: '//iris_error_text(h)
+      end if
       status = rc
     end if
     call system_clock(c1)
@@ -162,6 +173,51 @@ contains
     rc = iris_destroy(h)
     close(lg)
   end subroutine run_once
+
+  ! Snapshot files of every enabled plane and layer for output number ino
+  ! (reference code/driver.f90:468-495): E after the step and H of the step
+  ! before, fetched plane by plane.
+  subroutine write_snapshots(h, run, ino, rc, errmsg)
+    type(c_ptr), intent(in) :: h
+    type(iris_run), intent(in) :: run
+    integer, intent(in) :: ino
+    integer, intent(out) :: rc
+    character(len=*), intent(out) :: errmsg
+    complex(8), allocatable, target :: e(:,:,:), hp(:,:,:)
+    integer :: layer, il, c, fx, a0, a1, b0, b1, ierr
+    integer(c_int32_t) :: rng(4)
+    logical :: need_e, need_h
+    rc = 0
+    errmsg = ''
+    need_e = run%sdl%module_e == 1 .or. any(run%sdl%field(1:3) >= 1)
+    need_h = run%sdl%module_e == 2 .or. any(run%sdl%field(4:6) >= 1)
+    do layer = 1, 3
+      if (run%sdl%plane_on(layer) /= 1) cycle
+      do il = 1, run%sdl%nlayers(layer)
+        call sdl_geometry(run%sdl, layer, il, fx, a0, a1, b0, b1)
+        allocate(e(a0:a1, b0:b1, 3), hp(a0:a1, b0:b1, 3))
+        e = (0.0d0, 0.0d0)
+        hp = (0.0d0, 0.0d0)
+        rng = [a0, a1, b0, b1]
+        do c = 1, 3
+          if (need_e) then
+            rc = iris_get_plane(h, IRIS_PLANE_E + c - 1, layer, fx, rng, c_loc(e(a0, b0, c)))
+            if (rc /= 0) return
+          end if
+          if (need_h) then
+            rc = iris_get_plane(h, IRIS_PLANE_H_PREV + c - 1, layer, fx, rng, c_loc(hp(a0, b0, c)))
+            if (rc /= 0) return
+          end if
+        end do
+        call sdl_write_plane(run%sdl, layer, fx, a0, b0, ino, e, hp, run%p%q*run%p%eamp, ierr, errmsg)
+        deallocate(e, hp)
+        if (ierr /= 0) then
+          rc = SNAPSHOT_IO_ERROR
+          return
+        end if
+      end do
+    end do
+  end subroutine write_snapshots
 
   ! <box>_scatt<j>.dat and <box>_scatt.dat (reference code/dataprocess.f90:279-299)
   subroutine write_flux(nobox, it, jabs)

@@ -10,6 +10,8 @@
 !   UPML coefficient tables  reference code/boundary_II.f90:224-276 (set_sz)
 !   TF/SF ranges, dtcq2      reference code/driver.f90:310-335
 !   analytic film spectrum   reference code/common.f90:104-131
+!   snapshot options         reference code/sdl.f90:868-897 (setparam_SDL),
+!                            code/files.f90:291-319 (extra layers)
 !
 ! Differences from the reference, by design: errors are returned as a code
 ! plus message instead of STOP (reference code/files.f90:352-381), and an
@@ -22,6 +24,7 @@
 module iris_setup
   use iso_c_binding
   use iris_c_api, only: iris_params_t, IRIS_ABI_VERSION
+  use iris_sdl, only: sdl_config, sdl_geometry, sdl_write_plane
   implicit none
   private
 
@@ -52,6 +55,7 @@ module iris_setup
     integer :: pml_taflove, output_time_step, no_downloads_in
     integer :: dh, dx, aix, afx, tfsf_zi
     integer :: sdl_plane_on(3)
+    type(sdl_config) :: sdl                     ! snapshot options (read when a plane is on)
     integer :: ngrid_eps                        ! elements of eps_inv
     character(len=100) :: description
     complex(dp), allocatable :: eps_inv(:,:,:)  ! (0:nx+1, 0:ny+1, hi-1:hf+1)
@@ -256,6 +260,57 @@ contains
     hf = (hi + run%dh) - 1
     read(u, *, iostat=ios) run%dx;           if (bad(ios, 'dx')) return
     call slit_limits(nx, run%dx, run%aix, run%afx)
+
+    ! ---- snapshot options, read when any plane is on (code/files.f90:151-154,
+    ! code/driver.f90:289-292 -> setparam_SDL code/sdl.f90:868-897)
+    run%sdl%plane_on = run%sdl_plane_on
+    if (sum(run%sdl_plane_on) /= 0) then
+      call find_section(u, 'Time evolution options', found)
+      if (.not. found) then
+        call fail('SDL planes are on but section "Time evolution options" is missing'); return
+      end if
+      read(u, *, iostat=ios) run%sdl%copy_flag, run%sdl%copy_no; if (bad(ios, 'SDL_copy_flag')) return
+      read(u, *, iostat=ios) run%sdl%xi;        if (bad(ios, 'SDL sxi')) return
+      read(u, *, iostat=ios) run%sdl%xf;        if (bad(ios, 'SDL sxf')) return
+      read(u, *, iostat=ios) run%sdl%yi;        if (bad(ios, 'SDL syi')) return
+      read(u, *, iostat=ios) run%sdl%yf;        if (bad(ios, 'SDL syf')) return
+      read(u, *, iostat=ios) run%sdl%zi;        if (bad(ios, 'SDL szi')) return
+      read(u, *, iostat=ios) run%sdl%zf;        if (bad(ios, 'SDL szf')) return
+      read(u, *, iostat=ios) run%sdl%module_e;  if (bad(ios, 'SDL E')) return
+      do i = 1, 6
+        read(u, *, iostat=ios) run%sdl%field(i); if (bad(ios, 'SDL field flags')) return
+      end do
+      if (run%sdl%copy_flag /= 0 .and. run%sdl%copy_no < 1) then
+        call fail('SDL_copy_no must be >= 1'); return
+      end if
+      ! optional extra layers (code/files.f90:291-319)
+      run%sdl%nlayers = 1
+      call find_section(u, 'Extra parameters-SDLextended', found)
+      if (found) then
+        read(u, *, iostat=ios) run%sdl%nlayers; if (bad(ios, 'SDL extra layer counts')) return
+        if (any(run%sdl%nlayers < 1)) then; call fail('SDL extra layer counts must be >= 1'); return; end if
+      end if
+      allocate(run%sdl%delta(maxval(run%sdl%nlayers), 3))
+      run%sdl%delta = 0
+      if (found) then
+        do iy = 1, 3
+          do i = 1, run%sdl%nlayers(iy)
+            read(u, *, iostat=ios) run%sdl%delta(i, iy); if (bad(ios, 'SDL layer offset')) return
+          end do
+        end do
+      end if
+      ! every enabled plane must lie inside the grid (the reference would read
+      ! ghost or out-of-range cells)
+      do iy = 1, 3
+        if (run%sdl%plane_on(iy) /= 1) cycle
+        do i = 1, run%sdl%nlayers(iy)
+          if (.not. plane_ok(iy, i)) then
+            write(ebuf, '(A,I0,A,I0,A)') 'SDL plane ', iy, ' layer ', i, ' lies outside the grid'
+            call fail(trim(ebuf)); return
+          end if
+        end do
+      end do
+    end if
     close(u)
 
     ! ---- consistency checks the engine relies on (the reference would run
@@ -376,6 +431,20 @@ contains
       if (bad) call fail('cannot read '//what)
     end function bad
 
+    logical function plane_ok(layer, ilayer)
+      integer, intent(in) :: layer, ilayer
+      integer :: fx, a0, a1, b0, b1, n(3), na, nb
+      n = [nx, ny, nz]
+      call sdl_geometry(run%sdl, layer, ilayer, fx, a0, a1, b0, b1)
+      select case (layer)
+      case (1); na = ny; nb = nz
+      case (2); na = nx; nb = nz
+      case default; na = nx; nb = ny
+      end select
+      plane_ok = fx >= 1 .and. fx <= n(layer) .and. a0 >= 1 .and. a0 <= a1 .and. a1 <= na &
+                 .and. b0 >= 1 .and. b0 <= b1 .and. b1 <= nb
+    end function plane_ok
+
     logical function box_ok(a, b, c, d, e, f)
       integer, intent(in) :: a, b, c, d, e, f
       box_ok = a >= 1 .and. a <= b .and. b <= nx .and. c >= 1 .and. c <= d .and. d <= ny &
@@ -494,6 +563,53 @@ contains
             sum(c_run%sdl_plane_on), c_run%tfsf_zi]
     iris_host_extra = 0
   end function iris_host_extra
+
+  ! Snapshot options of the last loaded run: iout = plane_on(3), copy_flag,
+  ! copy_no, module_e, field(6), nlayers(3), then 1 when any plane is on.
+  integer(c_int) function iris_host_sdl_config(iout) bind(C, name="iris_host_sdl_config")
+    integer(c_int32_t), intent(out) :: iout(16)
+    iout = [c_run%sdl%plane_on, c_run%sdl%copy_flag, c_run%sdl%copy_no, c_run%sdl%module_e, &
+            c_run%sdl%field, c_run%sdl%nlayers, merge(1, 0, any(c_run%sdl%plane_on == 1))]
+    iris_host_sdl_config = 0
+  end function iris_host_sdl_config
+
+  ! Plane of layer `ilayer` of snapshot plane `layer`: geo = fixed
+  ! coordinate, a0, a1, b0, b1 (free ranges).
+  integer(c_int) function iris_host_sdl_geometry(layer, ilayer, geo) bind(C, name="iris_host_sdl_geometry")
+    integer(c_int32_t), value :: layer, ilayer
+    integer(c_int32_t), intent(out) :: geo(5)
+    integer :: fx, a0, a1, b0, b1
+    if (layer < 1 .or. layer > 3 .or. ilayer < 1 .or. ilayer > c_run%sdl%nlayers(max(1, min(3, layer)))) then
+      iris_host_sdl_geometry = -1
+      return
+    end if
+    call sdl_geometry(c_run%sdl, int(layer), int(ilayer), fx, a0, a1, b0, b1)
+    geo = [fx, a0, a1, b0, b1]
+    iris_host_sdl_geometry = 0
+  end function iris_host_sdl_geometry
+
+  ! Write the snapshot files of one plane of the last loaded run into the
+  ! current directory, from caller-supplied fields on that plane: e and hp
+  ! are complex (na, nb, 3) (E after the step, H of the step before), ino the
+  ! output number.  Lets a caller without the engine (the tests, with the
+  ! oracle's fields) drive the same writer as the program.
+  integer(c_int) function iris_host_sdl_write(layer, ilayer, ino, e, hp) bind(C, name="iris_host_sdl_write")
+    integer(c_int32_t), value :: layer, ilayer, ino
+    type(c_ptr), value :: e, hp
+    complex(c_double_complex), pointer :: pe(:, :, :), ph(:, :, :)
+    integer :: fx, a0, a1, b0, b1, ierr
+    character(len=200) :: msg
+    if (layer < 1 .or. layer > 3 .or. ilayer < 1 .or. ilayer > c_run%sdl%nlayers(max(1, min(3, layer)))) then
+      iris_host_sdl_write = -1
+      return
+    end if
+    call sdl_geometry(c_run%sdl, int(layer), int(ilayer), fx, a0, a1, b0, b1)
+    call c_f_pointer(e, pe, [a1 - a0 + 1, b1 - b0 + 1, 3])
+    call c_f_pointer(hp, ph, [a1 - a0 + 1, b1 - b0 + 1, 3])
+    call sdl_write_plane(c_run%sdl, int(layer), fx, a0, b0, int(ino), pe, ph, c_run%p%q*c_run%p%eamp, ierr, msg)
+    if (ierr /= 0) c_errmsg = msg
+    iris_host_sdl_write = ierr
+  end function iris_host_sdl_write
 
   ! Message of the last failed iris_host_load, NUL-terminated into buf.
   integer(c_int) function iris_host_error(buf, buflen) bind(C, name="iris_host_error")

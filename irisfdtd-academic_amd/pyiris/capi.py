@@ -89,6 +89,8 @@ def load_library(path=None):
     lib.iris_get_field.restype = ct.c_int
     lib.iris_set_field.argtypes = [H, ct.c_int32, ct.c_void_p]
     lib.iris_set_field.restype = ct.c_int
+    lib.iris_get_plane.argtypes = [H, ct.c_int32, ct.c_int32, ct.c_int32, ct.c_void_p, ct.c_void_p]
+    lib.iris_get_plane.restype = ct.c_int
     lib.iris_current_step.argtypes = [H, ct.POINTER(ct.c_int32)]
     lib.iris_current_step.restype = ct.c_int
     lib.iris_destroy.argtypes = [H]
@@ -106,8 +108,8 @@ def load_library(path=None):
 
 
 EXPORTED = ["iris_create", "iris_step", "iris_sync", "iris_flux", "iris_dft_points", "iris_get_dft",
-            "iris_get_field", "iris_set_field", "iris_current_step", "iris_destroy", "iris_last_error",
-            "iris_version", "iris_profile", "iris_profile_read"]
+            "iris_get_field", "iris_set_field", "iris_get_plane", "iris_current_step", "iris_destroy",
+            "iris_last_error", "iris_version", "iris_profile", "iris_profile_read"]
 
 KERNELS = ["epass", "hpass", "tfsf", "dft_sample", "dft_accum", "flux", "yee"]
 
@@ -175,6 +177,21 @@ class Engine:
         out = np.zeros(2 * 3 * (p.nx + 2) * (p.ny + 2) * (p.nz + 2), dtype=np.float64)
         self._check(self.lib.iris_get_field(self._h, int(which), _p(out)), "iris_get_field")
         return out.view(np.complex128).reshape(p.nz + 2, p.ny + 2, p.nx + 2, 3)
+
+    # iris_get_plane selectors: base + component (0: x, 1: y, 2: z)
+    PLANE_E, PLANE_H_PREV, PLANE_H = 0, 3, 6
+
+    def plane(self, what, axis, index, a, b):
+        """One field component on one plane as complex (nb, na): axis 1 is
+        x = index over (y, z), 2 is y = index over (x, z), 3 is z = index over
+        (x, y); a = (a0, a1) and b = (b0, b1) are the inclusive ranges of the
+        two free coordinates.  what = PLANE_E/PLANE_H_PREV/PLANE_H + component."""
+        na, nb = a[1] - a[0] + 1, b[1] - b[0] + 1
+        rng = (ct.c_int32 * 4)(a[0], a[1], b[0], b[1])
+        out = np.zeros(2 * max(na, 0) * max(nb, 0), dtype=np.float64)
+        self._check(self.lib.iris_get_plane(self._h, int(what), int(axis), int(index), rng, _p(out)),
+                    "iris_get_plane")
+        return out.view(np.complex128).reshape(nb, na)
 
     def set_field(self, which, arr):
         a = np.ascontiguousarray(arr, dtype=np.complex128)

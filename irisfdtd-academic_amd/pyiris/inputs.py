@@ -15,7 +15,17 @@ DEFAULTS = dict(
     box=(1, 1, 1, 1, 25, 1000), mbox=None,
     nof=200, dlambda=600.0 / 199.0, lno=400.0, to_fourier=0, sample_fourier=1,
     dh=20, dx=0, precision=None,
+    sdl_opts=None,
 )
+
+# Snapshot options ("Time evolution options", reference code/sdl.f90:868-897):
+# per axis X, Y, Z the ranges (xi, xf, yi, yf, zi, zf) -- only the ones of
+# enabled planes matter -- the modulus flag E (1: |E|, 2: |H|) and the six
+# component modes (Ex..Hz; 0 off, 1 Re, 2 Im, 3 |F|, 4 |Re|, 5 |Im|).
+# layers = ((nX, nY, nZ), [dX...], [dY...], [dZ...]) writes the optional
+# "Extra parameters-SDLextended" section (reference code/files.f90:291-319).
+SDL_DEFAULTS = dict(copy=(0, 1), xi=(1, 1, 1), xf=(1, 1, 1), yi=(1, 1, 1), yf=(1, 1, 1), zi=(1, 1, 1),
+                    zf=(1, 1, 1), module_e=1, fields=(0, 0, 0, 0, 0, 0), layers=None)
 
 # dt(fs) of a run: CFL/(c0*sqrt(3/Q^2)) * uat with the reference's constants;
 # used to turn a step count into the itmax (fs) entry so that
@@ -93,6 +103,25 @@ def render(**kw):
     ]
     if c["precision"] is not None:
         lines += ['"Engine options"', f"{c['precision']}\tprecision (8: FP64 fields, 4: FP32 fields)", ""]
+    if c["sdl_opts"] is not None:
+        o = dict(SDL_DEFAULTS)
+        unknown = set(c["sdl_opts"]) - set(o)
+        if unknown:
+            raise KeyError(f"unknown SDL keys: {sorted(unknown)}")
+        o.update(c["sdl_opts"])
+        lines += ['"Time evolution options"', f"{o['copy'][0]} {o['copy'][1]}\tSDL_copy_flag,SDL_copy_no"]
+        for key in ("xi", "xf", "yi", "yf", "zi", "zf"):
+            v = o[key]
+            lines.append(f"{v[0]} {v[1]} {v[2]}\ts{key}")
+        lines.append(f"{o['module_e']}\tE")
+        for name, v in zip(("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"), o["fields"]):
+            lines.append(f"{v}\t{name}")
+        lines.append("")
+        if o["layers"] is not None:
+            counts, dxs, dys, dzs = o["layers"]
+            lines += ['"Extra parameters-SDLextended"', f"{counts[0]} {counts[1]} {counts[2]}"]
+            lines += [str(d) for d in list(dxs) + list(dys) + list(dzs)]
+            lines.append("")
     lines += [
         '"Slit Array"',
         f"{c['dh']}\tDh",

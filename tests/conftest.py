@@ -12,7 +12,7 @@ for p in (PKG, os.path.join(ROOT, "oracle")):
         sys.path.insert(0, p)
 
 GOLDEN_CASES = ["example_1", "example_2_short", "cfg1_short", "small3d_ipol2", "small3d_ipol1_cos_2box",
-                "slit2d_x", "slit2d_y", "substrate3d"]
+                "slit2d_x", "slit2d_y", "substrate3d", "sdl3d", "sdl3d_hmod"]
 
 
 def pytest_configure(config):

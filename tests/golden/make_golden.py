@@ -3,14 +3,17 @@
 
 The reference (IrisFDTD-Academic's Fortran, built unmodified in numerics by
 oracle/patch_ref.py into oracle/_ref/irisfdtd_ref) is run on each input below;
-its output files (<box>_scatt<j>.dat blocks) and the opt-in dump of its DFT
-accumulators are stored as <case>.npz next to the input text <case>.dat.
+its output files (<box>_scatt<j>.dat blocks), the opt-in dump of its DFT
+accumulators and the bytes of every field-snapshot file it writes
+(X_SDL*/Y_SDL*/Z_SDL*, key "sdl__<file name>") are stored as <case>.npz next
+to the input text <case>.dat.
 
 Inputs: example_1 and example_2 are the reference's own example inputs
 (example_*/inputFDTD.dat), example_2 shortened to 1000 steps; the others are
 small grids chosen to cover the paths of the loop (3D with both
 polarisations, real source, two Fourier boxes, 2D slit arrays along x and y,
-a substrate different from the cover, config-1 geometry).
+a substrate different from the cover, config-1 geometry, field snapshots on
+all three planes with extra layers, copies and every value mode).
 
 Usage: python tests/golden/make_golden.py [case ...]   (needs the built
 reference binary; run `make -C oracle ref` first)
@@ -58,6 +61,23 @@ CASES = {
     "slit2d_y": (lambda: inputs.render(
         ixmax=1, iymax=12, izmax=70, pmlzi=10, pmlzf=60, hi=30, dh=6, dx=0, eps_bbox=6.0, eps1=2.25, tfsf_o=16,
         box=(1, 1, 2, 11, 12, 55), nof=5, nt=250, no_downloads=2), [2, 3]),
+    # snapshots: X planes x=3,5 over y 1..6, z 5..40; Y plane y=4 over x 1..8,
+    # z 10..30; Z planes z=30,20 over x 1..8, y 1..6; |E|; Ex Re, Ey Im, Ez
+    # |E|; H components on (each takes the mode of the same-index E
+    # component); two copies (both axes on Z planes)
+    "sdl3d": (lambda: inputs.render(
+        ixmax=8, iymax=6, izmax=64, pmlzi=8, pmlzf=56, hi=28, dh=6, dx=3, eps_bbox=4.0, tfsf_o=14,
+        box=(1, 8, 1, 6, 10, 50), nof=3, nt=120, no_downloads=3, sdl=(1, 1, 1),
+        sdl_opts=dict(copy=(1, 2), xi=(3, 1, 1), xf=(3, 8, 8), yi=(1, 4, 1), yf=(6, 4, 6), zi=(5, 10, 30),
+                      zf=(40, 30, 30), module_e=1, fields=(1, 2, 3, 1, 1, 1),
+                      layers=((2, 1, 2), [0, 2], [0], [0, -10]))), []),
+    # |H|, |Re| and |Im| modes, ranges not starting at 1, no copies, one layer
+    # per plane, x-polarised source
+    "sdl3d_hmod": (lambda: inputs.render(
+        i_pol=1, ixmax=7, iymax=5, izmax=48, pmlzi=8, pmlzf=40, hi=22, dh=4, dx=2, eps_bbox=2.25, tfsf_o=12,
+        box=(1, 7, 1, 5, 10, 38), nof=2, nt=90, no_downloads=2, sdl=(1, 0, 1),
+        sdl_opts=dict(copy=(0, 1), xi=(4, 1, 2), xf=(4, 1, 6), yi=(2, 1, 2), yf=(5, 1, 4), zi=(8, 1, 25),
+                      zf=(36, 1, 25), module_e=2, fields=(4, 5, 1, 1, 1, 0))), []),
     "substrate3d": (lambda: inputs.render(
         ixmax=6, iymax=6, izmax=72, pmlzi=10, pmlzf=62, hi=30, dh=8, dx=2, eps_bbox=2.5, tfsf_o=16, eps3=2.0,
         box=(1, 6, 1, 6, 12, 55), nof=6, nt=240, no_downloads=4, lno=450.0, dlambda=90.0), [3]),
@@ -85,6 +105,10 @@ def make(name):
                 out[f"emg_b{b}_p{pair}"] = pyoracle.read_emg_dump(os.path.join(run, f"emg{b}_{pair}.bin"))
         spec = np.loadtxt(os.path.join(run, "dielectric_layer_spectra.dat"))
         out["film_spectrum"] = spec
+        for fn in sorted(os.listdir(run)):
+            if fn[:6] in ("X_SDL", "Y_SDL", "Z_SDL") or fn[1:5] == "_SDL":
+                with open(os.path.join(run, fn), "rb") as f:
+                    out["sdl__" + fn] = np.frombuffer(f.read(), dtype=np.uint8)
     np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
     print(name, {k: v.shape for k, v in out.items()})
 

@@ -189,6 +189,81 @@ def test_fortran_program_end_to_end(tmp_path):
     np.testing.assert_allclose(np.stack([v for _, v in tot]), g["jtot_b1"], rtol=1e-9, atol=1e-20)
     spec = np.loadtxt(tmp_path / "dielectric_layer_spectra.dat")
     np.testing.assert_allclose(spec, g["film_spectrum"], rtol=1e-14)
+    _check_snapshots(tmp_path, g, "example_1")
+
+
+def _check_snapshots(outdir, g, name):
+    """Every snapshot file the reference wrote exists with the same record
+    framing and the same float32 values (FP64 fields; values within 1e-6 of
+    the plane's largest value -- float32 output of FP64 fields that agree to
+    ~1e-15 differs at most by a rounding flip)."""
+    from pyiris.sdl import read_record
+    want = {k[len("sdl__"):]: bytes(v) for k, v in g.items() if k.startswith("sdl__")}
+    got = sorted(f for f in os.listdir(outdir) if f[1:5] == "_SDL")
+    assert got == sorted(want), f"{name}: snapshot file names differ"
+    nexact = 0
+    for fn, raw in want.items():
+        v, graw = read_record(os.path.join(outdir, fn))
+        assert len(graw) == len(raw) and graw[:4] == raw[:4] and graw[-4:] == raw[-4:], f"{name}: {fn} framing"
+        ref = np.frombuffer(raw[4:-4], dtype="<f4")
+        scale = max(float(np.abs(ref).max()), 1e-30)
+        err = float(np.abs(v.astype(np.float64) - ref).max()) / scale
+        assert err <= 1e-6, f"{name}: {fn} max rel err {err:.3e}"
+        nexact += graw == raw
+    _announce(f"{name}: {nexact}/{len(want)} snapshot files bitwise equal")
+
+
+@pytest.mark.parametrize("name", ["sdl3d", "sdl3d_hmod"])
+def test_fortran_program_snapshots(tmp_path, name):
+    """bin/irisfdtd writes the reference's field snapshots (all three planes,
+    extra layers, copies, every value mode) from planes fetched with
+    iris_get_plane."""
+    _announce(f"fortran program snapshots {name}")
+    exe = os.path.join(PKG, "bin", "irisfdtd")
+    shutil.copy(golden_path(name, "dat"), tmp_path / "inputFDTD.dat")
+    subprocess.run([exe], cwd=tmp_path, check=True, timeout=300, stdout=subprocess.DEVNULL)
+    _check_snapshots(tmp_path, load_golden(name), name)
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_get_plane_matches_oracle(tmp_path, fused):
+    """iris_get_plane on all three axes against the oracle's fields: E and H
+    after step n, H after step n-1 (fused pass only; the in-place passes
+    report IRIS_EUNSUP for it)."""
+    _announce(f"get_plane fused={fused}")
+    s = _small_setup(tmp_path, nt=60)
+    n = 37
+    o = pyoracle.Oracle(s)
+    o.run_to(n - 1)
+    h_prev = o.field(1)
+    o.run_to(n)
+    e, h = o.field(0), o.field(1)
+    o.close()
+    p = s.params
+    geos = [(1, 4, (2, 6), (5, 70)), (2, 3, (1, 9), (12, 40)), (3, 36, (2, 9), (1, 7)), (3, 1, (1, 1), (7, 7))]
+    with _engine_with(s, 8, fused) as eng:
+        eng.run_to(n)
+        for axis, idx, a, b in geos:
+            for base, ref in [(Engine.PLANE_E, e), (Engine.PLANE_H, h), (Engine.PLANE_H_PREV, h_prev)]:
+                for c in range(3):
+                    if base == Engine.PLANE_H_PREV and not fused:
+                        with pytest.raises(IrisError):
+                            eng.plane(base + c, axis, idx, a, b)
+                        continue
+                    got = eng.plane(base + c, axis, idx, a, b)
+                    if axis == 1:
+                        want = ref[b[0]:b[1] + 1, a[0]:a[1] + 1, idx, c]
+                    elif axis == 2:
+                        want = ref[b[0]:b[1] + 1, idx, a[0]:a[1] + 1, c]
+                    else:
+                        want = ref[idx, b[0]:b[1] + 1, a[0]:a[1] + 1, c]
+                    scale = max(np.abs(ref[1:p.nz + 1, 1:p.ny + 1, 1:p.nx + 1, :]).max(), 1e-300)
+                    err = np.abs(got - want).max() / scale
+                    assert err <= 1e-12, f"axis {axis} base {base} comp {c}: {err:.3e}"
+        with pytest.raises(IrisError):
+            eng.plane(Engine.PLANE_E, 1, p.nx + 1, (1, 2), (1, 2))  # ghost plane
+        with pytest.raises(IrisError):
+            eng.plane(Engine.PLANE_E, 3, 5, (3, 2), (1, 2))          # reversed range
 
 
 def _engine_with(setup, precision, fused, ak=None):
