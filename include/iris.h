@@ -19,6 +19,7 @@
  *   iris_get_field<- e_cur / h_cur arrays (code/main.f90:166-167)
  *   iris_get_plane<- the field reads of the snapshot writer (code/sdl.f90:210-313,
  *                    called with e_cur, h_prev at code/driver.f90:468-495)
+ *   iris_diagnostics <- energy and divergence monitors (code/energy.f90:57-201)
  *   iris_destroy  <- driver teardown (code/driver.f90:503-552)
  *
  * Conventions: every entry point returns 0 on success and a negative
@@ -151,6 +152,22 @@ int iris_set_field(iris_handle* h, int32_t which, const double* in);
  * engine overwrites H; IRIS_EUNSUP).  Synchronous. */
 int iris_get_plane(iris_handle* h, int32_t what, int32_t axis, int32_t index, const int32_t range[4],
                    double* out);
+
+/* Field monitors after the step most recently completed, over x in 1..nx,
+ * y in 1..ny, z in zi..zf (within pmlzi+1..pmlzf-1, outside the UPML):
+ *   out[0] = sum Re(eps_i conj(E_i^n) E_i^(n-1))    (electric part)
+ *   out[1] = sum |H_i^(n-1)|^2 / dtcq2               (magnetic part)
+ *   out[2] = max |div D|, div D = sum_i eps_i E_i^n(c) - eps_i E_i^n(c - e_i)
+ *   out[3] = max |div B|, div B = sum_i H_i^n(c + e_i) - H_i^n(c)
+ * eps_i is the inverse of the factor the E update applies (real part for a
+ * complex eps_inv); neighbours across the x/y period take the Bloch factor.
+ * out[0]+out[1] and both divergences are invariants of the update in a
+ * lossless, source-free region.  Replaces the reference's diagnostics
+ * calc_energy_density, calc_div_D, calc_div_B (code/energy.f90:57-201, not
+ * called by the reference; its energy routine indexes eps_inv(i,i,...)
+ * outside the array and its div D keeps only real parts without eps).
+ * Needs the fused pass (IRIS_EUNSUP otherwise).  Synchronous. */
+int iris_diagnostics(iris_handle* h, int32_t zi, int32_t zf, double out[4]);
 
 /* The step most recently completed (0 before the first step). */
 int iris_current_step(const iris_handle* h, int32_t* it);

@@ -71,6 +71,7 @@ struct iris_handle {
   void* bn = nullptr;
   int64_t ps0 = 0, ps1 = 0, pb1 = 0;
   bool fused = true;        // fused E+H pass over double-buffered fields
+  double* diag_buf = nullptr; // partials and result of iris_diagnostics
   void* plane_buf = nullptr; // staging of iris_get_plane
   int64_t plane_cap = 0;      // its capacity in complex values
   int xcd_rows = 0;         // tile rows per XCD run of the fused pass (0: identity order)
@@ -563,6 +564,8 @@ void release(iris_handle* h) {
   h->pool.clear();
   for (void* p : h->allocs) (void)hipFree(p);
   h->allocs.clear();
+  if (h->diag_buf) (void)hipFree(h->diag_buf);
+  h->diag_buf = nullptr;
   if (h->plane_buf) (void)hipFree(h->plane_buf);
   h->plane_buf = nullptr;
   h->plane_cap = 0;
@@ -781,6 +784,37 @@ int iris_get_plane(iris_handle* h, int32_t what, int32_t axis, int32_t index, co
       out[2 * i + 1] = buf[(size_t)i].y;
     }
   }
+  return 0;
+}
+
+int iris_diagnostics(iris_handle* h, int32_t zi, int32_t zf, double out[4]) {
+  if (!h || !out) return fail(h, IRIS_EINVAL, "bad argument");
+  const iris_params& p = h->p;
+  if (zi < p.pmlzi + 1 || zf > p.pmlzf - 1 || zi > zf)
+    return fail(h, IRIS_EINVAL, "iris_diagnostics: z range [%d,%d] must lie in [pmlzi+1, pmlzf-1] = [%d,%d]", zi,
+                zf, p.pmlzi + 1, p.pmlzf - 1);
+  if (!h->fused) return fail(h, IRIS_EUNSUP, "iris_diagnostics: E and H of the previous step are kept only by the fused pass");
+  if (!h->diag_buf) {
+    if (hipMalloc((void**)&h->diag_buf, sizeof(double) * (4 * kDiagBlocks + 4)) != hipSuccess) {
+      (void)hipGetLastError();
+      h->diag_buf = nullptr;
+      return fail(h, IRIS_ENOMEM, "iris_diagnostics: partials");
+    }
+  }
+  double* res = h->diag_buf + 4 * kDiagBlocks;
+  if (h->prec == 8) {
+    const StepArgs<double> a = step_args<double>(h);
+    hipLaunchKernelGGL(k_diag<double>, dim3(kDiagBlocks), dim3(256), 0, h->stream, a, (const double2*)h->ge,
+                       (const double2*)h->gh, zi, zf, h->dtcq2, h->diag_buf);
+  } else {
+    const StepArgs<float> a = step_args<float>(h);
+    hipLaunchKernelGGL(k_diag<float>, dim3(kDiagBlocks), dim3(256), 0, h->stream, a, (const float2*)h->ge,
+                       (const float2*)h->gh, zi, zf, h->dtcq2, h->diag_buf);
+  }
+  hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(256), 0, h->stream, (const double*)h->diag_buf, kDiagBlocks, res);
+  HIP_TRY(h, hipGetLastError());
+  HIP_TRY(h, hipMemcpyAsync(out, res, 4 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
   return 0;
 }
 

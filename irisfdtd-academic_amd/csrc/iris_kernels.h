@@ -183,6 +183,115 @@ __global__ void k_plane(const cplx<T>* __restrict__ f, int axis, int index, int 
   out[t] = f[(int64_t)(x + xoff) + sx * y + sxy * z];
 }
 
+// ---------------------------------------------------------------------------
+// Field monitors (restating the reference's uncalled diagnostics,
+// code/energy.f90:57-201): over x 1..nx, y 1..ny, z zi..zf
+//   W_E = sum Re(eps_i conj(E_i^n) E_i^(n-1)),  W_H = sum |H_i^(n-1)|^2/dtcq2,
+//   max |div D| with div D = sum_i eps_i E_i^n(c) - eps_i E_i^n(c - e_i),
+//   max |div B| with div B = sum_i H_i^n(c + e_i) - H_i^n(c),
+// eps_i being the inverse of the factor the E update applies (real part for a
+// complex eps_inv).  x/y neighbours across the period take the Bloch factor.
+// Grid-stride over the cells with a fixed grid; each block writes its
+// partial (sum, sum, max, max) and k_diag_final reduces them in a fixed
+// order, so the result does not depend on scheduling.
+// ---------------------------------------------------------------------------
+constexpr int kDiagBlocks = 1024;
+
+template <typename T>
+__device__ inline double eps_weight(const StepArgs<T>& a, int i, int x, int y, int z) {
+  if (z < a.hi) return 1.0 / (double)a.a1[i];
+  if (z > a.hf) return 1.0 / (double)a.a3[i];
+  const int64_t idx = (int64_t)(x + a.xoff) + a.sx * y + a.sxy * (z - a.hi);
+  const double er = (double)a.eps_re[idx];
+  const double ei = a.eps_im ? (double)a.eps_im[idx] : 0.0;
+  return er / (er * er + ei * ei);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_diag(StepArgs<T> a, const cplx<T>* __restrict__ ep,
+                                              const cplx<T>* __restrict__ hp, int zi, int zf, double dtcq2,
+                                              double* __restrict__ partial) {
+  using C = cplx<T>;
+  const int64_t n = (int64_t)a.nx * a.ny * (zf - zi + 1);
+  double we = 0.0, wh = 0.0, dd = 0.0, db = 0.0;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+    const int x = (int)(t % a.nx) + 1;
+    const int y = (int)((t / a.nx) % a.ny) + 1;
+    const int z = (int)(t / ((int64_t)a.nx * a.ny)) + zi;
+    const int64_t idx = (int64_t)(x + a.xoff) + a.sx * y + a.sxy * z;
+    double dre = 0.0, dim = 0.0, bre = 0.0, bim = 0.0;
+    for (int i = 0; i < 3; ++i) {
+      const int64_t o = idx + i * a.cs;
+      const double w = eps_weight(a, i, x, y, z);
+      const C ec = a.e[o], epv = ep[o], hpv = hp[o], hc = a.h[o];
+      we += w * ((double)ec.x * (double)epv.x + (double)ec.y * (double)epv.y);
+      wh += ((double)hpv.x * (double)hpv.x + (double)hpv.y * (double)hpv.y) / dtcq2;
+      int bx = x, by = y, bz = z, fx = x, fy = y, fz = z;
+      C fb = mk<T>(1, 0), ff = mk<T>(1, 0);
+      if (i == 0) {
+        bx = x - 1; fx = x + 1;
+        if (bx == 0) { bx = a.nx; fb = a.fxm; }
+        if (fx == a.nx + 1) { fx = 1; ff = a.fxp; }
+      } else if (i == 1) {
+        by = y - 1; fy = y + 1;
+        if (by == 0) { by = a.ny; fb = a.fym; }
+        if (fy == a.ny + 1) { fy = 1; ff = a.fyp; }
+      } else {
+        bz = z - 1; fz = z + 1;
+      }
+      const C eb = cmul(fb, a.e[(int64_t)(bx + a.xoff) + a.sx * by + a.sxy * bz + i * a.cs]);
+      const double wb = eps_weight(a, i, bx, by, bz);
+      dre += w * (double)ec.x - wb * (double)eb.x;
+      dim += w * (double)ec.y - wb * (double)eb.y;
+      const C hf = cmul(ff, a.h[(int64_t)(fx + a.xoff) + a.sx * fy + a.sxy * fz + i * a.cs]);
+      bre += (double)hf.x - (double)hc.x;
+      bim += (double)hf.y - (double)hc.y;
+    }
+    dd = fmax(dd, hypot(dre, dim));
+    db = fmax(db, hypot(bre, bim));
+  }
+  __shared__ double red[4][256];
+  red[0][threadIdx.x] = we;
+  red[1][threadIdx.x] = wh;
+  red[2][threadIdx.x] = dd;
+  red[3][threadIdx.x] = db;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + w];
+      red[1][threadIdx.x] += red[1][threadIdx.x + w];
+      red[2][threadIdx.x] = fmax(red[2][threadIdx.x], red[2][threadIdx.x + w]);
+      red[3][threadIdx.x] = fmax(red[3][threadIdx.x], red[3][threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) partial[blockIdx.x * 4 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(256) void k_diag_final(const double* __restrict__ partial, int nblk,
+                                                    double* __restrict__ out) {
+  __shared__ double red[4][256];
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+    v[0] += partial[b * 4 + 0];
+    v[1] += partial[b * 4 + 1];
+    v[2] = fmax(v[2], partial[b * 4 + 2]);
+    v[3] = fmax(v[3], partial[b * 4 + 3]);
+  }
+  for (int k = 0; k < 4; ++k) red[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + w];
+      red[1][threadIdx.x] += red[1][threadIdx.x + w];
+      red[2][threadIdx.x] = fmax(red[2][threadIdx.x], red[2][threadIdx.x + w]);
+      red[3][threadIdx.x] = fmax(red[3][threadIdx.x], red[3][threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) out[threadIdx.x] = red[threadIdx.x][0];
+}
+
 // Face fluxes, reference code/dataprocess.f90:241-271: block = (pair, iw,
 // half); writes the unscaled sum with the reference's sign into
 // out[iw*6 + face].

@@ -77,6 +77,14 @@ module iris_c_api
       type(c_ptr), value :: out
     end function
 
+    ! energy and divergence monitors (reference code/energy.f90, uncalled there)
+    integer(c_int) function iris_diagnostics(h, zi, zf, out) bind(C, name="iris_diagnostics")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: zi, zf
+      type(c_ptr), value :: out
+    end function
+
     integer(c_int) function iris_destroy(h) bind(C, name="iris_destroy")
       import :: c_int, c_ptr
       type(c_ptr), value :: h

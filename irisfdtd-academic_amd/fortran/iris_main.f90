@@ -15,7 +15,12 @@
 !
 ! At the same output steps the field snapshots of the enabled SDL planes
 ! (reference code/driver.f90:468-495, code/sdl.f90) are written by the
-! iris_sdl writer from planes fetched with iris_get_plane.
+! iris_sdl writer from planes fetched with iris_get_plane.  With
+! IRIS_MONITOR=1 in the environment the program also appends, per output
+! step, the energy and divergence monitors of iris_diagnostics over the
+! region between the UPML slabs to monitor.dat (it, W_E, W_H, W_E+W_H,
+! max |div D|, max |div B|); the reference has these routines
+! (code/energy.f90) but never calls them.
 program irisfdtd
   use iso_c_binding
   use iris_c_api
@@ -27,12 +32,22 @@ program irisfdtd
   real(8) :: dxl, dyl, dzl
   character(len=16) :: envp
   integer, parameter :: SNAPSHOT_IO_ERROR = -20
+  logical :: monitor
 
   prec_default = 8
   call get_environment_variable('IRIS_PRECISION', envp, status=status)
   if (status == 0) then
     read(envp, *, iostat=ios) prec_default
     if (ios /= 0 .or. (prec_default /= 4 .and. prec_default /= 8)) prec_default = 8
+  end if
+
+  monitor = .false.
+  call get_environment_variable('IRIS_MONITOR', envp, status=status)
+  if (status == 0) monitor = envp(1:1) == '1'
+  if (monitor) then
+    ! one file per program invocation; the runs of a <<LOOP>> append to it
+    open(newunit=u, file='monitor.dat', status='replace', action='write')
+    close(u)
   end if
 
   open(newunit=u, file='inputFDTD.dat', status='old', action='read', iostat=ios)
@@ -85,7 +100,8 @@ contains
     type(iris_run), target :: run
     type(c_ptr) :: h
     character(len=400) :: errmsg
-    integer :: ierr, lg, it, stop_at, cand, nos, nb, ot, rc
+    integer :: ierr, lg, it, stop_at, cand, nos, nb, ot, rc, um
+    real(8), target :: diag(4)
     real(8), allocatable, target :: jabs(:,:)
     integer(8) :: c0, c1, crate
     real(8) :: secs
@@ -150,6 +166,13 @@ contains
         if (sdl_enabled(run%sdl)) then
           call write_snapshots(h, run, nos + 1, rc, errmsg)
           if (rc /= 0) exit
+        end if
+        if (monitor) then
+          rc = iris_diagnostics(h, run%p%pmlzi + 1, run%p%pmlzf - 1, c_loc(diag))
+          if (rc /= 0) exit
+          open(newunit=um, file='monitor.dat', status='unknown', action='write', position='append')
+          write(um, '(I10,5(1X,ES24.16E3))') stop_at, diag(1), diag(2), diag(1) + diag(2), diag(3), diag(4)
+          close(um)
         end if
         nos = nos + 1
       end if

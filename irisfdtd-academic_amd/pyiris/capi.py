@@ -91,6 +91,8 @@ def load_library(path=None):
     lib.iris_set_field.restype = ct.c_int
     lib.iris_get_plane.argtypes = [H, ct.c_int32, ct.c_int32, ct.c_int32, ct.c_void_p, ct.c_void_p]
     lib.iris_get_plane.restype = ct.c_int
+    lib.iris_diagnostics.argtypes = [H, ct.c_int32, ct.c_int32, ct.c_void_p]
+    lib.iris_diagnostics.restype = ct.c_int
     lib.iris_current_step.argtypes = [H, ct.POINTER(ct.c_int32)]
     lib.iris_current_step.restype = ct.c_int
     lib.iris_destroy.argtypes = [H]
@@ -108,8 +110,8 @@ def load_library(path=None):
 
 
 EXPORTED = ["iris_create", "iris_step", "iris_sync", "iris_flux", "iris_dft_points", "iris_get_dft",
-            "iris_get_field", "iris_set_field", "iris_get_plane", "iris_current_step", "iris_destroy",
-            "iris_last_error", "iris_version", "iris_profile", "iris_profile_read"]
+            "iris_get_field", "iris_set_field", "iris_get_plane", "iris_diagnostics", "iris_current_step",
+            "iris_destroy", "iris_last_error", "iris_version", "iris_profile", "iris_profile_read"]
 
 KERNELS = ["epass", "hpass", "tfsf", "dft_sample", "dft_accum", "flux", "yee"]
 
@@ -192,6 +194,16 @@ class Engine:
         self._check(self.lib.iris_get_plane(self._h, int(what), int(axis), int(index), rng, _p(out)),
                     "iris_get_plane")
         return out.view(np.complex128).reshape(nb, na)
+
+    def diagnostics(self, zi=None, zf=None):
+        """[W_E, W_H, max |div D|, max |div B|] over z in zi..zf (default: the
+        whole region between the UPML slabs)."""
+        p = self.params
+        zi = p.pmlzi + 1 if zi is None else zi
+        zf = p.pmlzf - 1 if zf is None else zf
+        out = np.zeros(4)
+        self._check(self.lib.iris_diagnostics(self._h, int(zi), int(zf), _p(out)), "iris_diagnostics")
+        return out
 
     def set_field(self, which, arr):
         a = np.ascontiguousarray(arr, dtype=np.complex128)

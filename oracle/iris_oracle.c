@@ -512,6 +512,75 @@ void oracle_step(oracle* o, int it_first, int it_last) {
   }
 }
 
+/* Weight eps_i of component i at cell (x,y,z): the inverse of the factor the
+ * E update applies to the curl there (code/fdtd.f90:86-147), real part for a
+ * complex eps_inv as the reference's dble(1/eps_inv) (code/energy.f90:184). */
+static double eps_weight(const oracle* o, int i, int ix, int iy, int iz) {
+  const iris_params* p = &o->p;
+  if (iz < p->hi) return 1.0 / p->epsa_inv[0][i][i];
+  if (iz > p->hf) return 1.0 / p->epsa_inv[1][i][i];
+  cx e = EPSI(o, ix, iy, iz);
+  return e.re / (e.re * e.re + e.im * e.im);
+}
+
+/* Field monitors over x in 1..nx, y in 1..ny, z in zi..zf, restating the
+ * reference's uncalled diagnostics (code/energy.f90):
+ *   out[0] = sum Re(eps_i conj(E_i^n) E_i^(n-1))          calc_energy_density :166-201
+ *   out[1] = sum |H_i^(n-1)|^2 / dtcq2                      (mu_inv = 1)
+ *   out[2] = max |sum_i eps_i E_i^n(c) - eps_i E_i^n(c - e_i)|   calc_div_D :110-162
+ *   out[3] = max |sum_i H_i^n(c + e_i) - H_i^n(c)|               calc_div_B :57-106
+ * The reference's calc_energy_density indexes eps_inv(i,i,...) on an array
+ * whose first two extents are 1 and outside its z range, and calc_div_D drops
+ * eps and the imaginary parts ("valid only in vacuum", :139); here eps is the
+ * weight of the E update and the fields are complex.  With these weights
+ * out[0]+out[1] and the two divergences are invariants of the update in a
+ * source-free, lossless region.  x/y neighbours across the period take the
+ * Bloch factor exp(-+i ak) (code/boundary.f90:163-348). */
+void oracle_diagnostics(const oracle* o, int zi, int zf, double* out) {
+  const iris_params* p = &o->p;
+  const double dtcq2 = ((p->dt * p->c0) / p->q) * ((p->dt * p->c0) / p->q);
+  const cx fxm = bloch_factor(o, 0, -1), fym = bloch_factor(o, 1, -1);
+  const cx fxp = bloch_factor(o, 0, +1), fyp = bloch_factor(o, 1, +1);
+  double we = 0.0, wh = 0.0, dd = 0.0, db = 0.0;
+  for (int iz = zi; iz <= zf; iz++)
+    for (int iy = 1; iy <= o->ny; iy++)
+      for (int ix = 1; ix <= o->nx; ix++) {
+        cx divd = {0.0, 0.0}, divb = {0.0, 0.0};
+        for (int i = 0; i < 3; i++) {
+          const double w = eps_weight(o, i, ix, iy, iz);
+          const cx ec = F(o, o->e_cur, i, ix, iy, iz), ep = F(o, o->e_prev, i, ix, iy, iz);
+          const cx hp = F(o, o->h_prev, i, ix, iy, iz);
+          we += w * (ec.re * ep.re + ec.im * ep.im);
+          wh += (hp.re * hp.re + hp.im * hp.im) / dtcq2;
+          /* backward neighbour of E_i and forward neighbour of H_i along axis i */
+          int bx = ix, by = iy, bz = iz, fx = ix, fy = iy, fz = iz;
+          cx fb = {1.0, 0.0}, ff = {1.0, 0.0};
+          if (i == 0) {
+            bx = ix - 1; fx = ix + 1;
+            if (bx == 0) { bx = o->nx; fb = fxm; }
+            if (fx == o->nx + 1) { fx = 1; ff = fxp; }
+          } else if (i == 1) {
+            by = iy - 1; fy = iy + 1;
+            if (by == 0) { by = o->ny; fb = fym; }
+            if (fy == o->ny + 1) { fy = 1; ff = fyp; }
+          } else {
+            bz = iz - 1; fz = iz + 1;
+          }
+          const cx eb = cmul(fb, F(o, o->e_cur, i, bx, by, bz));
+          divd = cadd(divd, csub(cmul_r(w, ec), cmul_r(eps_weight(o, i, bx, by, bz), eb)));
+          const cx hf = cmul(ff, F(o, o->h_cur, i, fx, fy, fz));
+          divb = cadd(divb, csub(hf, F(o, o->h_cur, i, ix, iy, iz)));
+        }
+        const double ad = hypot(divd.re, divd.im), ab = hypot(divb.re, divb.im);
+        if (ad > dd) dd = ad;
+        if (ab > db) db = ab;
+      }
+  out[0] = we;
+  out[1] = wh;
+  out[2] = dd;
+  out[3] = db;
+}
+
 /* code/dataprocess.f90:241-277 */
 void oracle_flux(const oracle* o, int nobox, double* jabs) {
   int k = nobox - 1, nof = o->p.nof;

@@ -41,6 +41,8 @@ def load():
     lib.oracle_get_field.restype = None
     lib.oracle_set_field.argtypes = [V, ct.c_int, V]
     lib.oracle_set_field.restype = None
+    lib.oracle_diagnostics.argtypes = [V, ct.c_int, ct.c_int, V]
+    lib.oracle_diagnostics.restype = None
     lib.oracle_current_step.argtypes = [V]
     lib.oracle_current_step.restype = ct.c_int
     lib.oracle_destroy.argtypes = [V]
@@ -100,6 +102,13 @@ class Oracle:
         a = np.ascontiguousarray(arr, dtype=np.complex128)
         self.lib.oracle_set_field(self._h, int(which), _p(a.view(np.float64)))
 
+    def diagnostics(self, zi, zf):
+        """[W_E, W_H, max |div D|, max |div B|] over z in zi..zf (the
+        oracle_diagnostics definitions)."""
+        out = np.zeros(4)
+        self.lib.oracle_diagnostics(self._h, int(zi), int(zf), _p(out))
+        return out
+
     def close(self):
         if self._h:
             self.lib.oracle_destroy(self._h)
@@ -116,9 +125,10 @@ def ref_available():
     return os.path.exists(REF_BIN)
 
 
-def run_reference(input_text, workdir, dump_emg=False, timeout=3600, pin_core=None):
+def run_reference(input_text, workdir, dump_emg=False, timeout=3600, pin_core=None, check=True):
     """Run the reference binary in `workdir` on `input_text`; returns the
-    workdir.  Raises if the binary is absent."""
+    workdir (check=False: returns (workdir, exit status, stderr text)).
+    Raises if the binary is absent."""
     if not ref_available():
         raise FileNotFoundError(REF_BIN)
     if os.path.exists(workdir):
@@ -132,8 +142,10 @@ def run_reference(input_text, workdir, dump_emg=False, timeout=3600, pin_core=No
     cmd = [REF_BIN]
     if pin_core is not None and shutil.which("taskset"):
         cmd = ["taskset", "-c", str(pin_core)] + cmd
-    subprocess.run(cmd, cwd=workdir, env=env, check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
-                   timeout=timeout)
+    r = subprocess.run(cmd, cwd=workdir, env=env, check=check, stdout=subprocess.DEVNULL,
+                       stderr=subprocess.PIPE, timeout=timeout)
+    if not check:
+        return workdir, r.returncode, r.stderr.decode(errors="replace")
     return workdir
 
 

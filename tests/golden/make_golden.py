@@ -78,10 +78,23 @@ CASES = {
         box=(1, 7, 1, 5, 10, 38), nof=2, nt=90, no_downloads=2, sdl=(1, 0, 1),
         sdl_opts=dict(copy=(0, 1), xi=(4, 1, 2), xf=(4, 1, 6), yi=(2, 1, 2), yf=(5, 1, 4), zi=(8, 1, 25),
                       zf=(36, 1, 25), module_e=2, fields=(4, 5, 1, 1, 1, 0))), []),
+    # <<LOOP>> header (reference code/program.f90:34-57): the run is to be
+    # repeated for i = 1..2 with the flux blocks of every run appended.  The
+    # Flang build of the reference completes the first run and aborts when
+    # the second reopens log.dat's unit (code/main.f90:115); the golden holds
+    # the first run's files and the abort message (key "ref_abort").
+    "loop_2runs": (lambda: "<<LOOP>>\n1\n2\n0.0\n0.0\n0.0\n" + inputs.render(
+        ixmax=5, iymax=4, izmax=48, pmlzi=8, pmlzf=40, hi=22, dh=5, dx=2, eps_bbox=3.0, tfsf_o=12,
+        box=(1, 5, 1, 4, 10, 38), nof=3, nt=80, no_downloads=2), []),
     "substrate3d": (lambda: inputs.render(
         ixmax=6, iymax=6, izmax=72, pmlzi=10, pmlzf=62, hi=30, dh=8, dx=2, eps_bbox=2.5, tfsf_o=16, eps3=2.0,
         box=(1, 6, 1, 6, 12, 55), nof=6, nt=240, no_downloads=4, lno=450.0, dlambda=90.0), [3]),
 }
+
+
+# cases on which the reference binary stops with a runtime error; the text
+# that must appear in its error output
+EXPECTED_ABORT = {"loop_2runs": "ACTION= may not be changed on an open unit"}
 
 
 def make(name):
@@ -90,9 +103,16 @@ def make(name):
     with open(os.path.join(HERE, name + ".dat"), "w", encoding="latin-1") as f:
         f.write(text)
     with tempfile.TemporaryDirectory(prefix="iris_golden_") as wd:
-        pyoracle.run_reference(text, os.path.join(wd, "run"), dump_emg=True)
-        run = os.path.join(wd, "run")
         out = {}
+        if name in EXPECTED_ABORT:
+            _, rc, err = pyoracle.run_reference(text, os.path.join(wd, "run"), dump_emg=True, check=False)
+            if rc == 0 or EXPECTED_ABORT[name] not in err:
+                raise SystemExit(f"{name}: expected the reference to abort with {EXPECTED_ABORT[name]!r}, "
+                                 f"got status {rc}: {err[-300:]}")
+            out["ref_abort"] = np.frombuffer(EXPECTED_ABORT[name].encode(), dtype=np.uint8)
+        else:
+            pyoracle.run_reference(text, os.path.join(wd, "run"), dump_emg=True)
+        run = os.path.join(wd, "run")
         for b in (1, 2):
             if not os.path.exists(os.path.join(run, f"{b}_scatt1.dat")):
                 continue
