@@ -190,8 +190,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this build "
-              "(z-slab sharding is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py: this engine runs on one GPU; a run on several GPUs (one z-slab per GPU with "
+              "plane exchanges) is not implemented", file=sys.stderr)
         sys.exit(2)
 
     r32 = run_gpu(4, a.steps, a.warmup)
