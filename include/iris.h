@@ -187,7 +187,10 @@ const char* iris_last_error(const iris_handle* h);
 #define IRIS_K_DFT_ACCUM 4  /* fold of buffered samples into emg         */
 #define IRIS_K_FLUX 5       /* face-flux reduction                       */
 #define IRIS_K_YEE 6        /* fused E+H pass (replaces EPASS/HPASS/TFSF) */
-#define IRIS_NKERNEL 7
+#define IRIS_K_YEE2 7       /* two-step pass over the plain z-chunks        */
+#define IRIS_K_YEE_REST 8   /* within a two-step pass: one single step of   */
+                            /* the other z-chunks                           */
+#define IRIS_NKERNEL 9
 
 /* Enable (1) or disable (0) per-kernel timing with HIP events recorded
  * around every launch on the handle's stream.  No counterpart in the
@@ -198,6 +201,13 @@ int iris_profile(iris_handle* h, int32_t enable);
  * last reset (arrays of IRIS_NKERNEL); waits for pending work.  reset != 0
  * clears the totals after reading. */
 int iris_profile_read(iris_handle* h, double* ms, int64_t* launches, int32_t reset);
+
+/* Layout of the step passes: info[0] = planes per z-chunk; info[1] = planes
+ * advanced two steps per pass (0 when the two-step pass is off: Bloch phase
+ * in x or y, a grid narrower than a tile, IRIS_TWO_STEP=0); info[2] = z-chunks
+ * of the two-step pass; info[3] = 1 when it is on.  No counterpart in the
+ * reference; used to account the algorithmic bytes of the passes. */
+int iris_pass_info(const iris_handle* h, int64_t info[4]);
 
 /* Library version string. */
 const char* iris_version(void);

@@ -79,6 +79,18 @@ struct iris_handle {
   TfsfConst* tfc = nullptr; // device copy of the TF/SF constants
   int* chunks = nullptr;    // first plane of each fused z-chunk: bulk ones, then general ones
   int nbulk = 0, ngen = 0;
+  // two steps per pass (k_yee2) over the chunks whose planes z0-1..z1+2 are
+  // plain; the others take two single steps (lean ones, then general ones)
+  bool two_step = false;
+  void* ke = nullptr;       // third E and H buffers (step n+2 of a two-step pass)
+  void* kh = nullptr;
+  int* chunks2 = nullptr;   // two-step chunks (z0 | export flags)
+  int n2 = 0;
+  int* chunks_rest = nullptr; // the other chunks: lean ones, then general ones
+  int nrest_lean = 0, nrest_gen = 0;
+  unsigned char* zmask = nullptr; // 1 on the planes of two-step chunks
+  int tx2 = 0, ty2 = 0;
+  int xcd_rows2 = 4;        // tile rows per XCD run of the two-step pass
   void* eps_re = nullptr;
   void* eps_im = nullptr;
   void* pml = nullptr;
@@ -250,6 +262,105 @@ TfsfConst tfsf_const(const iris_handle* h) {
   return t;
 }
 
+// StepArgs with explicit buffers (the two-step pass chains passes through
+// the three field buffers)
+template <typename T>
+StepArgs<T> step_args_on(const iris_handle* h, void* e, void* hf, void* en, void* hn, void* d, void* b, void* dn,
+                         void* bn) {
+  using C = cplx<T>;
+  StepArgs<T> a = step_args<T>(h);
+  a.e = (C*)e; a.h = (C*)hf; a.en = (C*)en; a.hn = (C*)hn;
+  a.d = (C*)d; a.b = (C*)b; a.dn = (C*)dn; a.bn = (C*)bn;
+  return a;
+}
+
+// single-step fused launches over lists of lean and general chunks
+template <typename T>
+void launch_fused_lists(iris_handle* h, const StepArgs<T>& a, int it, const int* lean, int nlean, const int* gen,
+                        int ngen) {
+  TileMap tm{(int)h->grd_fused.x, (int)h->grd_fused.y, 0, (int)h->grd_fused.x * h->xcd_rows};
+  if (nlean) {
+    tm.nblk = tm.tx * tm.ty * nlean;
+    hipLaunchKernelGGL((k_yee_fused<T, false>), dim3(tm.nblk), h->blk_fused, 0, h->stream, a,
+                       (const TfsfConst*)h->tfc, it, lean, tm);
+  }
+  if (ngen) {
+    tm.nblk = tm.tx * tm.ty * ngen;
+    hipLaunchKernelGGL((k_yee_fused<T, true>), dim3(tm.nblk), h->blk_fused, 0, h->stream, a,
+                       (const TfsfConst*)h->tfc, it, gen, tm);
+  }
+}
+
+template <typename T>
+int dft_sample(iris_handle* h, int n);
+template <typename T>
+int dft_flush(iris_handle* h);
+
+// Steps it and it+1 in one pass: the two-step kernel over its chunks (step n
+// from the current buffers, step n+2 into the third), the other chunks by two
+// single-step launches (current -> second buffer -> third), then the DFT
+// samples of both steps.
+template <typename T>
+int launch_double(iris_handle* h, int it) {
+  using C = cplx<T>;
+  const iris_params& p = h->p;
+  const bool s1 = it >= p.to_fourier && (it % p.sample_fourier) == 0;
+  const bool s2 = it + 1 >= p.to_fourier && ((it + 1) % p.sample_fourier) == 0;
+  Yee2Samples<T> smp{};
+  smp.nbox = p.nbox;
+  for (int b = 0; b < p.nbox; ++b) {
+    smp.g[b] = h->box[b].g;
+    smp.samp[b] = (s1 && h->box[b].g.ntot) ? (C*)h->box[b].samp + (int64_t)h->nsamp * h->box[b].g.ntot : nullptr;
+  }
+  const int* rl = h->chunks_rest;
+  const int* rg = h->chunks_rest + h->nrest_lean;
+  if (h->nrest_lean + h->nrest_gen) {
+    ProfScope ps(h, IRIS_K_YEE_REST);
+    const StepArgs<T> a1 = step_args_on<T>(h, h->fe, h->fh, h->ge, h->gh, h->d, h->b, h->dn, h->bn);
+    launch_fused_lists<T>(h, a1, it, rl, h->nrest_lean, rg, h->nrest_gen);
+  }
+  {
+    ProfScope ps(h, IRIS_K_YEE2);
+    const StepArgs<T> a2 = step_args_on<T>(h, h->fe, h->fh, h->ke, h->kh, h->d, h->b, h->dn, h->bn);
+    const TileMap tm{h->tx2, h->ty2, h->tx2 * h->ty2 * h->n2, h->tx2 * h->xcd_rows2};
+    hipLaunchKernelGGL(k_yee2<T>, dim3((unsigned)tm.nblk), dim3(kT2X, Yee2Shape<T>::R), 0, h->stream, a2, (C*)h->ge,
+                       (C*)h->gh, (const int*)h->chunks2, tm, smp);
+  }
+  if (h->nrest_lean + h->nrest_gen) {
+    ProfScope ps(h, IRIS_K_YEE_REST);
+    const StepArgs<T> a3 = step_args_on<T>(h, h->ge, h->gh, h->ke, h->kh, h->dn, h->bn, h->d, h->b);
+    launch_fused_lists<T>(h, a3, it + 1, rl, h->nrest_lean, rg, h->nrest_gen);
+  }
+  HIP_TRY(h, hipGetLastError());
+  if (s1) {
+    // step it outside the two-step chunks is in the second buffer
+    const StepArgs<T> ab = step_args_on<T>(h, h->ge, h->gh, h->ge, h->gh, h->d, h->b, h->d, h->b);
+    if (h->nsamp == 0) h->samp_n0 = it + 1 - p.to_fourier;
+    for (int b = 0; b < p.nbox; ++b) {
+      DeviceBox& B = h->box[b];
+      if (B.g.ntot == 0) continue;
+      ProfScope ps(h, IRIS_K_DFT_SAMPLE);
+      hipLaunchKernelGGL(k_dft_sample_masked<T>, dim3((unsigned)((B.g.ntot + 255) / 256)), dim3(256), 0, h->stream,
+                         ab, B.g, (C*)B.samp, h->nsamp, (const unsigned char*)h->zmask);
+    }
+    HIP_TRY(h, hipGetLastError());
+    h->nsamp++;
+    if (h->nsamp == dft_slots<T>()) {
+      int rc = dft_flush<T>(h);
+      if (rc) return rc;
+    }
+  }
+  std::swap(h->fe, h->ke);
+  std::swap(h->fh, h->kh);
+  h->it = it;
+  if (s2) {
+    int rc = dft_sample<T>(h, it + 2 - p.to_fourier);
+    if (rc) return rc;
+  }
+  h->it = it + 1;
+  return 0;
+}
+
 template <typename T>
 int launch_step(iris_handle* h, int it) {
   StepArgs<T> a = step_args<T>(h);
@@ -332,7 +443,15 @@ int dft_sample(iris_handle* h, int n) {
 
 template <typename T>
 int run_steps(iris_handle* h, int it_first, int it_last) {
-  for (int it = it_first; it <= it_last; ++it) {
+  int it = it_first;
+  // two steps per pass while at least three steps remain: every call ends
+  // with a single-step pass, so the previous step's E and H stay whole
+  while (h->two_step && it_last - it + 1 >= 3) {
+    int rc = launch_double<T>(h, it);
+    if (rc) return rc;
+    it += 2;
+  }
+  for (; it <= it_last; ++it) {
     int rc = launch_step<T>(h, it);
     if (rc) return rc;
     if (it >= h->p.to_fourier && (it % h->p.sample_fourier) == 0) {
@@ -511,7 +630,10 @@ int create_impl(iris_handle* h, const double* eps_inv, const double* fpmn, const
   h->grd_yee = dim3(gx, gy, (p.nz + zc - 1) / zc);
   {
     const int fgx = (p.nx + kFTX - 1) / kFTX, fgy = (p.ny + kFTY - 1) / kFTY;
-    const int fzc = (int)std::min<int64_t>(32, std::max<int64_t>(1, (int64_t)fgx * fgy * p.nz / 4096));
+    int fzc = (int)std::min<int64_t>(32, std::max<int64_t>(1, (int64_t)fgx * fgy * p.nz / 4096));
+    // planes per z-chunk can be set for testing the chunk logic on small grids
+    const char* zenv = std::getenv("IRIS_ZCHUNK");
+    if (zenv && std::atoi(zenv) > 0) fzc = std::min(std::atoi(zenv), p.nz);
     h->blk_fused = dim3(kFTX, kFTY + 1, 1);
     h->grd_fused = dim3(fgx, fgy, (p.nz + fzc - 1) / fzc);
     if (h->fused) h->zchunk = fzc;
@@ -551,6 +673,60 @@ int create_impl(iris_handle* h, const double* eps_inv, const double* fpmn, const
     if ((rc = dalloc(h, (void**)&h->chunks, sizeof(int) * all.size()))) return rc;
     HIP_TRY(h, hipMemcpyAsync(h->chunks, all.data(), sizeof(int) * all.size(), hipMemcpyHostToDevice, h->stream));
     HIP_TRY(h, hipStreamSynchronize(h->stream));
+
+    // two steps per pass: chunks whose planes z0-1 .. z1+2 are all plain, on
+    // grids without a Bloch phase in x and y and wide enough that a tile's
+    // periodic images stay inside the grid
+    const char* env = std::getenv("IRIS_TWO_STEP");
+    const bool want = !(env && env[0] == '0');
+    const bool no_phase = p.ak[0][0] == 0.0 && p.ak[0][1] == 0.0 && p.ak[1][0] == 0.0 && p.ak[1][1] == 0.0;
+    std::vector<char> dbl;
+    int ndbl = 0;
+    for (int z0 = 1; z0 <= p.nz; z0 += zc) {
+      const int z1 = std::min(z0 + zc - 1, p.nz);
+      bool ok = h->fused && want && no_phase && p.nx >= kT2X && p.ny >= Yee2Shape<T>::R && z1 + 2 <= p.nz &&
+                z1 - z0 + 1 == zc;
+      for (int z = z0 - 1; ok && z <= z1 + 2; ++z) ok = is_bulk_plane(z);
+      dbl.push_back(ok ? 1 : 0);
+      ndbl += ok;
+    }
+    if (ndbl > 0) {
+      std::vector<int> c2, rlean, rgen;
+      std::vector<unsigned char> zm((size_t)p.nz + 2, 0);
+      const int nch = (int)dbl.size();
+      for (int k = 0; k < nch; ++k) {
+        const int z0 = 1 + k * zc, z1 = std::min(z0 + zc - 1, p.nz);
+        if (dbl[k]) {
+          int v = z0;
+          if (k == 0 || !dbl[k - 1]) v |= kChunkExportBelow;
+          if (k == nch - 1 || !dbl[k + 1]) v |= kChunkExportAbove;
+          c2.push_back(v);
+          for (int z = z0; z <= z1; ++z) zm[z] = 1;
+        } else {
+          bool lean = z1 < p.nz;
+          for (int z = z0 - 1; lean && z <= z1 + 1; ++z) lean = is_bulk_plane(z);
+          (lean ? rlean : rgen).push_back(z0);
+        }
+      }
+      h->two_step = true;
+      h->n2 = (int)c2.size();
+      h->nrest_lean = (int)rlean.size();
+      h->nrest_gen = (int)rgen.size();
+      h->tx2 = (p.nx + kT2OX - 1) / kT2OX;
+      h->ty2 = (p.ny + Yee2Shape<T>::OY - 1) / Yee2Shape<T>::OY;
+      std::vector<int> rest(rlean);
+      rest.insert(rest.end(), rgen.begin(), rgen.end());
+      if ((rc = dalloc(h, (void**)&h->chunks2, sizeof(int) * c2.size()))) return rc;
+      if ((rc = dalloc(h, (void**)&h->chunks_rest, sizeof(int) * std::max<size_t>(rest.size(), 1)))) return rc;
+      if ((rc = dalloc(h, (void**)&h->zmask, zm.size()))) return rc;
+      if ((rc = dalloc(h, &h->ke, fbytes)) || (rc = dalloc(h, &h->kh, fbytes))) return rc;
+      HIP_TRY(h, hipMemcpyAsync(h->chunks2, c2.data(), sizeof(int) * c2.size(), hipMemcpyHostToDevice, h->stream));
+      if (!rest.empty())
+        HIP_TRY(h, hipMemcpyAsync(h->chunks_rest, rest.data(), sizeof(int) * rest.size(), hipMemcpyHostToDevice,
+                                  h->stream));
+      HIP_TRY(h, hipMemcpyAsync(h->zmask, zm.data(), zm.size(), hipMemcpyHostToDevice, h->stream));
+      HIP_TRY(h, hipStreamSynchronize(h->stream));
+    }
   }
   HIP_TRY(h, hipStreamSynchronize(h->stream));
   return 0;
@@ -600,6 +776,8 @@ int iris_create(const iris_params* p, const double* eps_inv, const double* fpzmn
     h->fused = !(env && env[0] == '0');
     const char* xenv = std::getenv("IRIS_XCD_ROWS");
     if (xenv && *xenv) h->xcd_rows = std::max(0, std::atoi(xenv));
+    const char* x2env = std::getenv("IRIS_XCD_ROWS2");
+    if (x2env && *x2env) h->xcd_rows2 = std::max(0, std::atoi(x2env));
   }
   (void)hipGetDevice(&h->device);
   e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
@@ -815,6 +993,15 @@ int iris_diagnostics(iris_handle* h, int32_t zi, int32_t zf, double out[4]) {
   HIP_TRY(h, hipGetLastError());
   HIP_TRY(h, hipMemcpyAsync(out, res, 4 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int iris_pass_info(const iris_handle* h, int64_t info[4]) {
+  if (!h || !info) return IRIS_EINVAL;
+  info[0] = h->zchunk;
+  info[1] = h->two_step ? (int64_t)h->n2 * h->zchunk : 0;
+  info[2] = h->two_step ? h->n2 : 0;
+  info[3] = h->two_step ? 1 : 0;
   return 0;
 }
 

@@ -325,3 +325,5 @@ __global__ __launch_bounds__(256) void k_flux(BoxGeom g, const double2* emg, int
 }
 
 }  // namespace iris
+
+#include "iris_yee2.h"

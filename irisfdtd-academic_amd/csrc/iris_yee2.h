@@ -1,0 +1,252 @@
+// iris_yee2.h -- two Yee steps per pass over the bulk of the grid.
+//
+// k_yee2 advances E and H from step n to n+2 in one pass over z-chunks whose
+// planes z0-1 .. z1+2 all lie in the plain regions I or III (between the UPML
+// slabs, outside the structure and the TF/SF planes), for grids without a
+// Bloch phase (exp(+-i ak) = 1: the x/y wraps are plain copies).  Per cell it
+// performs exactly the operations of two single steps (reference
+// code/fdtd.f90:57-248 with the periodic wraps of code/boundary.f90:163-348),
+// so its results are those of two launches of k_yee_fused; what it saves is
+// HBM traffic: step n is read and step n+2 written once.
+//
+// Work decomposition.  A workgroup is kT2X lanes (x) by kT2R waves (y) and
+// covers the footprint x0-1 .. x0+kT2X-2, y0-1 .. y0+kT2R-2 of an output tile
+// of kT2OX x kT2OY cells.  The E update reads H at c-x, c-y and the H update
+// reads E at c+x, c+y, so the footprint reaches one cell back and two ahead.
+// Lanes outside 1..n take their values from the periodic image (the wrap
+// copies the reference writes).  The workgroup marches z with a pipeline:
+// at plane z it forms E1(z), H1(z-1), E2(z-1), H2(z-2) (1: step n+1,
+// 2: step n+2); the z-1 and z-2 values of each lane stay in registers, and the
+// lateral neighbours come through LDS planes (E1 double-buffered, H1, E2)
+// with two barriers per plane.
+//
+// Besides E2/H2 (and the H min-side ghost planes) of its chunk it writes:
+//   - E1 and H1 (with H's min-side ghosts) of its first plane when the chunk
+//     below is not a two-step chunk, and H1 of its last plane when the chunk
+//     above is not: the single-step passes of the neighbouring chunks read
+//     these at step n+1;
+//   - the DFT face samples of step n+1 of its own cells (the samples of step
+//     n+2 are gathered afterwards from the output buffer).
+#pragma once
+#include "iris_kernels.h"
+
+namespace iris {
+
+constexpr int kT2X = 64;
+constexpr int kT2OX = kT2X - 3;  // output cells per row of a tile
+// Waves per workgroup (footprint rows) and the waves per SIMD the register
+// allocation must allow: FP32 fields 16 rows, one workgroup per CU (98 KB of
+// LDS); FP64 fields 11 rows, one workgroup per CU (135 KB of LDS).
+template <typename T> struct Yee2Shape {
+  static constexpr int R = sizeof(T) == 4 ? 16 : 11;
+  static constexpr int OY = R - 3;  // output rows per tile
+  static constexpr int WAVES_PER_EU = (R + 3) / 4;
+};
+constexpr int kChunkExportBelow = 1 << 24;         // chunk flags in the high bits of chunks[]
+constexpr int kChunkExportAbove = 2 << 24;
+constexpr int kChunkZMask = (1 << 24) - 1;
+
+// Index in the concatenated face-point list of box g (inverse of box_point,
+// reference code/dataprocess.f90:813-901): store E (base 0) or H (base 3) of
+// cell (x,y,z) into every face point the cell is.
+template <typename C>
+__device__ inline void sample_cell(const BoxGeom& g, C* __restrict__ samp, int x, int y, int z, int base,
+                                   const C v[3]) {
+  if (g.npts[0] && y >= g.byi && y <= g.byf && z >= g.bzi && z <= g.bzf) {
+    const int64_t q = g.start[0] + 6 * ((int64_t)(y - g.byi) + (int64_t)g.nyb * (z - g.bzi)) + base;
+    if (x == g.bxi) { samp[q] = v[0]; samp[q + 1] = v[1]; samp[q + 2] = v[2]; }
+    if (x == g.bxf) {
+      const int64_t r = q + g.npts[0] / 2;
+      samp[r] = v[0]; samp[r + 1] = v[1]; samp[r + 2] = v[2];
+    }
+  }
+  if (g.npts[1] && x >= g.bxi && x <= g.bxf && z >= g.bzi && z <= g.bzf) {
+    const int64_t q = g.start[1] + 6 * ((int64_t)(x - g.bxi) + (int64_t)g.nxb * (z - g.bzi)) + base;
+    if (y == g.byi) { samp[q] = v[0]; samp[q + 1] = v[1]; samp[q + 2] = v[2]; }
+    if (y == g.byf) {
+      const int64_t r = q + g.npts[1] / 2;
+      samp[r] = v[0]; samp[r + 1] = v[1]; samp[r + 2] = v[2];
+    }
+  }
+  if (g.npts[2] && x >= g.bxi && x <= g.bxf && y >= g.byi && y <= g.byf) {
+    const int64_t q = g.start[2] + 6 * ((int64_t)(x - g.bxi) + (int64_t)g.nxb * (y - g.byi)) + base;
+    if (z == g.bzi) { samp[q] = v[0]; samp[q + 1] = v[1]; samp[q + 2] = v[2]; }
+    if (z == g.bzf) {
+      const int64_t r = q + g.npts[2] / 2;
+      samp[r] = v[0]; samp[r + 1] = v[1]; samp[r + 2] = v[2];
+    }
+  }
+}
+
+// Sampling of step n+1 by k_yee2: up to two boxes; samp[b] already points at
+// the ring slot of that step, or is null when the step is not sampled.
+template <typename T> struct Yee2Samples {
+  BoxGeom g[2];
+  cplx<T>* samp[2];
+  int nbox;
+};
+
+template <typename T>
+__global__ __launch_bounds__(kT2X * Yee2Shape<T>::R, Yee2Shape<T>::WAVES_PER_EU) void k_yee2(StepArgs<T> a, cplx<T>* __restrict__ eb,
+                                                      cplx<T>* __restrict__ hb, const int* __restrict__ chunks,
+                                                      TileMap tm, Yee2Samples<T> smp) {
+  using C = cplx<T>;
+  constexpr int kT2R = Yee2Shape<T>::R, kT2OY = Yee2Shape<T>::OY;
+  __shared__ C s1[2][3][kT2R][kT2X];  // E1 of planes z (z & 1) and z-1
+  __shared__ C sh[3][kT2R][kT2X];  // H1 of plane z-1
+  __shared__ C s2[3][kT2R][kT2X];  // E2 of plane z-1
+  const int lane = threadIdx.x, wy = threadIdx.y;
+  const int nx = a.nx, ny = a.ny;
+  const int64_t cs = a.cs, sxy = a.sxy;
+  int bx, by, bz;
+  tile_of(tm, blockIdx.x, &bx, &by, &bz);
+  const int craw = chunks[bz];
+  const int z0 = craw & kChunkZMask;
+  const int z1 = z0 + a.zchunk - 1;
+  const bool exp_below = (craw & kChunkExportBelow) != 0, exp_above = (craw & kChunkExportAbove) != 0;
+
+  // footprint cell of this lane and its periodic source
+  const int xg = bx * kT2OX + lane, yg = by * kT2OY + wy;  // x0-1+lane, y0-1+wy
+  const int xs = xg < 1 ? xg + nx : (xg > nx ? xg - nx : xg);
+  const int ys = yg < 1 ? yg + ny : (yg > ny ? yg - ny : yg);
+  const bool own = lane >= 1 && lane <= kT2OX && wy >= 1 && wy <= kT2OY && xg <= nx && yg <= ny;
+  const int64_t col = (int64_t)(xs + a.xoff) + a.sx * ys;
+  const int64_t colxm = col - 1;     // x-1 (the ghost x = 0 when xs = 1)
+  const int64_t colym = col - a.sx;  // y-1 (the ghost y = 0 when ys = 1)
+  // LDS neighbours, clamped into the tile (edge lanes compute values nobody uses)
+  const int lp = min(lane + 1, kT2X - 1), lm = max(lane - 1, 0);
+  const int wp = min(wy + 1, kT2R - 1), wm = max(wy - 1, 0);
+  const C* __restrict__ e0 = a.e;
+  const C* __restrict__ h0 = a.h;
+
+  // pipeline registers of this lane: H0(z-1), E1(z-1), H1(z-2), E2(z-2)
+  C hz1[3], e1p[3] = {}, h1pp[3] = {}, e2p[3] = {};
+  {
+    const int64_t i = col + sxy * (z0 - 2);
+    hz1[0] = h0[i];
+    hz1[1] = h0[i + cs];
+    hz1[2] = h0[i + 2 * cs];
+  }
+
+  for (int z = z0 - 1; z <= z1 + 2; ++z) {
+    const int64_t iz = col + sxy * z;
+    // ---- E1(z) from H0(z), H0(z-1), E0(z) (code/fdtd.f90:78-96, :137-147)
+    C h0z[3], e1[3];
+    {
+      h0z[0] = h0[iz];
+      h0z[1] = h0[iz + cs];
+      h0z[2] = h0[iz + 2 * cs];
+      const C h3ym = h0[colym + sxy * z + 2 * cs], h1ym = h0[colym + sxy * z];
+      const C h3xm = h0[colxm + sxy * z + 2 * cs], h2xm = h0[colxm + sxy * z + cs];
+      const C c0 = cadd(csub(csub(h0z[2], h3ym), h0z[1]), hz1[1]);
+      const C c1 = cadd(csub(csub(h0z[0], hz1[0]), h0z[2]), h3xm);
+      const C c2 = cadd(csub(csub(h0z[1], h2xm), h0z[0]), h1ym);
+      const T* ac = (z < a.hi) ? a.a1 : a.a3;
+      e1[0] = cadd(e0[iz], cscale(ac[0], c0));
+      e1[1] = cadd(e0[iz + cs], cscale(ac[1], c1));
+      e1[2] = cadd(e0[iz + 2 * cs], cscale(ac[2], c2));
+    }
+    // ---- H1(z-1) from H0(z-1), E1(z-1) (own and LDS +x/+y), E1(z)
+    // (code/fdtd.f90:182-195)
+    C h1n[3] = {};
+    const int pz = z & 1, pm = pz ^ 1;  // LDS slot of E1(z) and of E1(z-1)
+    if (z >= z0) {
+      const C e3yp = s1[pm][2][wp][lane], e1yp = s1[pm][0][wp][lane];
+      const C e3xp = s1[pm][2][wy][lp], e2xp = s1[pm][1][wy][lp];
+      const C c0 = cadd(csub(csub(e3yp, e1p[2]), e1[1]), e1p[1]);
+      const C c1 = cadd(csub(csub(e1[0], e1p[0]), e3xp), e1p[2]);
+      const C c2 = cadd(csub(csub(e2xp, e1p[1]), e1yp), e1p[0]);
+      h1n[0] = csub(hz1[0], cscale(a.dtcq2, c0));
+      h1n[1] = csub(hz1[1], cscale(a.dtcq2, c1));
+      h1n[2] = csub(hz1[2], cscale(a.dtcq2, c2));
+    }
+    // slot pz last held E1(z-2), read before the previous plane's second
+    // barrier; sh was last read before the previous plane's second barrier
+    s1[pz][0][wy][lane] = e1[0];
+    s1[pz][1][wy][lane] = e1[1];
+    s1[pz][2][wy][lane] = e1[2];
+    sh[0][wy][lane] = h1n[0];
+    sh[1][wy][lane] = h1n[1];
+    sh[2][wy][lane] = h1n[2];
+    __syncthreads();
+    // ---- E2(z-1) from E1(z-1), H1(z-1) (own and LDS -x/-y), H1(z-2)
+    C e2n[3] = {};
+    if (z >= z0 + 1) {
+      const C h3ym = sh[2][wm][lane], h1ym = sh[0][wm][lane];
+      const C h3xm = sh[2][wy][lm], h2xm = sh[1][wy][lm];
+      const C c0 = cadd(csub(csub(h1n[2], h3ym), h1n[1]), h1pp[1]);
+      const C c1 = cadd(csub(csub(h1n[0], h1pp[0]), h1n[2]), h3xm);
+      const C c2 = cadd(csub(csub(h1n[1], h2xm), h1n[0]), h1ym);
+      const T* ac = (z - 1 < a.hi) ? a.a1 : a.a3;
+      e2n[0] = cadd(e1p[0], cscale(ac[0], c0));
+      e2n[1] = cadd(e1p[1], cscale(ac[1], c1));
+      e2n[2] = cadd(e1p[2], cscale(ac[2], c2));
+    }
+    // ---- H2(z-2) from H1(z-2), E2(z-2) (own and LDS +x/+y), E2(z-1)
+    C h2n[3] = {};
+    if (z >= z0 + 2) {
+      const C e3yp = s2[2][wp][lane], e1yp = s2[0][wp][lane];
+      const C e3xp = s2[2][wy][lp], e2xp = s2[1][wy][lp];
+      const C c0 = cadd(csub(csub(e3yp, e2p[2]), e2n[1]), e2p[1]);
+      const C c1 = cadd(csub(csub(e2n[0], e2p[0]), e3xp), e2p[2]);
+      const C c2 = cadd(csub(csub(e2xp, e2p[1]), e1yp), e2p[0]);
+      h2n[0] = csub(h1pp[0], cscale(a.dtcq2, c0));
+      h2n[1] = csub(h1pp[1], cscale(a.dtcq2, c1));
+      h2n[2] = csub(h1pp[2], cscale(a.dtcq2, c2));
+    }
+    __syncthreads();  // the E2(z-2) neighbours and H1(z-1) neighbours have been read
+    s2[0][wy][lane] = e2n[0];
+    s2[1][wy][lane] = e2n[1];
+    s2[2][wy][lane] = e2n[2];
+
+    if (own) {
+      // step n+2 of the chunk's planes, with H's min-side ghost planes
+      if (z >= z0 + 1 && z - 1 <= z1) store3(a.en, col + sxy * (z - 1), cs, e2n);
+      if (z >= z0 + 2) {
+        const int64_t ih = col + sxy * (z - 2);
+        store3(a.hn, ih, cs, h2n);
+        if (xs == nx) store3_phase(a.hn, ih - nx, cs, a.fxm, h2n);
+        if (ys == ny) store3_phase(a.hn, ih - a.sx * ny, cs, a.fym, h2n);
+      }
+      // step n+1 planes read by the single-step passes of the neighbouring chunks
+      if (exp_below && z == z0) store3(eb, iz, cs, e1);
+      if (exp_below && z - 1 == z0) {
+        const int64_t ih = col + sxy * z0;
+        store3(hb, ih, cs, h1n);
+        if (xs == nx) store3_phase(hb, ih - nx, cs, a.fxm, h1n);
+        if (ys == ny) store3_phase(hb, ih - a.sx * ny, cs, a.fym, h1n);
+      }
+      if (exp_above && z - 1 == z1) store3(hb, col + sxy * z1, cs, h1n);
+      // DFT samples of step n+1
+      for (int b = 0; b < smp.nbox; ++b) {
+        if (!smp.samp[b]) continue;
+        if (z <= z1 && z >= z0) sample_cell(smp.g[b], smp.samp[b], xs, ys, z, 0, e1);
+        if (z - 1 >= z0 && z - 1 <= z1) sample_cell(smp.g[b], smp.samp[b], xs, ys, z - 1, 3, h1n);
+      }
+    }
+    // rotate the pipeline
+    for (int k = 0; k < 3; ++k) {
+      hz1[k] = h0z[k];
+      e1p[k] = e1[k];
+      if (z >= z0) h1pp[k] = h1n[k];
+      e2p[k] = e2n[k];
+    }
+  }
+}
+
+// DFT samples of one step from a buffer that holds that step only on the
+// planes outside the two-step chunks (zmask[z] = 0); the two-step kernel
+// stores the samples of the others itself.
+template <typename T>
+__global__ __launch_bounds__(256) void k_dft_sample_masked(StepArgs<T> a, BoxGeom g, cplx<T>* samp, int slot,
+                                                           const unsigned char* __restrict__ zmask) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.ntot) return;
+  int c, x, y, z;
+  box_point(g, i, &c, &x, &y, &z);
+  if (zmask[z]) return;
+  const int64_t idx = (int64_t)(x + a.xoff) + a.sx * y + a.sxy * z;
+  samp[(int64_t)slot * g.ntot + i] = (c < 3) ? a.e[idx + c * a.cs] : a.h[idx + (c - 3) * a.cs];
+}
+
+}  // namespace iris

@@ -93,6 +93,8 @@ def load_library(path=None):
     lib.iris_get_plane.restype = ct.c_int
     lib.iris_diagnostics.argtypes = [H, ct.c_int32, ct.c_int32, ct.c_void_p]
     lib.iris_diagnostics.restype = ct.c_int
+    lib.iris_pass_info.argtypes = [H, ct.c_void_p]
+    lib.iris_pass_info.restype = ct.c_int
     lib.iris_current_step.argtypes = [H, ct.POINTER(ct.c_int32)]
     lib.iris_current_step.restype = ct.c_int
     lib.iris_destroy.argtypes = [H]
@@ -111,9 +113,10 @@ def load_library(path=None):
 
 EXPORTED = ["iris_create", "iris_step", "iris_sync", "iris_flux", "iris_dft_points", "iris_get_dft",
             "iris_get_field", "iris_set_field", "iris_get_plane", "iris_diagnostics", "iris_current_step",
-            "iris_destroy", "iris_last_error", "iris_version", "iris_profile", "iris_profile_read"]
+            "iris_destroy", "iris_last_error", "iris_version", "iris_profile", "iris_profile_read",
+            "iris_pass_info"]
 
-KERNELS = ["epass", "hpass", "tfsf", "dft_sample", "dft_accum", "flux", "yee"]
+KERNELS = ["epass", "hpass", "tfsf", "dft_sample", "dft_accum", "flux", "yee", "yee2", "yee_rest"]
 
 
 class Engine:
@@ -204,6 +207,13 @@ class Engine:
         out = np.zeros(4)
         self._check(self.lib.iris_diagnostics(self._h, int(zi), int(zf), _p(out)), "iris_diagnostics")
         return out
+
+    def pass_info(self):
+        """{zchunk, two_step_planes, two_step_chunks, two_step}: the layout of
+        the step passes."""
+        v = np.zeros(4, dtype=np.int64)
+        self._check(self.lib.iris_pass_info(self._h, _p(v)), "iris_pass_info")
+        return dict(zchunk=int(v[0]), two_step_planes=int(v[1]), two_step_chunks=int(v[2]), two_step=bool(v[3]))
 
     def set_field(self, which, arr):
         a = np.ascontiguousarray(arr, dtype=np.complex128)

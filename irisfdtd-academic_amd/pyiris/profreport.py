@@ -15,8 +15,9 @@
       Writes <class>_fp32_pmc.json / <class>_fp64_pmc.json and a markdown
       summary into DIR.
 
-Kernel classes: "yee" = both instantiations of the fused E+H pass (the lean
-bulk one and the general one, launched back to back every step), "dft_fold",
+Kernel classes: "yee2" = the two-step pass over the plain z-chunks (one
+launch per two steps); "yee" = both instantiations of the single-step fused
+E+H pass (the lean one and the general one), "dft_fold",
 "dft_sample", "epass", "hpass", "tfsf", "flux", "copy" (the calibration).
 """
 import argparse
@@ -32,6 +33,7 @@ sys.path.insert(0, HERE)
 from pmcprobe import COPY_BYTES  # noqa: E402
 
 CLASSES = [
+    ("yee2", r"k_yee2"),
     ("yee", r"k_yee_fused"),
     ("epass", r"k_epass"),
     ("hpass", r"k_hpass"),
@@ -110,6 +112,7 @@ def read_counter(d, counter):
 
 def pmc(args):
     res = {}
+    nlaunch = {}
     for counter, d in (("FETCH_SIZE", args.fetch), ("WRITE_SIZE", args.write)):
         rows = read_counter(d, counter)
         copies = [float(r["Counter_Value"]) for r in rows if kernel_class(r["Kernel_Name"]) == "copy"]
@@ -123,6 +126,8 @@ def pmc(args):
             if not cls or cls == "copy" or not prec:
                 continue
             sums[(cls, prec)] = sums.get((cls, prec), 0.0) + float(r["Counter_Value"]) * 1024.0 * scale
+            if counter == "FETCH_SIZE":
+                nlaunch[(cls, prec)] = nlaunch.get((cls, prec), 0) + 1
         res[counter] = {"copy_kib": ref_kib, "scale": scale, "bytes": sums}
     os.makedirs(args.out_dir, exist_ok=True)
     md = ["Counter runs of `irisfdtd-academic_amd/pyiris/pmcprobe.py` (512x512x1024 slit-array workload,",
@@ -134,16 +139,25 @@ def pmc(args):
     f, w = res["FETCH_SIZE"], res["WRITE_SIZE"]
     md += [f"Calibration ({COPY_BYTES} B device-to-device copy): FETCH_SIZE reads {f['copy_kib']:.0f} KiB "
            f"(scale x{f['scale']:.3f}), WRITE_SIZE reads {w['copy_kib']:.0f} KiB (scale x{w['scale']:.3f}).", "",
-           "| class | fields | fetch GB/step | write GB/step | HBM GB/step |", "|---|---|---:|---:|---:|"]
+           "| class | fields | launches | fetch GB/step | write GB/step | HBM GB/step |",
+           "|---|---|---:|---:|---:|---:|"]
     keys = sorted(set(f["bytes"]) | set(w["bytes"]), key=lambda k: (k[1], k[0]))
     for cls, prec in keys:
         fb = f["bytes"].get((cls, prec), 0.0) / args.steps
         wb = w["bytes"].get((cls, prec), 0.0) / args.steps
-        md.append(f"| {cls} | fp{32 if prec == 4 else 64} | {fb / 1e9:.3f} | {wb / 1e9:.3f} | {(fb + wb) / 1e9:.3f} |")
-        if cls == "yee":
-            name = f"yee_fp{32 if prec == 4 else 64}_pmc.json"
+        md.append(f"| {cls} | fp{32 if prec == 4 else 64} | {nlaunch.get((cls, prec), 0)} | {fb / 1e9:.3f} | "
+                  f"{wb / 1e9:.3f} | {(fb + wb) / 1e9:.3f} |")
+        # the single-step class is a per-step figure only when no two-step
+        # pass ran (otherwise it also holds the other chunks' steps)
+        if cls == "yee2" or (cls == "yee" and ("yee2", prec) not in nlaunch):
+            name = f"{cls}_fp{32 if prec == 4 else 64}_pmc.json"
+            launches = args.steps if cls == "yee" else nlaunch.get((cls, prec), 1)
+            fb = f["bytes"].get((cls, prec), 0.0) / launches
+            wb = w["bytes"].get((cls, prec), 0.0) / launches
+            what = ("yee2 (two-step pass over the plain z-chunks, per launch = two steps)" if cls == "yee2"
+                    else "yee (fused E+H pass, all launches of one step)")
             with open(os.path.join(args.out_dir, name), "w") as fo:
-                json.dump({"kernel": "yee (fused E+H pass, all launches of one step)",
+                json.dump({"kernel": what,
                            "workload": "512x512x1024 slit-array film, nof=100",
                            "fetch_bytes_per_launch": round(fb), "write_bytes_per_launch": round(wb),
                            "hbm_bytes_per_launch": round(fb + wb),

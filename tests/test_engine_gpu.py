@@ -351,3 +351,67 @@ def test_bloch_phases_match_oracle(tmp_path, fused):
             assert np.linalg.norm(a - b) / np.linalg.norm(b) <= 1e-12, w
         for pair in (1, 2, 3):
             assert dft_rel_err(eng.dft(1, pair), o.dft(1, pair)) <= 1e-12
+
+
+TWO_STEP_CASES = {
+    # partial tiles in x (70 = 61 + 9) and y (21 = 13 + 8 with FP32 fields,
+    # 2*8 + 5 with FP64), structure, two boxes whose side faces cross the
+    # two-step chunks, odd cadence segments
+    "ipol2_2box": dict(ixmax=70, iymax=21, izmax=160, pmlzi=10, pmlzf=150, hi=80, dh=9, dx=7, eps_bbox=3.0,
+                       tfsf_o=20, box=(3, 68, 2, 20, 15, 140), mbox=(1, 70, 5, 9, 40, 120), nof=3, nt=101,
+                       no_downloads=3),
+    # x polarisation, real source, DFT from step 7 every 3 steps
+    "ipol1_stride": dict(i_pol=1, ixmax=66, iymax=17, izmax=120, pmlzi=10, pmlzf=110, hi=60, dh=6, dx=4,
+                         eps_bbox=2.0, tfsf_o=20, cosine=1, box=(1, 66, 1, 17, 14, 100), nof=2, nt=80,
+                         no_downloads=4, to_fourier=7, sample_fourier=3),
+    # region III different from I, box faces inside the top plain region
+    "substrate": dict(ixmax=130, iymax=24, izmax=140, pmlzi=10, pmlzf=130, hi=40, dh=10, dx=20, eps_bbox=4.0,
+                      eps3=2.25, tfsf_o=20, box=(10, 120, 3, 22, 60, 110), nof=2, nt=64, no_downloads=2),
+}
+
+
+def _run_two_step(setup, precision, two_step, zchunk):
+    keys = {"IRIS_TWO_STEP": "1" if two_step else "0", "IRIS_ZCHUNK": str(zchunk)}
+    old = {k: os.environ.get(k) for k in keys}
+    os.environ.update(keys)
+    try:
+        eng = Engine(setup, precision=precision)
+    finally:
+        for k, v in old.items():
+            if v is None:
+                del os.environ[k]
+            else:
+                os.environ[k] = v
+    with eng:
+        eng.profile(True)
+        res = run_with_cadence(eng, setup.params, setup.params.nbox)
+        used = eng.profile_read()["yee2"][1]
+        p = setup.params
+        out = dict(res=res, e=eng.field(0), h=eng.field(1), used=used,
+                   hp=eng.plane(Engine.PLANE_H_PREV + 1, 3, p.nz // 2, (1, p.nx), (1, p.ny)),
+                   dft={(b, j): eng.dft(b, j) for b in range(1, p.nbox + 1) for j in (1, 2, 3)
+                        if eng.dft_points(b, j) > 1})
+    return out
+
+
+@pytest.mark.parametrize("name", list(TWO_STEP_CASES))
+@pytest.mark.parametrize("precision", [8, 4])
+@pytest.mark.parametrize("zchunk", [1, 5, 16])
+def test_two_step_pass_is_bitwise_two_single_steps(tmp_path, name, precision, zchunk):
+    """Two steps per pass over the plain z-chunks give bit for bit the fields,
+    DFT accumulators, fluxes and previous-step H of single steps."""
+    _announce(f"two-step {name} prec={precision} zchunk={zchunk}")
+    s = load_input(write_input(tmp_path, inputs.render(**TWO_STEP_CASES[name])))
+    one = _run_two_step(s, precision, False, zchunk)
+    two = _run_two_step(s, precision, True, zchunk)
+    assert one["used"] == 0 and two["used"] > 0
+    for b in one["res"]:
+        assert [it for it, _ in one["res"][b]] == [it for it, _ in two["res"][b]]
+        for (_, j1), (_, j2) in zip(one["res"][b], two["res"][b]):
+            assert np.array_equal(j1, j2), f"box {b} flux differs"
+    interior = (slice(1, -1), slice(1, -1), slice(1, -1))
+    assert np.array_equal(one["e"][interior], two["e"][interior])
+    assert np.array_equal(one["h"][interior], two["h"][interior])
+    assert np.array_equal(one["hp"], two["hp"])
+    for k in one["dft"]:
+        assert np.array_equal(one["dft"][k], two["dft"][k]), f"dft {k} differs"
