@@ -86,6 +86,7 @@ struct iris_handle {
   void* kh = nullptr;
   int* chunks2 = nullptr;   // two-step chunks (z0 | export flags)
   int n2 = 0;
+  int64_t planes2 = 0;      // planes advanced by the two-step pass
   int* chunks_rest = nullptr; // the other chunks: lean ones, then general ones
   int nrest_lean = 0, nrest_gen = 0;
   unsigned char* zmask = nullptr; // 1 on the planes of two-step chunks
@@ -674,9 +675,15 @@ int create_impl(iris_handle* h, const double* eps_inv, const double* fpmn, const
     HIP_TRY(h, hipMemcpyAsync(h->chunks, all.data(), sizeof(int) * all.size(), hipMemcpyHostToDevice, h->stream));
     HIP_TRY(h, hipStreamSynchronize(h->stream));
 
-    // two steps per pass: chunks whose planes z0-1 .. z1+2 are all plain, on
+    // two steps per pass: chunks whose planes z0-1 .. z1+2 lie between the
+    // UPML slabs and away from the TF/SF planes (the structure is allowed), on
     // grids without a Bloch phase in x and y and wide enough that a tile's
     // periodic images stay inside the grid
+    auto is_two_step_plane = [&](int z) {
+      if (z <= p.pmlzi || z >= p.pmlzf) return false;
+      if (p.em_source && (z == p.iz_min - 1 || z == p.iz_min)) return false;
+      return true;
+    };
     const char* env = std::getenv("IRIS_TWO_STEP");
     const bool want = !(env && env[0] == '0');
     const bool no_phase = p.ak[0][0] == 0.0 && p.ak[0][1] == 0.0 && p.ak[1][0] == 0.0 && p.ak[1][1] == 0.0;
@@ -686,27 +693,42 @@ int create_impl(iris_handle* h, const double* eps_inv, const double* fpmn, const
       const int z1 = std::min(z0 + zc - 1, p.nz);
       bool ok = h->fused && want && no_phase && p.nx >= kT2X && p.ny >= Yee2Shape<T>::R && z1 + 2 <= p.nz &&
                 z1 - z0 + 1 == zc;
-      for (int z = z0 - 1; ok && z <= z1 + 2; ++z) ok = is_bulk_plane(z);
+      for (int z = z0 - 1; ok && z <= z1 + 2; ++z) ok = is_two_step_plane(z);
       dbl.push_back(ok ? 1 : 0);
       ndbl += ok;
     }
     if (ndbl > 0) {
+      // runs of consecutive two-step chunks are cut into chunks of up to
+      // 2*zc planes (less recomputed halo per plane); the other chunks keep zc
       std::vector<int> c2, rlean, rgen;
       std::vector<unsigned char> zm((size_t)p.nz + 2, 0);
       const int nch = (int)dbl.size();
-      for (int k = 0; k < nch; ++k) {
-        const int z0 = 1 + k * zc, z1 = std::min(z0 + zc - 1, p.nz);
-        if (dbl[k]) {
-          int v = z0;
-          if (k == 0 || !dbl[k - 1]) v |= kChunkExportBelow;
-          if (k == nch - 1 || !dbl[k + 1]) v |= kChunkExportAbove;
-          c2.push_back(v);
-          for (int z = z0; z <= z1; ++z) zm[z] = 1;
-        } else {
+      int zc2 = std::min(2 * zc, kChunkLenMask + 1);
+      const char* z2env = std::getenv("IRIS_ZCHUNK2");  // planes per two-step chunk (tuning)
+      if (z2env && std::atoi(z2env) > 0) zc2 = std::min(std::atoi(z2env), kChunkLenMask + 1);
+      for (int k = 0; k < nch;) {
+        const int z0 = 1 + k * zc;
+        if (!dbl[k]) {
+          const int z1 = std::min(z0 + zc - 1, p.nz);
           bool lean = z1 < p.nz;
           for (int z = z0 - 1; lean && z <= z1 + 1; ++z) lean = is_bulk_plane(z);
           (lean ? rlean : rgen).push_back(z0);
+          ++k;
+          continue;
         }
+        int k1 = k;
+        while (k1 + 1 < nch && dbl[k1 + 1]) ++k1;
+        const int r0 = z0, r1 = 1 + (k1 + 1) * zc - 1;  // the run's planes
+        for (int c0 = r0; c0 <= r1; c0 += zc2) {
+          const int len = std::min(zc2, r1 - c0 + 1);
+          int v = c0 | ((len - 1) << kChunkLenShift);
+          if (c0 == r0) v |= kChunkExportBelow;
+          if (c0 + len - 1 == r1) v |= kChunkExportAbove;
+          c2.push_back(v);
+        }
+        for (int z = r0; z <= r1; ++z) zm[z] = 1;
+        h->planes2 += r1 - r0 + 1;
+        k = k1 + 1;
       }
       h->two_step = true;
       h->n2 = (int)c2.size();
@@ -999,7 +1021,7 @@ int iris_diagnostics(iris_handle* h, int32_t zi, int32_t zf, double out[4]) {
 int iris_pass_info(const iris_handle* h, int64_t info[4]) {
   if (!h || !info) return IRIS_EINVAL;
   info[0] = h->zchunk;
-  info[1] = h->two_step ? (int64_t)h->n2 * h->zchunk : 0;
+  info[1] = h->two_step ? h->planes2 : 0;
   info[2] = h->two_step ? h->n2 : 0;
   info[3] = h->two_step ? 1 : 0;
   return 0;

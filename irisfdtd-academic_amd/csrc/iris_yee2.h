@@ -1,9 +1,9 @@
 // iris_yee2.h -- two Yee steps per pass over the bulk of the grid.
 //
 // k_yee2 advances E and H from step n to n+2 in one pass over z-chunks whose
-// planes z0-1 .. z1+2 all lie in the plain regions I or III (between the UPML
-// slabs, outside the structure and the TF/SF planes), for grids without a
-// Bloch phase (exp(+-i ak) = 1: the x/y wraps are plain copies).  Per cell it
+// planes z0-1 .. z1+2 all lie between the UPML slabs and away from the TF/SF
+// planes (regions I and III and the structure), for grids without a Bloch
+// phase (exp(+-i ak) = 1: the x/y wraps are plain copies).  Per cell it
 // performs exactly the operations of two single steps (reference
 // code/fdtd.f90:57-248 with the periodic wraps of code/boundary.f90:163-348),
 // so its results are those of two launches of k_yee_fused; what it saves is
@@ -42,9 +42,12 @@ template <typename T> struct Yee2Shape {
   static constexpr int OY = R - 3;  // output rows per tile
   static constexpr int WAVES_PER_EU = (R + 3) / 4;
 };
-constexpr int kChunkExportBelow = 1 << 24;         // chunk flags in the high bits of chunks[]
-constexpr int kChunkExportAbove = 2 << 24;
-constexpr int kChunkZMask = (1 << 24) - 1;
+// chunks[] word of the two-step pass: first plane (bits 0-19), planes - 1
+// (bits 20-27), export flags (bits 28-29)
+constexpr int kChunkZMask = (1 << 20) - 1;
+constexpr int kChunkLenShift = 20, kChunkLenMask = 0xff;
+constexpr int kChunkExportBelow = 1 << 28;
+constexpr int kChunkExportAbove = 1 << 29;
 
 // Index in the concatenated face-point list of box g (inverse of box_point,
 // reference code/dataprocess.f90:813-901): store E (base 0) or H (base 3) of
@@ -78,6 +81,33 @@ __device__ inline void sample_cell(const BoxGeom& g, C* __restrict__ samp, int x
   }
 }
 
+// E update of one cell from its curl outside the UPML: region I or III
+// (diagonal epsa_inv) or the structure (per-cell eps_inv, complex only when
+// its imaginary part is stored), as e_cell does (code/fdtd.f90:86-147).
+template <typename T>
+__device__ inline void e_update_plain(const StepArgs<T>& a, int64_t col, int z, const cplx<T> e[3],
+                                      const cplx<T> c[3], cplx<T> n[3]) {
+  if (z < a.hi || z > a.hf) {
+    const T* ac = (z < a.hi) ? a.a1 : a.a3;
+    n[0] = cadd(e[0], cscale(ac[0], c[0]));
+    n[1] = cadd(e[1], cscale(ac[1], c[1]));
+    n[2] = cadd(e[2], cscale(ac[2], c[2]));
+  } else {
+    const int64_t eidx = col + a.sxy * (z - a.hi);
+    const T er = a.eps_re[eidx];
+    if (a.eps_im) {
+      const cplx<T> ei = mk<T>(er, a.eps_im[eidx]);
+      n[0] = cadd(e[0], cmul(ei, c[0]));
+      n[1] = cadd(e[1], cmul(ei, c[1]));
+      n[2] = cadd(e[2], cmul(ei, c[2]));
+    } else {
+      n[0] = cadd(e[0], cscale(er, c[0]));
+      n[1] = cadd(e[1], cscale(er, c[1]));
+      n[2] = cadd(e[2], cscale(er, c[2]));
+    }
+  }
+}
+
 // Sampling of step n+1 by k_yee2: up to two boxes; samp[b] already points at
 // the ring slot of that step, or is null when the step is not sampled.
 template <typename T> struct Yee2Samples {
@@ -102,7 +132,7 @@ __global__ __launch_bounds__(kT2X * Yee2Shape<T>::R, Yee2Shape<T>::WAVES_PER_EU)
   tile_of(tm, blockIdx.x, &bx, &by, &bz);
   const int craw = chunks[bz];
   const int z0 = craw & kChunkZMask;
-  const int z1 = z0 + a.zchunk - 1;
+  const int z1 = z0 + ((craw >> kChunkLenShift) & kChunkLenMask);
   const bool exp_below = (craw & kChunkExportBelow) != 0, exp_above = (craw & kChunkExportAbove) != 0;
 
   // footprint cell of this lane and its periodic source
@@ -138,13 +168,11 @@ __global__ __launch_bounds__(kT2X * Yee2Shape<T>::R, Yee2Shape<T>::WAVES_PER_EU)
       h0z[2] = h0[iz + 2 * cs];
       const C h3ym = h0[colym + sxy * z + 2 * cs], h1ym = h0[colym + sxy * z];
       const C h3xm = h0[colxm + sxy * z + 2 * cs], h2xm = h0[colxm + sxy * z + cs];
-      const C c0 = cadd(csub(csub(h0z[2], h3ym), h0z[1]), hz1[1]);
-      const C c1 = cadd(csub(csub(h0z[0], hz1[0]), h0z[2]), h3xm);
-      const C c2 = cadd(csub(csub(h0z[1], h2xm), h0z[0]), h1ym);
-      const T* ac = (z < a.hi) ? a.a1 : a.a3;
-      e1[0] = cadd(e0[iz], cscale(ac[0], c0));
-      e1[1] = cadd(e0[iz + cs], cscale(ac[1], c1));
-      e1[2] = cadd(e0[iz + 2 * cs], cscale(ac[2], c2));
+      const C c[3] = {cadd(csub(csub(h0z[2], h3ym), h0z[1]), hz1[1]),
+                      cadd(csub(csub(h0z[0], hz1[0]), h0z[2]), h3xm),
+                      cadd(csub(csub(h0z[1], h2xm), h0z[0]), h1ym)};
+      const C eo[3] = {e0[iz], e0[iz + cs], e0[iz + 2 * cs]};
+      e_update_plain<T>(a, col, z, eo, c, e1);
     }
     // ---- H1(z-1) from H0(z-1), E1(z-1) (own and LDS +x/+y), E1(z)
     // (code/fdtd.f90:182-195)
@@ -174,13 +202,10 @@ __global__ __launch_bounds__(kT2X * Yee2Shape<T>::R, Yee2Shape<T>::WAVES_PER_EU)
     if (z >= z0 + 1) {
       const C h3ym = sh[2][wm][lane], h1ym = sh[0][wm][lane];
       const C h3xm = sh[2][wy][lm], h2xm = sh[1][wy][lm];
-      const C c0 = cadd(csub(csub(h1n[2], h3ym), h1n[1]), h1pp[1]);
-      const C c1 = cadd(csub(csub(h1n[0], h1pp[0]), h1n[2]), h3xm);
-      const C c2 = cadd(csub(csub(h1n[1], h2xm), h1n[0]), h1ym);
-      const T* ac = (z - 1 < a.hi) ? a.a1 : a.a3;
-      e2n[0] = cadd(e1p[0], cscale(ac[0], c0));
-      e2n[1] = cadd(e1p[1], cscale(ac[1], c1));
-      e2n[2] = cadd(e1p[2], cscale(ac[2], c2));
+      const C c[3] = {cadd(csub(csub(h1n[2], h3ym), h1n[1]), h1pp[1]),
+                      cadd(csub(csub(h1n[0], h1pp[0]), h1n[2]), h3xm),
+                      cadd(csub(csub(h1n[1], h2xm), h1n[0]), h1ym)};
+      e_update_plain<T>(a, col, z - 1, e1p, c, e2n);
     }
     // ---- H2(z-2) from H1(z-2), E2(z-2) (own and LDS +x/+y), E2(z-1)
     C h2n[3] = {};

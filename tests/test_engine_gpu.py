@@ -364,6 +364,10 @@ TWO_STEP_CASES = {
     "ipol1_stride": dict(i_pol=1, ixmax=66, iymax=17, izmax=120, pmlzi=10, pmlzf=110, hi=60, dh=6, dx=4,
                          eps_bbox=2.0, tfsf_o=20, cosine=1, box=(1, 66, 1, 17, 14, 100), nof=2, nt=80,
                          no_downloads=4, to_fourier=7, sample_fourier=3),
+    # lossy structure (complex eps_inv) inside the two-step chunks
+    "lossy_structure": dict(ixmax=64, iymax=16, izmax=128, pmlzi=10, pmlzf=118, hi=50, dh=30, dx=8,
+                            eps_bbox=complex(4.0, 0.6), tfsf_o=20, box=(1, 64, 1, 16, 15, 110), nof=2, nt=50,
+                            no_downloads=2),
     # region III different from I, box faces inside the top plain region
     "substrate": dict(ixmax=130, iymax=24, izmax=140, pmlzi=10, pmlzf=130, hi=40, dh=10, dx=20, eps_bbox=4.0,
                       eps3=2.25, tfsf_o=20, box=(10, 120, 3, 22, 60, 110), nof=2, nt=64, no_downloads=2),
