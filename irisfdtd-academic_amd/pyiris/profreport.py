@@ -154,7 +154,7 @@ def pmc(args):
             launches = args.steps if cls == "yee" else nlaunch.get((cls, prec), 1)
             fb = f["bytes"].get((cls, prec), 0.0) / launches
             wb = w["bytes"].get((cls, prec), 0.0) / launches
-            what = ("yee2 (two-step pass over the plain z-chunks, per launch = two steps)" if cls == "yee2"
+            what = ("yee2 (two-step pass over the z-chunks between the UPML slabs and away from the TF/SF planes; per launch = two steps)" if cls == "yee2"
                     else "yee (fused E+H pass, all launches of one step)")
             with open(os.path.join(args.out_dir, name), "w") as fo:
                 json.dump({"kernel": what,
