@@ -823,6 +823,7 @@ int iris_step(iris_handle* h, int32_t it_first, int32_t it_last) {
   if (!h) return fail(nullptr, IRIS_EINVAL, "handle is NULL");
   if (it_first != h->it + 1) return fail(h, IRIS_ESTATE, "it_first=%d but the next step is %d", it_first, h->it + 1);
   if (it_last < it_first) return 0;
+  if (it_last > h->p.nt) return fail(h, IRIS_ESTATE, "it_last=%d is past nt=%d", it_last, h->p.nt);
   (void)hipSetDevice(h->device);
   return h->prec == 8 ? run_steps<double>(h, it_first, it_last) : run_steps<float>(h, it_first, it_last);
 }

@@ -170,6 +170,10 @@ def test_step_out_of_order_is_an_error(tmp_path):
         eng.run_to(3)
         with pytest.raises(IrisError, match="next step is 4"):
             eng.step(7, 9)
+        with pytest.raises(IrisError, match="past nt=10"):
+            eng.step(4, 11)
+        eng.step(4, 10)   # the engine is still usable after the refusals
+        assert eng.it == 10
 
 
 def test_fortran_program_end_to_end(tmp_path):
@@ -419,3 +423,54 @@ def test_two_step_pass_is_bitwise_two_single_steps(tmp_path, name, precision, zc
     assert np.array_equal(one["hp"], two["hp"])
     for k in one["dft"]:
         assert np.array_equal(one["dft"][k], two["dft"][k]), f"dft {k} differs"
+
+
+def test_fp32_tr_with_two_step_pass(tmp_path):
+    """FP32 fields (two-step pass engaged) against FP64 fields on a periodic
+    slit array: T and R within 1e-5."""
+    _announce("fp32 two-step T/R")
+    s = load_input(write_input(tmp_path, inputs.render(
+        ixmax=64, iymax=16, izmax=300, pmlzi=20, pmlzf=280, hi=140, dh=20, dx=12, eps_bbox=4.0, tfsf_o=30,
+        box=(1, 64, 1, 16, 25, 270), nof=10, dlambda=60.0, lno=450.0, nt=2500, no_downloads=1)))
+    res = {}
+    for prec in (4, 8):
+        with Engine(s, precision=prec) as eng:
+            assert eng.pass_info()["two_step"]
+            eng.run_to(s.params.nt)
+            j = eng.flux(1)
+        tot = j[:, 4] + j[:, 5]
+        res[prec] = (j[:, 5] / tot, j[:, 4] / tot)
+    for k in (0, 1):
+        d = np.abs(res[4][k] - res[8][k]).max()
+        assert d <= 1e-5, f"{'TR'[k]} differs by {d:.2e}"
+
+
+def test_two_step_bitwise_at_full_size():
+    """512 x 512 x 1024, FP32: the two-step and single-step passes give the
+    same monitors, planes and fluxes bit for bit after an odd step count."""
+    _announce("two-step full size")
+    import bench
+    s = bench.make_setup(4, nof=4)
+    out = {}
+    for two in (True, False):
+        old = os.environ.get("IRIS_TWO_STEP")
+        os.environ["IRIS_TWO_STEP"] = "1" if two else "0"
+        try:
+            eng = Engine(s, precision=4)
+        finally:
+            if old is None:
+                del os.environ["IRIS_TWO_STEP"]
+            else:
+                os.environ["IRIS_TWO_STEP"] = old
+        with eng:
+            assert eng.pass_info()["two_step"] == two
+            eng.run_to(37)
+            p = s.params
+            out[two] = dict(diag=eng.diagnostics(), flux=eng.flux(1),
+                            planes=[eng.plane(Engine.PLANE_E + 0, 3, 500, (1, p.nx), (1, p.ny)),
+                                    eng.plane(Engine.PLANE_H_PREV + 1, 2, 256, (1, p.nx), (25, 1000)),
+                                    eng.plane(Engine.PLANE_H + 2, 1, 1, (1, p.ny), (480, 540))])
+    assert np.array_equal(out[True]["diag"], out[False]["diag"])
+    assert np.array_equal(out[True]["flux"], out[False]["flux"])
+    for a, b in zip(out[True]["planes"], out[False]["planes"]):
+        assert np.array_equal(a, b)
