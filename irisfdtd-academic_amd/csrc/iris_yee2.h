@@ -36,9 +36,9 @@ constexpr int kT2X = 64;
 constexpr int kT2OX = kT2X - 3;  // output cells per row of a tile
 // Waves per workgroup (footprint rows) and the waves per SIMD the register
 // allocation must allow: FP32 fields 16 rows, one workgroup per CU (98 KB of
-// LDS); FP64 fields 11 rows, one workgroup per CU (135 KB of LDS).
+// LDS); FP64 fields 12 rows, one workgroup per CU (147 KB of LDS).
 template <typename T> struct Yee2Shape {
-  static constexpr int R = sizeof(T) == 4 ? 16 : 11;
+  static constexpr int R = sizeof(T) == 4 ? 16 : 12;
   static constexpr int OY = R - 3;  // output rows per tile
   static constexpr int WAVES_PER_EU = (R + 3) / 4;
 };
