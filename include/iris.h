@@ -190,7 +190,9 @@ const char* iris_last_error(const iris_handle* h);
 #define IRIS_K_YEE2 7       /* two-step pass over the plain z-chunks        */
 #define IRIS_K_YEE_REST 8   /* within a two-step pass: one single step of   */
                             /* the other z-chunks                           */
-#define IRIS_NKERNEL 9
+#define IRIS_K_ADJOINT 9    /* one adjoint step (all its kernels)           */
+#define IRIS_K_CHECKPOINT 10 /* state snapshots, restores and H records     */
+#define IRIS_NKERNEL 11
 
 /* Enable (1) or disable (0) per-kernel timing with HIP events recorded
  * around every launch on the handle's stream.  No counterpart in the
@@ -208,6 +210,31 @@ int iris_profile_read(iris_handle* h, double* ms, int64_t* launches, int32_t res
  * of the two-step pass; info[3] = 1 when it is on.  No counterpart in the
  * reference; used to account the algorithmic bytes of the passes. */
 int iris_pass_info(const iris_handle* h, int64_t info[4]);
+
+/* ---- Gradient of a face-flux objective.  The reference has no adjoint;
+ * the differentiable form stands beside the plain solver on the same handle.
+ * Its forward run is the plain one (iris_step), so the flux it returns is
+ * the reference's (code/dataprocess.f90:241-277). ---- */
+
+/* Keep a copy of the whole field state (E, H and the UPML aux fields D, B)
+ * every `segment` steps of the forward run (before steps 1, segment+1, ...),
+ * for iris_adjoint's segment-wise recomputation.  Call before the first
+ * step.  Allocates ceil(nt/segment) snapshots on the device up front
+ * (IRIS_ENOMEM when they do not fit).  segment <= 0 frees them.  No
+ * counterpart in the reference. */
+int iris_checkpoint(iris_handle* h, int32_t segment);
+
+/* Gradient of J = sum_{iw, face} weights[iw*6 + face] * Jabs of box nobox
+ * (the iris_flux numbers at step nt) with respect to the per-cell factor
+ * eps_inv that the E update applies on the structure planes (reference
+ * code/fdtd.f90:115-117), on x in 1..nx, y in 1..ny, z in hi..hf.  grad
+ * receives nx*ny*(hf-hi+1) complex values, x fastest then y then z:
+ * (dJ/d Re eps_inv, dJ/d Im eps_inv).  The engine runs the discrete adjoint
+ * of its own time step backwards from nt, recomputing each segment forward
+ * from its snapshot, and restores the state at nt afterwards.  Needs the
+ * handle at step nt with iris_checkpoint called before the first step
+ * (IRIS_ESTATE otherwise).  Synchronous.  No counterpart in the reference. */
+int iris_adjoint(iris_handle* h, int32_t nobox, const double* weights, double* grad);
 
 /* Library version string. */
 const char* iris_version(void);

@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "../../include/iris.h"
+#include "iris_adjoint.h"
 #include "iris_kernels.h"
 
 using namespace iris;
@@ -113,6 +114,25 @@ struct iris_handle {
   std::vector<hipEvent_t> pool;
   double prof_ms[IRIS_NKERNEL] = {};
   int64_t prof_n[IRIS_NKERNEL] = {};
+  // gradient: forward snapshots every ck_seg steps (E, H, D, B back to back)
+  int ck_seg = 0;
+  std::vector<void*> ck;    // snapshot k holds the state after step k*ck_seg
+  std::vector<char> ck_ok;
+  void* ck_final = nullptr; // the state at nt while iris_adjoint recomputes
+  // adjoint fields and work buffers (allocated by the first iris_adjoint)
+  void* ae = nullptr;
+  void* ah = nullptr;
+  void* ad = nullptr;
+  void* ab = nullptr;
+  void* atmp = nullptr;
+  void* hrec = nullptr;     // H on planes hi-1..hf for each step of a segment
+  int64_t rec_plane = 0;    // complex values per component of one record
+  int rec_cap = 0;          // records hrec holds
+  double2* adj_g = nullptr; // dJ/d(emg) of the objective's box
+  int64_t adj_g_cap = 0;
+  double2* grad = nullptr;  // accumulated gradient on the structure cells
+  double* wdev = nullptr;   // objective weights (6*nof)
+  std::vector<void*> adj_allocs;
 };
 
 namespace {
@@ -754,6 +774,200 @@ int create_impl(iris_handle* h, const double* eps_inv, const double* fpmn, const
   return 0;
 }
 
+// ---- gradient: snapshots, recomputation, adjoint steps ---------------------
+
+size_t field_bytes(const iris_handle* h) { return h->csize * 3 * (size_t)h->cs; }
+size_t aux_bytes(const iris_handle* h) { return h->csize * 3 * (size_t)(h->ps0 + h->ps1); }
+size_t state_bytes(const iris_handle* h) { return 2 * field_bytes(h) + 2 * aux_bytes(h); }
+
+// copy the current state (E, H, D, B) into a snapshot buffer, or back
+int state_copy(iris_handle* h, void* snap, bool save) {
+  ProfScope ps(h, IRIS_K_CHECKPOINT);
+  const size_t fb = field_bytes(h), ab = aux_bytes(h);
+  char* s = (char*)snap;
+  void* parts[4] = {h->fe, h->fh, h->d, h->b};
+  const size_t off[4] = {0, fb, 2 * fb, 2 * fb + ab};
+  const size_t len[4] = {fb, fb, ab, ab};
+  for (int k = 0; k < 4; ++k) {
+    if (save) HIP_TRY(h, hipMemcpyAsync(s + off[k], parts[k], len[k], hipMemcpyDeviceToDevice, h->stream));
+    else HIP_TRY(h, hipMemcpyAsync(parts[k], s + off[k], len[k], hipMemcpyDeviceToDevice, h->stream));
+  }
+  return 0;
+}
+
+int adj_alloc(iris_handle* h, void** p, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    return fail(h, IRIS_ENOMEM, "hipMalloc(%zu bytes) for the adjoint failed: %s", bytes, hipGetErrorString(e));
+  }
+  h->adj_allocs.push_back(*p);
+  return 0;
+}
+
+// iris_step with snapshots: the steps are cut at the snapshot boundaries so
+// that the state after step k*ck_seg is whole when it is copied
+int step_with_checkpoints(iris_handle* h, int it_first, int it_last) {
+  int it = it_first;
+  while (it <= it_last) {
+    const int k = (it - 1) / h->ck_seg;
+    if ((it - 1) % h->ck_seg == 0 && k < (int)h->ck.size() && !h->ck_ok[k]) {
+      int rc = state_copy(h, h->ck[k], true);
+      if (rc) return rc;
+      h->ck_ok[k] = 1;
+    }
+    const int end = std::min(it_last, (k + 1) * h->ck_seg);
+    int rc = h->prec == 8 ? run_steps<double>(h, it, end) : run_steps<float>(h, it, end);
+    if (rc) return rc;
+    it = end + 1;
+  }
+  return 0;
+}
+
+// H of the current state on planes hi-1..hf into record slot r
+int record_h(iris_handle* h, int r) {
+  ProfScope ps(h, IRIS_K_CHECKPOINT);
+  const size_t n = h->csize * (size_t)h->rec_plane;
+  for (int c = 0; c < 3; ++c) {
+    const char* src = (const char*)h->fh + h->csize * ((size_t)h->cs * c + (size_t)h->sxy * (h->p.hi - 1));
+    char* dst = (char*)h->hrec + n * (3 * (size_t)r + c);
+    HIP_TRY(h, hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToDevice, h->stream));
+  }
+  return 0;
+}
+
+inline unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+// One step of the adjoint, the transpose of step n (the operations of the
+// forward step in reverse order; the TF/SF terms do not depend on the state).
+// face_on bit 2j+s: face half s of pair j carries objective weight.
+template <typename T>
+int adjoint_step(iris_handle* h, int n, int nobox, int face_on, int rec_slot) {
+  using C = cplx<T>;
+  const iris_params& p = h->p;
+  ProfScope ps(h, IRIS_K_ADJOINT);
+  StepArgs<T> a = step_args<T>(h);
+  C* ae = (C*)h->ae;
+  C* ah = (C*)h->ah;
+  C* tmp = (C*)h->atmp;
+  const int64_t ncells = (int64_t)p.nx * p.ny * p.nz;
+  const int64_t ncells1 = (int64_t)(p.nx + 1) * (p.ny + 1) * (p.nz + 1);
+  // 1. sources: the DFT sample taken after step n
+  if (n >= p.to_fourier && (n % p.sample_fourier) == 0 && face_on) {
+    hipLaunchKernelGGL(k_dft_phase, dim3((p.nof + 255) / 256), dim3(256), 0, h->stream, h->fw, p.nof, 1, 1,
+                       n + 1 - p.to_fourier, p.sample_fourier, 1.0, h->ph);
+    const DeviceBox& B = h->box[nobox - 1];
+    for (int j = 0; j < 3; ++j)
+      for (int s = 0; s < 2; ++s) {
+        if (!((face_on >> (2 * j + s)) & 1)) continue;
+        hipLaunchKernelGGL(k_adj_source<T>, dim3(blocks_for(B.g.npts[j] / 2)), dim3(256), 0, h->stream, B.g, j, s,
+                           (const double2*)h->adj_g, p.nof, (const double2*)h->ph, p.dt, ae, ah, h->cs, h->sx,
+                           h->sxy, h->xoff);
+      }
+  }
+  // 2. min-side wraps of H (ghost 0 <- exp(-i ak) H(n))
+  const int nn[3] = {p.nx, p.ny, p.nz};
+  const C fm[3] = {a.fxm, a.fym, a.fzm}, fp[3] = {a.fxp, a.fyp, a.fzp};
+  for (int ax = 0; ax < 3; ++ax) {
+    const int na = (ax == 0) ? p.ny + 2 : p.nx + 2, nb = (ax == 2) ? p.ny + 2 : p.nz + 2;
+    hipLaunchKernelGGL(k_adj_wrap<T>, dim3(blocks_for(3LL * na * nb)), dim3(256), 0, h->stream, ah, h->cs, ax, 0,
+                       nn[ax], fm[ax], p.nx, p.ny, p.nz, h->sx, h->sxy, h->xoff);
+  }
+  // 3. H update: local part, then E^ += curl_b(.)
+  a.d = (C*)h->ad;
+  a.b = (C*)h->ab;
+  hipLaunchKernelGGL(k_adj_h_local<T>, dim3(blocks_for(ncells)), dim3(256), 0, h->stream, a, ah, (C*)h->ab, tmp);
+  hipLaunchKernelGGL(k_adj_curl_b<T>, dim3(blocks_for(ncells1)), dim3(256), 0, h->stream, a, (const C*)tmp, ae);
+  // 4. max-side wraps of E (ghost n+1 <- exp(+i ak) E(1))
+  for (int ax = 0; ax < 3; ++ax) {
+    const int na = (ax == 0) ? p.ny + 2 : p.nx + 2, nb = (ax == 2) ? p.ny + 2 : p.nz + 2;
+    hipLaunchKernelGGL(k_adj_wrap<T>, dim3(blocks_for(3LL * na * nb)), dim3(256), 0, h->stream, ae, h->cs, ax,
+                       nn[ax] + 1, 1, fp[ax], p.nx, p.ny, p.nz, h->sx, h->sxy, h->xoff);
+  }
+  // 5. gradient on the structure cells with H of step n-1
+  const int64_t nstruct = (int64_t)p.nx * p.ny * (p.hf - p.hi + 1);
+  hipLaunchKernelGGL(k_adj_grad<T>, dim3(blocks_for(nstruct)), dim3(256), 0, h->stream, a, (const C*)ae,
+                     (const C*)h->hrec + 3 * h->rec_plane * (int64_t)rec_slot, h->rec_plane, h->grad);
+  // 6. E update: local part, then H^ += curl_f(.)
+  hipLaunchKernelGGL(k_adj_e_local<T>, dim3(blocks_for(ncells)), dim3(256), 0, h->stream, a, ae, (C*)h->ad, tmp);
+  hipLaunchKernelGGL(k_adj_curl_f<T>, dim3(blocks_for(ncells1)), dim3(256), 0, h->stream, a, (const C*)tmp, ah);
+  HIP_TRY(h, hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+int adjoint_impl(iris_handle* h, int nobox, const double* weights, double* grad_out) {
+  const iris_params& p = h->p;
+  const int nt = p.nt, seg = h->ck_seg;
+  const int nseg = (int)h->ck.size();
+  const size_t fb = field_bytes(h), abytes = aux_bytes(h);
+  int rc;
+  if ((rc = flush(h))) return rc;
+  // buffers
+  if (!h->ae) {
+    h->rec_plane = h->sxy * (p.hf - p.hi + 2);
+    h->rec_cap = seg;
+    const size_t gbytes = sizeof(double2) * (size_t)p.nx * p.ny * (p.hf - p.hi + 1);
+    if ((rc = adj_alloc(h, &h->ae, fb)) || (rc = adj_alloc(h, &h->ah, fb)) || (rc = adj_alloc(h, &h->atmp, fb)) ||
+        (rc = adj_alloc(h, &h->ad, abytes)) || (rc = adj_alloc(h, &h->ab, abytes)) ||
+        (rc = adj_alloc(h, &h->hrec, h->csize * 3 * (size_t)h->rec_plane * h->rec_cap)) ||
+        (rc = adj_alloc(h, (void**)&h->grad, gbytes)) ||
+        (rc = adj_alloc(h, (void**)&h->wdev, sizeof(double) * 6 * p.nof)) ||
+        (rc = adj_alloc(h, &h->ck_final, state_bytes(h))))
+      return rc;
+  }
+  if (h->rec_cap < seg) {
+    if ((rc = adj_alloc(h, &h->hrec, h->csize * 3 * (size_t)h->rec_plane * seg))) return rc;
+    h->rec_cap = seg;
+  }
+  const DeviceBox& B = h->box[nobox - 1];
+  const int64_t gneed = std::max<int64_t>(B.g.ntot, 1) * p.nof;
+  if (gneed > h->adj_g_cap) {
+    if ((rc = adj_alloc(h, (void**)&h->adj_g, sizeof(double2) * (size_t)gneed))) return rc;
+    h->adj_g_cap = gneed;
+  }
+  // dJ/d(emg) of the weighted faces
+  int face_on = 0;
+  for (int j = 0; j < 3; ++j)
+    for (int s = 0; s < 2; ++s) {
+      bool any = false;
+      for (int iw = 0; iw < p.nof; ++iw) any = any || weights[iw * 6 + 2 * j + s] != 0.0;
+      if (any && B.g.npts[j]) face_on |= 1 << (2 * j + s);
+    }
+  HIP_TRY(h, hipMemcpyAsync(h->wdev, weights, sizeof(double) * 6 * p.nof, hipMemcpyHostToDevice, h->stream));
+  HIP_TRY(h, hipMemsetAsync(h->adj_g, 0, sizeof(double2) * (size_t)gneed, h->stream));
+  const double sflux = (p.eps0 * p.q) / (2.0 * p.dt);
+  for (int j = 0; j < 3; ++j)
+    if (B.g.npts[j])
+      hipLaunchKernelGGL(k_adj_g, dim3(blocks_for(B.g.npts[j])), dim3(256), 0, h->stream, B.g, j,
+                         (const double2*)B.emg, p.nof, (const double*)h->wdev, sflux, h->adj_g);
+  HIP_TRY(h, hipGetLastError());
+  // zero adjoint state and gradient; keep the state at nt
+  for (void* f : {h->ae, h->ah, h->atmp}) HIP_TRY(h, hipMemsetAsync(f, 0, fb, h->stream));
+  for (void* f : {h->ad, h->ab}) HIP_TRY(h, hipMemsetAsync(f, 0, abytes, h->stream));
+  const size_t gbytes = sizeof(double2) * (size_t)p.nx * p.ny * (p.hf - p.hi + 1);
+  HIP_TRY(h, hipMemsetAsync(h->grad, 0, gbytes, h->stream));
+  if ((rc = state_copy(h, h->ck_final, true))) return rc;
+  for (int k = nseg - 1; k >= 0; --k) {
+    const int s0 = k * seg, s1 = std::min(nt, s0 + seg);
+    // forward again from the snapshot, recording H of steps s0..s1-1
+    if ((rc = state_copy(h, h->ck[k], false))) return rc;
+    if ((rc = record_h(h, 0))) return rc;
+    for (int n = s0 + 1; n < s1; ++n) {
+      if ((rc = launch_step<T>(h, n))) return rc;
+      if ((rc = record_h(h, n - s0))) return rc;
+    }
+    for (int n = s1; n > s0; --n)
+      if ((rc = adjoint_step<T>(h, n, nobox, face_on, n - 1 - s0))) return rc;
+  }
+  if ((rc = state_copy(h, h->ck_final, false))) return rc;
+  HIP_TRY(h, hipMemcpyAsync(grad_out, h->grad, gbytes, hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
 void release(iris_handle* h) {
   if (!h) return;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
@@ -762,6 +976,12 @@ void release(iris_handle* h) {
   h->pool.clear();
   for (void* p : h->allocs) (void)hipFree(p);
   h->allocs.clear();
+  for (void* p : h->ck) (void)hipFree(p);
+  h->ck.clear();
+  h->ck_ok.clear();
+  h->ck_final = nullptr;  // one of adj_allocs
+  for (void* p : h->adj_allocs) (void)hipFree(p);
+  h->adj_allocs.clear();
   if (h->diag_buf) (void)hipFree(h->diag_buf);
   h->diag_buf = nullptr;
   if (h->plane_buf) (void)hipFree(h->plane_buf);
@@ -825,7 +1045,46 @@ int iris_step(iris_handle* h, int32_t it_first, int32_t it_last) {
   if (it_last < it_first) return 0;
   if (it_last > h->p.nt) return fail(h, IRIS_ESTATE, "it_last=%d is past nt=%d", it_last, h->p.nt);
   (void)hipSetDevice(h->device);
+  if (h->ck_seg > 0) return step_with_checkpoints(h, it_first, it_last);
   return h->prec == 8 ? run_steps<double>(h, it_first, it_last) : run_steps<float>(h, it_first, it_last);
+}
+
+int iris_checkpoint(iris_handle* h, int32_t segment) {
+  if (!h) return fail(nullptr, IRIS_EINVAL, "handle is NULL");
+  if (h->it != 0) return fail(h, IRIS_ESTATE, "iris_checkpoint: call it before the first step (now at step %d)", h->it);
+  (void)hipSetDevice(h->device);
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  for (void* p : h->ck) (void)hipFree(p);
+  h->ck.clear();
+  h->ck_ok.clear();
+  h->ck_seg = 0;
+  if (segment <= 0) return 0;
+  const int n = (h->p.nt + segment - 1) / segment;
+  const size_t bytes = state_bytes(h);
+  for (int k = 0; k < std::max(n, 1); ++k) {
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      for (void* q : h->ck) (void)hipFree(q);
+      h->ck.clear();
+      return fail(h, IRIS_ENOMEM, "iris_checkpoint: %d snapshots of %zu bytes do not fit", n, bytes);
+    }
+    h->ck.push_back(p);
+  }
+  h->ck_ok.assign(h->ck.size(), 0);
+  h->ck_seg = segment;
+  return 0;
+}
+
+int iris_adjoint(iris_handle* h, int32_t nobox, const double* weights, double* grad) {
+  if (!h || !weights || !grad) return fail(h, IRIS_EINVAL, "NULL argument");
+  if (nobox < 1 || nobox > h->p.nbox) return fail(h, IRIS_EINVAL, "box %d does not exist", nobox);
+  if (h->ck_seg <= 0) return fail(h, IRIS_ESTATE, "iris_adjoint: no snapshots (call iris_checkpoint before the first step)");
+  if (h->it != h->p.nt) return fail(h, IRIS_ESTATE, "iris_adjoint: the forward run is at step %d, not nt=%d", h->it, h->p.nt);
+  for (size_t k = 0; k < h->ck_ok.size(); ++k)
+    if (!h->ck_ok[k]) return fail(h, IRIS_ESTATE, "iris_adjoint: snapshot %zu was not taken", k);
+  (void)hipSetDevice(h->device);
+  return h->prec == 8 ? adjoint_impl<double>(h, nobox, weights, grad) : adjoint_impl<float>(h, nobox, weights, grad);
 }
 
 int iris_sync(iris_handle* h) {

@@ -95,6 +95,10 @@ def load_library(path=None):
     lib.iris_diagnostics.restype = ct.c_int
     lib.iris_pass_info.argtypes = [H, ct.c_void_p]
     lib.iris_pass_info.restype = ct.c_int
+    lib.iris_checkpoint.argtypes = [H, ct.c_int32]
+    lib.iris_checkpoint.restype = ct.c_int
+    lib.iris_adjoint.argtypes = [H, ct.c_int32, ct.c_void_p, ct.c_void_p]
+    lib.iris_adjoint.restype = ct.c_int
     lib.iris_current_step.argtypes = [H, ct.POINTER(ct.c_int32)]
     lib.iris_current_step.restype = ct.c_int
     lib.iris_destroy.argtypes = [H]
@@ -114,9 +118,10 @@ def load_library(path=None):
 EXPORTED = ["iris_create", "iris_step", "iris_sync", "iris_flux", "iris_dft_points", "iris_get_dft",
             "iris_get_field", "iris_set_field", "iris_get_plane", "iris_diagnostics", "iris_current_step",
             "iris_destroy", "iris_last_error", "iris_version", "iris_profile", "iris_profile_read",
-            "iris_pass_info"]
+            "iris_pass_info", "iris_checkpoint", "iris_adjoint"]
 
-KERNELS = ["epass", "hpass", "tfsf", "dft_sample", "dft_accum", "flux", "yee", "yee2", "yee_rest"]
+KERNELS = ["epass", "hpass", "tfsf", "dft_sample", "dft_accum", "flux", "yee", "yee2", "yee_rest", "adjoint",
+           "checkpoint"]
 
 
 class Engine:
@@ -214,6 +219,22 @@ class Engine:
         v = np.zeros(4, dtype=np.int64)
         self._check(self.lib.iris_pass_info(self._h, _p(v)), "iris_pass_info")
         return dict(zchunk=int(v[0]), two_step_planes=int(v[1]), two_step_chunks=int(v[2]), two_step=bool(v[3]))
+
+    def checkpoint(self, segment):
+        """Keep a snapshot of the state every `segment` steps of the coming
+        forward run (call before the first step); segment <= 0 turns it off."""
+        self._check(self.lib.iris_checkpoint(self._h, int(segment)), "iris_checkpoint")
+
+    def adjoint(self, nobox, weights):
+        """dJ/d eps_inv on the structure cells for J = sum(weights * Jabs)
+        (weights shaped like flux(): (nof, 6)), as complex (hf-hi+1, ny, nx):
+        real part dJ/d Re eps_inv, imaginary part dJ/d Im eps_inv.  Needs the
+        forward run at nt with checkpoint() called before it."""
+        p = self.params
+        w = np.ascontiguousarray(weights, dtype=np.float64).reshape(p.nof, 6)
+        out = np.zeros(2 * p.nx * p.ny * (p.hf - p.hi + 1), dtype=np.float64)
+        self._check(self.lib.iris_adjoint(self._h, int(nobox), _p(w), _p(out)), "iris_adjoint")
+        return out.view(np.complex128).reshape(p.hf - p.hi + 1, p.ny, p.nx)
 
     def set_field(self, which, arr):
         a = np.ascontiguousarray(arr, dtype=np.complex128)
