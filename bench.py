@@ -44,13 +44,25 @@ WORKLOAD = dict(description="slit-array film 512x512x1024 (roofline workload)",
                 nt=NT_CONFIG, no_downloads=10, to_fourier=0, sample_fourier=1)
 
 
+# gradient workload: a random design layer (eps_r uniform in [1, 4] per cell,
+# seed 4) of 128 x 128 x 64 cells inside a 128 x 128 x 768 grid, objective
+# the transmission T = J6/(J5+J6) summed over 10 wavelengths in 500-900 nm,
+# FP32 fields, N = 12000 steps, forward snapshots every ceil(sqrt(N)) steps
+ADJ_NT = 12000
+ADJ_WORKLOAD = dict(description="random design layer 128x128x64 in 128x128x768 (gradient workload)",
+                    ixmax=128, iymax=128, izmax=768, pmlzi=20, pmlzf=749, hi=353, dh=64, dx=0,
+                    eps1=1.0, eps_bbox=2.5, eps3=1.0, tfsf_o=30, wlpump=700.0, pump_dt=4.0, def_pulse=4,
+                    box=(1, 128, 1, 128, 25, 744), nof=10, dlambda=400.0 / 9.0, lno=500.0,
+                    nt=ADJ_NT, no_downloads=ADJ_NT, to_fourier=0, sample_fourier=1)
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def make_setup(precision, **over):
+def make_setup(precision, base=None, **over):
     from pyiris import inputs, load_input
-    kw = dict(WORKLOAD)
+    kw = dict(base or WORKLOAD)
     kw.update(over)
     with tempfile.TemporaryDirectory() as d:
         path = os.path.join(d, "inputFDTD.dat")
@@ -138,6 +150,64 @@ def run_gpu(precision, steps, warmup):
                 step_two_pass_frac=ab["step_two_pass"] / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS)
 
 
+def run_adjoint(precision=4):
+    """Wall time of the plain forward run and of the gradient (forward with
+    snapshots, then the adjoint sweep with segment-wise recomputation) on
+    the gradient workload; both with inputs resident on the device."""
+    import math
+
+    import numpy as np
+    from pyiris import Engine
+    from pyiris.adjoint import structure_of, with_structure
+    s0 = make_setup(precision, ADJ_WORKLOAD)
+    rng = np.random.default_rng(4)
+    s = with_structure(s0, 1.0 / rng.uniform(1.0, 4.0, size=structure_of(s0).shape))
+    p = s.params
+    seg = int(math.ceil(math.sqrt(p.nt)))
+    log(f"adjoint: {p.nx}x{p.ny}x{p.nz}, design {p.nx}x{p.ny}x{p.hf - p.hi + 1}, nt={p.nt}, segment={seg}")
+    with Engine(s, precision=precision) as eng:   # warm-up: module load, first launches
+        eng.run_to(50)
+        eng.sync()
+    with Engine(s, precision=precision) as eng:
+        eng.sync()
+        t0 = time.perf_counter()
+        eng.run_to(p.nt)
+        j = eng.flux(1)
+        t_fwd = time.perf_counter() - t0
+    with Engine(s, precision=precision) as eng:
+        eng.checkpoint(seg)
+        eng.sync()
+        t0 = time.perf_counter()
+        eng.run_to(p.nt)
+        j2 = eng.flux(1)
+        t_mid = time.perf_counter()
+        eng.profile(True)
+        # weights of sum_w T(w), T = J6/(J5+J6)
+        den = j2[:, 4] + j2[:, 5]
+        w = np.zeros_like(j2)
+        w[:, 4] = -j2[:, 5] / den ** 2
+        w[:, 5] = j2[:, 4] / den ** 2
+        g = eng.adjoint(1, w)
+        t_grad = time.perf_counter() - t0
+        prof = eng.profile_read(reset=True)
+        snap = len(range(0, p.nt, seg))
+    T = float((j[:, 5] / (j[:, 4] + j[:, 5])).sum())
+    adj_ms = prof["adjoint"][0]
+    out = {"workload": "random design layer 128x128x64 (eps_r U[1,4], seed 4) in 128x128x768, z-UPML 20, "
+                       "objective sum T over 10 wavelengths 500-900 nm",
+           "grid": [p.nx, p.ny, p.nz], "design": [p.nx, p.ny, p.hf - p.hi + 1], "nt": p.nt, "segment": seg,
+           "snapshots": snap, "fields": "fp32" if precision == 4 else "fp64",
+           "forward_s": round(t_fwd, 4), "gradient_s": round(t_grad, 4),
+           "gradient_forward_part_s": round(t_mid - t0, 4),
+           "ratio": round(t_grad / t_fwd, 3),
+           "adjoint_ms_per_step": round(adj_ms / p.nt, 4),
+           "recompute_ms_per_step": round((prof["yee"][0] + prof["checkpoint"][0]) / p.nt, 4),
+           "objective_sum_T": T, "forward_flux_equal": bool(np.array_equal(j, j2)),
+           "grad_max_abs": float(np.abs(g).max())}
+    log(f"adjoint: forward {t_fwd:.2f} s, gradient {t_grad:.2f} s, ratio {t_grad / t_fwd:.2f}")
+    return out
+
+
 def measured_traffic(kernel, precision):
     """HBM bytes per launch from the committed rocprofv3 PMC summary, if any."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_fp{32 if precision == 4 else 64}_pmc.json")
@@ -200,6 +270,7 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this engine runs on one GPU; a run on several GPUs (one z-slab per GPU with "
@@ -216,6 +287,7 @@ def main():
         d64 = r64["kernels"][r64["dominant"]]
         log(f"fp64: {r64['gcells']:.2f} Gcell/s, {r64['ms_per_step']:.3f} ms/step, {r64['dominant']} "
             f"{d64.get('gbs', 0):.0f} GB/s")
+    adj = None if a.no_adjoint else run_adjoint(4)
     cpu = None if a.no_cpu else cpu_baseline()
     if cpu:
         log(f"cpu baseline: {cpu['value'] * 1e3:.2f} Mcell/s")
@@ -261,6 +333,8 @@ def main():
                         "two_pass_bytes_roofline_frac": round(r64["step_two_pass_frac"], 4),
                         "steps": a.fp64_steps or max(a.steps // 2, 1),
                         "kernels_us": {n: round(v["avg_us"], 2) for n, v in r64["kernels"].items()}}
+    if adj:
+        line["adjoint"] = adj
     if cpu:
         line["speedup_vs_cpu"] = round(r32["gcells"] / cpu["value"], 1)
     print(json.dumps(line), flush=True)

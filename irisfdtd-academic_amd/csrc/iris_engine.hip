@@ -46,6 +46,15 @@ struct DeviceBox {
   void* samp = nullptr;
 };
 
+// chunk lists of a two-step pass: two-step chunks (z0 | length | export
+// flags) and the chunks left to single steps, lean ones first
+struct PassLists {
+  const int* chunks2 = nullptr;
+  int n2 = 0;
+  const int* rest = nullptr;
+  int nlean = 0, ngen = 0;
+};
+
 }  // namespace
 
 struct iris_handle {
@@ -125,6 +134,23 @@ struct iris_handle {
   void* ad = nullptr;
   void* ab = nullptr;
   void* atmp = nullptr;
+  void* ae2 = nullptr;      // fused adjoint step: the other adjoint buffers
+  void* ah2 = nullptr;
+  void* ad2 = nullptr;
+  void* ab2 = nullptr;
+  void* ae3 = nullptr;      // third adjoint E and H (two-step adjoint passes)
+  void* ah3 = nullptr;
+  // two-step lists of the gradient sweeps: the structure planes hi-1..hf,
+  // the TF/SF planes and the planes of weighted faces stay in single-step
+  // chunks (their H is recorded, or sources enter between the two steps)
+  bool gl_on = false;
+  int gl_key = -1;          // (nobox, face mask) the lists were made for
+  int* gl_chunks2 = nullptr;
+  int gl_n2 = 0;
+  int* gl_rest = nullptr;
+  int gl_nlean = 0, gl_ngen = 0;
+  bool adj_fused = true;    // fused adjoint step (IRIS_ADJ_FUSED=0: one kernel per part)
+  int adj_ty = 0;           // tile rows of the general adjoint pass (0: 7 with FP32 fields, 8 with FP64)
   void* hrec = nullptr;     // H on planes hi-1..hf for each step of a segment
   int64_t rec_plane = 0;    // complex values per component of one record
   int rec_cap = 0;          // records hrec holds
@@ -322,35 +348,35 @@ int dft_flush(iris_handle* h);
 // single-step launches (current -> second buffer -> third), then the DFT
 // samples of both steps.
 template <typename T>
-int launch_double(iris_handle* h, int it) {
+int launch_double(iris_handle* h, int it, const PassLists& L, bool dft) {
   using C = cplx<T>;
   const iris_params& p = h->p;
-  const bool s1 = it >= p.to_fourier && (it % p.sample_fourier) == 0;
-  const bool s2 = it + 1 >= p.to_fourier && ((it + 1) % p.sample_fourier) == 0;
+  const bool s1 = dft && it >= p.to_fourier && (it % p.sample_fourier) == 0;
+  const bool s2 = dft && it + 1 >= p.to_fourier && ((it + 1) % p.sample_fourier) == 0;
   Yee2Samples<T> smp{};
   smp.nbox = p.nbox;
   for (int b = 0; b < p.nbox; ++b) {
     smp.g[b] = h->box[b].g;
     smp.samp[b] = (s1 && h->box[b].g.ntot) ? (C*)h->box[b].samp + (int64_t)h->nsamp * h->box[b].g.ntot : nullptr;
   }
-  const int* rl = h->chunks_rest;
-  const int* rg = h->chunks_rest + h->nrest_lean;
-  if (h->nrest_lean + h->nrest_gen) {
+  const int* rl = L.rest;
+  const int* rg = L.rest + L.nlean;
+  if (L.nlean + L.ngen) {
     ProfScope ps(h, IRIS_K_YEE_REST);
     const StepArgs<T> a1 = step_args_on<T>(h, h->fe, h->fh, h->ge, h->gh, h->d, h->b, h->dn, h->bn);
-    launch_fused_lists<T>(h, a1, it, rl, h->nrest_lean, rg, h->nrest_gen);
+    launch_fused_lists<T>(h, a1, it, rl, L.nlean, rg, L.ngen);
   }
   {
     ProfScope ps(h, IRIS_K_YEE2);
     const StepArgs<T> a2 = step_args_on<T>(h, h->fe, h->fh, h->ke, h->kh, h->d, h->b, h->dn, h->bn);
-    const TileMap tm{h->tx2, h->ty2, h->tx2 * h->ty2 * h->n2, h->tx2 * h->xcd_rows2};
+    const TileMap tm{h->tx2, h->ty2, h->tx2 * h->ty2 * L.n2, h->tx2 * h->xcd_rows2};
     hipLaunchKernelGGL(k_yee2<T>, dim3((unsigned)tm.nblk), dim3(kT2X, Yee2Shape<T>::R), 0, h->stream, a2, (C*)h->ge,
-                       (C*)h->gh, (const int*)h->chunks2, tm, smp);
+                       (C*)h->gh, L.chunks2, tm, smp);
   }
-  if (h->nrest_lean + h->nrest_gen) {
+  if (L.nlean + L.ngen) {
     ProfScope ps(h, IRIS_K_YEE_REST);
     const StepArgs<T> a3 = step_args_on<T>(h, h->ge, h->gh, h->ke, h->kh, h->dn, h->bn, h->d, h->b);
-    launch_fused_lists<T>(h, a3, it + 1, rl, h->nrest_lean, rg, h->nrest_gen);
+    launch_fused_lists<T>(h, a3, it + 1, rl, L.nlean, rg, L.ngen);
   }
   HIP_TRY(h, hipGetLastError());
   if (s1) {
@@ -467,8 +493,14 @@ int run_steps(iris_handle* h, int it_first, int it_last) {
   int it = it_first;
   // two steps per pass while at least three steps remain: every call ends
   // with a single-step pass, so the previous step's E and H stay whole
+  PassLists L;
+  L.chunks2 = h->chunks2;
+  L.n2 = h->n2;
+  L.rest = h->chunks_rest;
+  L.nlean = h->nrest_lean;
+  L.ngen = h->nrest_gen;
   while (h->two_step && it_last - it + 1 >= 3) {
-    int rc = launch_double<T>(h, it);
+    int rc = launch_double<T>(h, it, L, true);
     if (rc) return rc;
     it += 2;
   }
@@ -826,12 +858,12 @@ int step_with_checkpoints(iris_handle* h, int it_first, int it_last) {
   return 0;
 }
 
-// H of the current state on planes hi-1..hf into record slot r
-int record_h(iris_handle* h, int r) {
+// H on planes hi-1..hf of field buffer f into record slot r
+int record_h(iris_handle* h, const void* f, int r) {
   ProfScope ps(h, IRIS_K_CHECKPOINT);
   const size_t n = h->csize * (size_t)h->rec_plane;
   for (int c = 0; c < 3; ++c) {
-    const char* src = (const char*)h->fh + h->csize * ((size_t)h->cs * c + (size_t)h->sxy * (h->p.hi - 1));
+    const char* src = (const char*)f + h->csize * ((size_t)h->cs * c + (size_t)h->sxy * (h->p.hi - 1));
     char* dst = (char*)h->hrec + n * (3 * (size_t)r + c);
     HIP_TRY(h, hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToDevice, h->stream));
   }
@@ -897,6 +929,202 @@ int adjoint_step(iris_handle* h, int n, int nobox, int face_on, int rec_slot) {
   return 0;
 }
 
+// StepArgs of the fused adjoint passes: the forward geometry and
+// coefficients with the adjoint's Bloch factors, conj(exp(-+i ak)) in place
+// of exp(+-i ak) (fxm forms the min-side ghosts of Y, fxp the max-side
+// images of X)
+template <typename T>
+StepArgs<T> adj_args(const iris_handle* h, void* e, void* hf, void* en, void* hn, void* d, void* b, void* dn,
+                     void* bn) {
+  StepArgs<T> a = step_args_on<T>(h, e, hf, en, hn, d, b, dn, bn);
+  const cplx<T> xp = a.fxp, yp = a.fyp, zp = a.fzp;
+  a.fxp = cconj(a.fxm); a.fyp = cconj(a.fym); a.fzp = cconj(a.fzm);
+  a.fxm = cconj(xp); a.fym = cconj(yp); a.fzm = cconj(zp);
+  return a;
+}
+
+// objective sources of step n into the adjoint state held by a (e, h)
+template <typename T>
+void adj_sources(iris_handle* h, const StepArgs<T>& a, int n, int nobox, int face_on) {
+  const iris_params& p = h->p;
+  if (!(n >= p.to_fourier && (n % p.sample_fourier) == 0 && face_on)) return;
+  hipLaunchKernelGGL(k_dft_phase, dim3((p.nof + 255) / 256), dim3(256), 0, h->stream, h->fw, p.nof, 1, 1,
+                     n + 1 - p.to_fourier, p.sample_fourier, 1.0, h->ph);
+  const DeviceBox& B = h->box[nobox - 1];
+  for (int j = 0; j < 3; ++j)
+    for (int s = 0; s < 2; ++s) {
+      if (!((face_on >> (2 * j + s)) & 1)) continue;
+      hipLaunchKernelGGL(k_adj_source_mixed<T>, dim3(blocks_for(B.g.npts[j] / 2)), dim3(256), 0, h->stream, a, B.g,
+                         j, s, (const double2*)h->adj_g, p.nof, (const double2*)h->ph, p.dt);
+    }
+}
+
+// one fused adjoint step over lists of lean chunks (the forward's lean pass
+// on the plain planes) and general chunks (k_adj_gen)
+template <typename T>
+void adj_lists(iris_handle* h, const StepArgs<T>& a, const int* lean, int nlean, const int* gen, int ngen, int slot) {
+  using C = cplx<T>;
+  const C* rec = (const C*)h->hrec + 3 * h->rec_plane * (int64_t)slot;
+  TileMap tm{(int)h->grd_fused.x, (int)h->grd_fused.y, 0, (int)h->grd_fused.x * h->xcd_rows};
+  if (nlean) {
+    tm.nblk = tm.tx * tm.ty * nlean;
+    hipLaunchKernelGGL((k_yee_fused<T, false>), dim3(tm.nblk), h->blk_fused, 0, h->stream, a,
+                       (const TfsfConst*)h->tfc, 0, lean, tm);
+  }
+  if (ngen) {
+    // tile rows of the general adjoint pass: 7 rows + the halo row make 8
+    // waves, two workgroups per CU at the FP32 kernel's 122 VGPRs (tuning:
+    // IRIS_ADJ_TY = 7, 8 or 15)
+    const int ty = h->adj_ty ? h->adj_ty : (sizeof(T) == 4 ? 7 : 8);
+    TileMap tg{(int)h->grd_fused.x, (h->p.ny + ty - 1) / ty, 0, 0};
+    tg.run = tg.tx * h->xcd_rows;
+    tg.nblk = tg.tx * tg.ty * ngen;
+    if (ty == 7)
+      hipLaunchKernelGGL((k_adj_gen<T, 7>), dim3(tg.nblk), dim3(kFTX, 8), 0, h->stream, a, gen, tg, rec,
+                         h->rec_plane, h->grad);
+    else if (ty == 15)
+      hipLaunchKernelGGL((k_adj_gen<T, 15>), dim3(tg.nblk), dim3(kFTX, 16), 0, h->stream, a, gen, tg, rec,
+                         h->rec_plane, h->grad);
+    else
+      hipLaunchKernelGGL((k_adj_gen<T, 8>), dim3(tg.nblk), dim3(kFTX, 9), 0, h->stream, a, gen, tg, rec,
+                         h->rec_plane, h->grad);
+  }
+}
+
+// adjoint of step n: A -> B, then A and B swap
+template <typename T>
+int adjoint_single(iris_handle* h, int n, int nobox, int face_on, int slot) {
+  ProfScope ps(h, IRIS_K_ADJOINT);
+  const StepArgs<T> a = adj_args<T>(h, h->ae, h->ah, h->ae2, h->ah2, h->ad, h->ab, h->ad2, h->ab2);
+  adj_sources<T>(h, a, n, nobox, face_on);
+  adj_lists<T>(h, a, (const int*)h->chunks, h->nbulk, (const int*)h->chunks + h->nbulk, h->ngen, slot);
+  HIP_TRY(h, hipGetLastError());
+  std::swap(h->ae, h->ae2);
+  std::swap(h->ah, h->ah2);
+  std::swap(h->ad, h->ad2);
+  std::swap(h->ab, h->ab2);
+  return 0;
+}
+
+// adjoint of steps n and n-1: the two-step pass on the gradient lists' plain
+// chunks (A -> C, exports into B), the other chunks by two single steps
+// (A -> B -> C), the sources of step n-1 into B between them
+template <typename T>
+int adjoint_double(iris_handle* h, int n, int nobox, int face_on, int slot) {
+  using C = cplx<T>;
+  ProfScope ps(h, IRIS_K_ADJOINT);
+  const int* rl = h->gl_rest;
+  const int* rg = h->gl_rest + h->gl_nlean;
+  const StepArgs<T> a1 = adj_args<T>(h, h->ae, h->ah, h->ae2, h->ah2, h->ad, h->ab, h->ad2, h->ab2);
+  adj_sources<T>(h, a1, n, nobox, face_on);
+  adj_lists<T>(h, a1, rl, h->gl_nlean, rg, h->gl_ngen, slot);
+  {
+    const StepArgs<T> a2 = adj_args<T>(h, h->ae, h->ah, h->ae3, h->ah3, h->ad, h->ab, h->ad2, h->ab2);
+    const TileMap tm{h->tx2, h->ty2, h->tx2 * h->ty2 * h->gl_n2, h->tx2 * h->xcd_rows2};
+    Yee2Samples<T> none{};
+    hipLaunchKernelGGL(k_yee2<T>, dim3((unsigned)tm.nblk), dim3(kT2X, Yee2Shape<T>::R), 0, h->stream, a2,
+                       (C*)h->ae2, (C*)h->ah2, (const int*)h->gl_chunks2, tm, none);
+  }
+  const StepArgs<T> a3 = adj_args<T>(h, h->ae2, h->ah2, h->ae3, h->ah3, h->ad2, h->ab2, h->ad, h->ab);
+  adj_sources<T>(h, a3, n - 1, nobox, face_on);
+  adj_lists<T>(h, a3, rl, h->gl_nlean, rg, h->gl_ngen, slot - 1);
+  HIP_TRY(h, hipGetLastError());
+  std::swap(h->ae, h->ae3);
+  std::swap(h->ah, h->ah3);
+  return 0;
+}
+
+// Two-step lists of the gradient sweeps for box nobox with weighted faces
+// face_on (bit 2j+s): two-step chunks need every plane z0-2 .. z1+2 between
+// the UPML slabs, off the structure and record planes hi-1..hf, off the TF/SF
+// planes and off the planes holding weighted face points (the z-faces, or the
+// box's whole z range when an x or y face is weighted).
+template <typename T>
+int build_grad_lists(iris_handle* h, int nobox, int face_on) {
+  const iris_params& p = h->p;
+  const int key = nobox * 64 + face_on;
+  if (h->gl_key == key) return 0;
+  h->gl_key = key;
+  h->gl_on = false;
+  h->gl_n2 = h->gl_nlean = h->gl_ngen = 0;
+  const BoxGeom& g = h->box[nobox - 1].g;
+  const bool side = (face_on & 0xf) != 0;
+  auto eligible = [&](int z) {
+    if (z < 1 || z <= p.pmlzi || z >= p.pmlzf) return false;
+    if (z >= p.hi - 1 && z <= p.hf) return false;
+    if (p.em_source && (z == p.iz_min - 1 || z == p.iz_min)) return false;
+    if (side && z >= g.bzi && z <= g.bzf) return false;
+    if (((face_on >> 4) & 1) && z == g.bzi) return false;
+    if (((face_on >> 5) & 1) && z == g.bzf) return false;
+    return true;
+  };
+  auto is_bulk_plane = [&](int z) {
+    if (z <= p.pmlzi || z >= p.pmlzf) return false;
+    if (z >= p.hi && z <= p.hf) return false;
+    if (p.em_source && (z == p.iz_min - 1 || z == p.iz_min)) return false;
+    return true;
+  };
+  const bool no_phase = p.ak[0][0] == 0.0 && p.ak[0][1] == 0.0 && p.ak[1][0] == 0.0 && p.ak[1][1] == 0.0;
+  const bool shape = h->two_step && no_phase && p.nx >= kT2X && p.ny >= Yee2Shape<T>::R;
+  const int zc = h->zchunk;
+  std::vector<char> dbl;
+  int ndbl = 0;
+  for (int z0 = 1; z0 <= p.nz; z0 += zc) {
+    const int z1 = std::min(z0 + zc - 1, p.nz);
+    bool ok = shape && z1 + 2 <= p.nz && z1 - z0 + 1 == zc;
+    for (int z = z0 - 2; ok && z <= z1 + 2; ++z) ok = eligible(z);
+    dbl.push_back(ok ? 1 : 0);
+    ndbl += ok;
+  }
+  if (ndbl == 0) return 0;
+  int zc2 = std::min(2 * zc, kChunkLenMask + 1);
+  const char* z2env = std::getenv("IRIS_ZCHUNK2");
+  if (z2env && std::atoi(z2env) > 0) zc2 = std::min(std::atoi(z2env), kChunkLenMask + 1);
+  std::vector<int> c2, rlean, rgen;
+  const int nch = (int)dbl.size();
+  for (int k = 0; k < nch;) {
+    const int z0 = 1 + k * zc;
+    if (!dbl[k]) {
+      const int z1 = std::min(z0 + zc - 1, p.nz);
+      bool lean = z1 < p.nz;
+      for (int z = z0 - 1; lean && z <= z1 + 1; ++z) lean = is_bulk_plane(z);
+      (lean ? rlean : rgen).push_back(z0);
+      ++k;
+      continue;
+    }
+    int k1 = k;
+    while (k1 + 1 < nch && dbl[k1 + 1]) ++k1;
+    const int r0 = z0, r1 = 1 + (k1 + 1) * zc - 1;
+    for (int c0 = r0; c0 <= r1; c0 += zc2) {
+      const int len = std::min(zc2, r1 - c0 + 1);
+      int v = c0 | ((len - 1) << kChunkLenShift);
+      if (c0 == r0) v |= kChunkExportBelow;
+      if (c0 + len - 1 == r1) v |= kChunkExportAbove;
+      c2.push_back(v);
+    }
+    k = k1 + 1;
+  }
+  std::vector<int> rest(rlean);
+  rest.insert(rest.end(), rgen.begin(), rgen.end());
+  int rc;
+  if (!h->gl_chunks2) {
+    // sized for the most chunks any face mask can give
+    const size_t cap = (size_t)nch + 1;
+    if ((rc = adj_alloc(h, (void**)&h->gl_chunks2, sizeof(int) * cap)) ||
+        (rc = adj_alloc(h, (void**)&h->gl_rest, sizeof(int) * cap)))
+      return rc;
+  }
+  HIP_TRY(h, hipMemcpyAsync(h->gl_chunks2, c2.data(), sizeof(int) * c2.size(), hipMemcpyHostToDevice, h->stream));
+  if (!rest.empty())
+    HIP_TRY(h, hipMemcpyAsync(h->gl_rest, rest.data(), sizeof(int) * rest.size(), hipMemcpyHostToDevice, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  h->gl_n2 = (int)c2.size();
+  h->gl_nlean = (int)rlean.size();
+  h->gl_ngen = (int)rgen.size();
+  h->gl_on = true;
+  return 0;
+}
+
 template <typename T>
 int adjoint_impl(iris_handle* h, int nobox, const double* weights, double* grad_out) {
   const iris_params& p = h->p;
@@ -916,6 +1144,12 @@ int adjoint_impl(iris_handle* h, int nobox, const double* weights, double* grad_
         (rc = adj_alloc(h, (void**)&h->grad, gbytes)) ||
         (rc = adj_alloc(h, (void**)&h->wdev, sizeof(double) * 6 * p.nof)) ||
         (rc = adj_alloc(h, &h->ck_final, state_bytes(h))))
+      return rc;
+  }
+  const bool fused = h->fused && h->adj_fused;
+  if (fused && !h->ae2) {
+    if ((rc = adj_alloc(h, &h->ae2, fb)) || (rc = adj_alloc(h, &h->ah2, fb)) || (rc = adj_alloc(h, &h->ad2, abytes)) ||
+        (rc = adj_alloc(h, &h->ab2, abytes)))
       return rc;
   }
   if (h->rec_cap < seg) {
@@ -944,25 +1178,62 @@ int adjoint_impl(iris_handle* h, int nobox, const double* weights, double* grad_
       hipLaunchKernelGGL(k_adj_g, dim3(blocks_for(B.g.npts[j])), dim3(256), 0, h->stream, B.g, j,
                          (const double2*)B.emg, p.nof, (const double*)h->wdev, sflux, h->adj_g);
   HIP_TRY(h, hipGetLastError());
-  // zero adjoint state and gradient; keep the state at nt
+  if (fused && (rc = build_grad_lists<T>(h, nobox, face_on))) return rc;
+  if (fused && h->gl_on && !h->ae3) {
+    if ((rc = adj_alloc(h, &h->ae3, fb)) || (rc = adj_alloc(h, &h->ah3, fb))) return rc;
+  }
+  // zero adjoint state (ghosts included) and gradient; keep the state at nt
   for (void* f : {h->ae, h->ah, h->atmp}) HIP_TRY(h, hipMemsetAsync(f, 0, fb, h->stream));
   for (void* f : {h->ad, h->ab}) HIP_TRY(h, hipMemsetAsync(f, 0, abytes, h->stream));
+  if (fused) {
+    for (void* f : {h->ae2, h->ah2}) HIP_TRY(h, hipMemsetAsync(f, 0, fb, h->stream));
+    for (void* f : {h->ad2, h->ab2}) HIP_TRY(h, hipMemsetAsync(f, 0, abytes, h->stream));
+    if (h->ae3)
+      for (void* f : {h->ae3, h->ah3}) HIP_TRY(h, hipMemsetAsync(f, 0, fb, h->stream));
+  }
   const size_t gbytes = sizeof(double2) * (size_t)p.nx * p.ny * (p.hf - p.hi + 1);
   HIP_TRY(h, hipMemsetAsync(h->grad, 0, gbytes, h->stream));
   if ((rc = state_copy(h, h->ck_final, true))) return rc;
   for (int k = nseg - 1; k >= 0; --k) {
     const int s0 = k * seg, s1 = std::min(nt, s0 + seg);
-    // forward again from the snapshot, recording H of steps s0..s1-1
+    // forward again from the snapshot, recording H of steps s0..s1-1 (two
+    // steps per pass on the gradient lists, whose single-step chunks hold the
+    // record planes: H of the first step is then in the second buffer)
     if ((rc = state_copy(h, h->ck[k], false))) return rc;
-    if ((rc = record_h(h, 0))) return rc;
-    for (int n = s0 + 1; n < s1; ++n) {
-      if ((rc = launch_step<T>(h, n))) return rc;
-      if ((rc = record_h(h, n - s0))) return rc;
+    if ((rc = record_h(h, h->fh, 0))) return rc;
+    int n = s0 + 1;
+    if (fused && h->gl_on) {
+      PassLists L;
+      L.chunks2 = h->gl_chunks2;
+      L.n2 = h->gl_n2;
+      L.rest = h->gl_rest;
+      L.nlean = h->gl_nlean;
+      L.ngen = h->gl_ngen;
+      for (; n + 1 < s1; n += 2) {
+        if ((rc = launch_double<T>(h, n, L, false))) return rc;
+        if ((rc = record_h(h, h->gh, n - s0)) || (rc = record_h(h, h->fh, n + 1 - s0))) return rc;
+      }
     }
-    for (int n = s1; n > s0; --n)
-      if ((rc = adjoint_step<T>(h, n, nobox, face_on, n - 1 - s0))) return rc;
+    for (; n < s1; ++n) {
+      if ((rc = launch_step<T>(h, n))) return rc;
+      if ((rc = record_h(h, h->fh, n - s0))) return rc;
+    }
+    for (int m = s1; m > s0;) {
+      if (!fused) {
+        rc = adjoint_step<T>(h, m, nobox, face_on, m - 1 - s0);
+        m -= 1;
+      } else if (h->gl_on && m - s0 >= 2) {
+        rc = adjoint_double<T>(h, m, nobox, face_on, m - 1 - s0);
+        m -= 2;
+      } else {
+        rc = adjoint_single<T>(h, m, nobox, face_on, m - 1 - s0);
+        m -= 1;
+      }
+      if (rc) return rc;
+    }
   }
   if ((rc = state_copy(h, h->ck_final, false))) return rc;
+  h->it = nt;
   HIP_TRY(h, hipMemcpyAsync(grad_out, h->grad, gbytes, hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(h, hipStreamSynchronize(h->stream));
   return 0;
@@ -1018,6 +1289,11 @@ int iris_create(const iris_params* p, const double* eps_inv, const double* fpzmn
     h->fused = !(env && env[0] == '0');
     const char* xenv = std::getenv("IRIS_XCD_ROWS");
     if (xenv && *xenv) h->xcd_rows = std::max(0, std::atoi(xenv));
+    const char* aenv = std::getenv("IRIS_ADJ_FUSED");
+    h->adj_fused = !(aenv && aenv[0] == '0');
+    const char* tyenv = std::getenv("IRIS_ADJ_TY");
+    if (tyenv && (std::atoi(tyenv) == 7 || std::atoi(tyenv) == 8 || std::atoi(tyenv) == 15))
+      h->adj_ty = std::atoi(tyenv);
     const char* x2env = std::getenv("IRIS_XCD_ROWS2");
     if (x2env && *x2env) h->xcd_rows2 = std::max(0, std::atoi(x2env));
   }

@@ -197,6 +197,24 @@ def test_adjoint_segment_and_linearity(tmp_path, precision):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("case", ["slab_ipol2", "bloch_lossy_ipol1", "two_step"])
+def test_fused_adjoint_matches_part_kernels(tmp_path, monkeypatch, case):
+    """The fused adjoint sweep (default) against the one-kernel-per-part
+    adjoint (IRIS_ADJ_FUSED=0), which forms the wraps' transposes as ghost
+    deposits instead of ghost images: equal up to rounding order."""
+    print(f"[gpu-test] fused adjoint vs part kernels {case}", flush=True)
+    s = make_setup(tmp_path, case)
+    w = weights_for(s.params)
+    for prec, tol in ((8, 1e-12), (4, 1e-5)):
+        g_f, _ = engine_gradient(s, w, prec, 9)
+        monkeypatch.setenv("IRIS_ADJ_FUSED", "0")
+        g_p, _ = engine_gradient(s, w, prec, 9)
+        monkeypatch.delenv("IRIS_ADJ_FUSED")
+        assert np.abs(g_p).max() > 0
+        assert rel(g_f, g_p) < tol, (prec, rel(g_f, g_p))
+
+
+@pytest.mark.gpu
 def test_adjoint_fp32_close_to_fp64(tmp_path):
     print("[gpu-test] adjoint fp32 vs fp64", flush=True)
     s = make_setup(tmp_path, "grid16")
@@ -224,6 +242,39 @@ def test_adjoint_with_two_step_forward(tmp_path, monkeypatch):
         monkeypatch.delenv("IRIS_TWO_STEP")
         assert np.array_equal(g1, g2)
         assert np.abs(g1).max() > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [8, 4])
+def test_two_step_adjoint_sweep(tmp_path, monkeypatch, precision):
+    """With only the z-faces weighted (a transmission objective) the
+    gradient sweeps advance the plain planes two steps per pass, forward and
+    adjoint.  The result is bitwise that of single steps (snapshot interval 1,
+    or no two-step pass at all) and matches the one-kernel-per-part adjoint."""
+    print(f"[gpu-test] two-step adjoint sweep precision={precision}", flush=True)
+    s = make_setup(tmp_path, "two_step")
+    p = s.params
+    w = np.zeros((p.nof, 6))
+    w[:, 4:] = weights_for(p)[:, 4:]
+    g1, _ = engine_gradient(s, w, precision, 1)
+    with Engine(s, precision=precision) as eng:
+        eng.checkpoint(17)
+        eng.run_to(p.nt)
+        eng.profile(True)
+        g17 = eng.adjoint(1, w)
+        prof = eng.profile_read(reset=True)
+    assert prof["yee2"][1] > 0, "the recomputation did not take the two-step pass"
+    gnt, _ = engine_gradient(s, w, precision, p.nt)
+    assert np.array_equal(g1, g17) and np.array_equal(g1, gnt)
+    monkeypatch.setenv("IRIS_TWO_STEP", "0")
+    g_single, _ = engine_gradient(s, w, precision, 17)
+    monkeypatch.setenv("IRIS_ADJ_FUSED", "0")
+    g_part, _ = engine_gradient(s, w, precision, 17)
+    monkeypatch.delenv("IRIS_TWO_STEP")
+    monkeypatch.delenv("IRIS_ADJ_FUSED")
+    assert np.array_equal(g1, g_single)
+    assert np.abs(g1).max() > 0
+    assert rel(g1, g_part) < (1e-12 if precision == 8 else 1e-5)
 
 
 @pytest.mark.gpu
