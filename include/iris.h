@@ -216,13 +216,22 @@ int iris_pass_info(const iris_handle* h, int64_t info[4]);
  * Its forward run is the plain one (iris_step), so the flux it returns is
  * the reference's (code/dataprocess.f90:241-277). ---- */
 
-/* Keep a copy of the whole field state (E, H and the UPML aux fields D, B)
- * every `segment` steps of the forward run (before steps 1, segment+1, ...),
- * for iris_adjoint's segment-wise recomputation.  Call before the first
- * step.  Allocates ceil(nt/segment) snapshots on the device up front
- * (IRIS_ENOMEM when they do not fit).  segment <= 0 frees them.  No
- * counterpart in the reference. */
-int iris_checkpoint(iris_handle* h, int32_t segment);
+/* Prepare the forward run for iris_adjoint.  The last R steps before nt
+ * record H on the structure planes hi-1..hf as they are taken (these steps
+ * are never recomputed); before them the whole field state (E, H and the
+ * UPML aux fields D, B) is copied every `segment` steps, and iris_adjoint
+ * recomputes each segment from its copy.  R is the number of records that
+ * fit in record_bytes (0: none; < 0: as many as fit in 90% of the free
+ * device memory beside the snapshots and the adjoint's buffers), at most nt.
+ * Call before the first step.  Allocates everything up front (IRIS_ENOMEM
+ * when it does not fit).  segment <= 0 frees it all.  No counterpart in the
+ * reference. */
+int iris_checkpoint(iris_handle* h, int32_t segment, int64_t record_bytes);
+
+/* Layout chosen by iris_checkpoint: info[0] = segment, info[1] = snapshots,
+ * info[2] = recorded steps R, info[3] = first recorded step nt - R,
+ * info[4] = bytes of one step's record, info[5] = bytes of one snapshot. */
+int iris_checkpoint_info(const iris_handle* h, int64_t info[6]);
 
 /* Gradient of J = sum_{iw, face} weights[iw*6 + face] * Jabs of box nobox
  * (the iris_flux numbers at step nt) with respect to the per-cell factor
@@ -230,10 +239,12 @@ int iris_checkpoint(iris_handle* h, int32_t segment);
  * code/fdtd.f90:115-117), on x in 1..nx, y in 1..ny, z in hi..hf.  grad
  * receives nx*ny*(hf-hi+1) complex values, x fastest then y then z:
  * (dJ/d Re eps_inv, dJ/d Im eps_inv).  The engine runs the discrete adjoint
- * of its own time step backwards from nt, recomputing each segment forward
- * from its snapshot, and restores the state at nt afterwards.  Needs the
- * handle at step nt with iris_checkpoint called before the first step
- * (IRIS_ESTATE otherwise).  Synchronous.  No counterpart in the reference. */
+ * of its own time step backwards from nt, over the recorded steps first and
+ * then segment by segment, recomputing each segment forward from its
+ * snapshot; it restores the state at nt afterwards and may be called again
+ * with other weights.  Needs the handle at step nt with iris_checkpoint
+ * called before the first step (IRIS_ESTATE otherwise).  Synchronous.  No
+ * counterpart in the reference. */
 int iris_adjoint(iris_handle* h, int32_t nobox, const double* weights, double* grad);
 
 /* Library version string. */

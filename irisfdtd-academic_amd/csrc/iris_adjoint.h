@@ -550,14 +550,16 @@ __global__ __launch_bounds__(kFTX * (TY + 1), 1) void k_adj_gen(
 // Objective source at the face points in the mixed representation: on a
 // plain plane the E part goes in as a E^ (X) and the H part as -dtcq2 H^ (Y),
 // and the stored min-side ghosts of Y follow their cells.
+// which = 0 / 1: the first / second face of pair j; 2: both (faces that
+// share no cell)
 template <typename T>
-__global__ void k_adj_source_mixed(StepArgs<T> a, BoxGeom g, int j, int second, const double2* __restrict__ G,
+__global__ void k_adj_source_mixed(StepArgs<T> a, BoxGeom g, int j, int which, const double2* __restrict__ G,
                                    int nof, const double2* __restrict__ ph, double dt) {
   const int64_t np = g.npts[j];
   const int64_t half = np / 2;
   const int64_t r0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r0 >= half) return;
-  const int64_t r = r0 + (second ? half : 0);
+  if (r0 >= (which == 2 ? np : half)) return;
+  const int64_t r = r0 + (which == 1 ? half : 0);
   int c, x, y, z;
   box_point(g, g.start[j] + r, &c, &x, &y, &z);
   const double2* Gp = G + g.start[j] * (int64_t)nof + r;
@@ -580,6 +582,16 @@ __global__ void k_adj_source_mixed(StepArgs<T> a, BoxGeom g, int j, int second, 
     if (x == a.nx) fld[idx - a.nx] = cadd(fld[idx - a.nx], cmul(a.fxm, d));
     if (y == a.ny) fld[idx - a.sx * a.ny] = cadd(fld[idx - a.sx * a.ny], cmul(a.fym, d));
   }
+}
+
+// H on the record planes of one step: three components, n16 16-byte words
+// each, component c at src + c*cs16 words; the record holds them back to back
+__global__ __launch_bounds__(256) void k_record(const int4* __restrict__ src, int64_t cs16, int64_t n16,
+                                                int4* __restrict__ dst) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 3 * n16) return;
+  const int64_t c = t / n16;
+  dst[t] = src[c * cs16 + (t - c * n16)];
 }
 
 }  // namespace iris

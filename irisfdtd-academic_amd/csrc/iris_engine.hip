@@ -53,6 +53,23 @@ struct PassLists {
   int n2 = 0;
   const int* rest = nullptr;
   int nlean = 0, ngen = 0;
+  const unsigned char* zmask = nullptr;  // 1 on the planes of two-step chunks
+};
+
+// device copy of the lists of one two-step classification
+struct ListSet {
+  bool on = false;
+  int key = -1;
+  int* chunks2 = nullptr;
+  int n2 = 0;
+  int* rest = nullptr;
+  int nlean = 0, ngen = 0;
+  unsigned char* zmask = nullptr;
+  PassLists view() const {
+    PassLists L;
+    L.chunks2 = chunks2; L.n2 = n2; L.rest = rest; L.nlean = nlean; L.ngen = ngen; L.zmask = zmask;
+    return L;
+  }
 };
 
 }  // namespace
@@ -140,15 +157,18 @@ struct iris_handle {
   void* ab2 = nullptr;
   void* ae3 = nullptr;      // third adjoint E and H (two-step adjoint passes)
   void* ah3 = nullptr;
-  // two-step lists of the gradient sweeps: the structure planes hi-1..hf,
-  // the TF/SF planes and the planes of weighted faces stay in single-step
-  // chunks (their H is recorded, or sources enter between the two steps)
-  bool gl_on = false;
-  int gl_key = -1;          // (nobox, face mask) the lists were made for
-  int* gl_chunks2 = nullptr;
-  int gl_n2 = 0;
-  int* gl_rest = nullptr;
-  int gl_nlean = 0, gl_ngen = 0;
+  // two-step lists of the gradient runs: the structure planes hi-1..hf and
+  // the TF/SF planes stay in single-step chunks (their H is recorded), and
+  // for the adjoint sweep also the planes of weighted faces (their sources
+  // enter between the two steps)
+  ListSet rl;               // forward steps that record H (face independent)
+  ListSet al;               // adjoint steps (for one box and face mask)
+  // H records of the last steps of the forward run: H of steps t0..nt-1,
+  // kept while stepping, so that those steps are never recomputed
+  int tail_t0 = 0;
+  int tail_n = 0;
+  void* tail_rec = nullptr;
+  std::vector<char> tail_ok;
   bool adj_fused = true;    // fused adjoint step (IRIS_ADJ_FUSED=0: one kernel per part)
   int adj_ty = 0;           // tile rows of the general adjoint pass (0: 7 with FP32 fields, 8 with FP64)
   void* hrec = nullptr;     // H on planes hi-1..hf for each step of a segment
@@ -388,7 +408,7 @@ int launch_double(iris_handle* h, int it, const PassLists& L, bool dft) {
       if (B.g.ntot == 0) continue;
       ProfScope ps(h, IRIS_K_DFT_SAMPLE);
       hipLaunchKernelGGL(k_dft_sample_masked<T>, dim3((unsigned)((B.g.ntot + 255) / 256)), dim3(256), 0, h->stream,
-                         ab, B.g, (C*)B.samp, h->nsamp, (const unsigned char*)h->zmask);
+                         ab, B.g, (C*)B.samp, h->nsamp, L.zmask);
     }
     HIP_TRY(h, hipGetLastError());
     h->nsamp++;
@@ -499,6 +519,7 @@ int run_steps(iris_handle* h, int it_first, int it_last) {
   L.rest = h->chunks_rest;
   L.nlean = h->nrest_lean;
   L.ngen = h->nrest_gen;
+  L.zmask = h->zmask;
   while (h->two_step && it_last - it + 1 >= 3) {
     int rc = launch_double<T>(h, it, L, true);
     if (rc) return rc;
@@ -839,34 +860,53 @@ int adj_alloc(iris_handle* h, void** p, size_t bytes) {
   return 0;
 }
 
-// iris_step with snapshots: the steps are cut at the snapshot boundaries so
-// that the state after step k*ck_seg is whole when it is copied
+template <typename T>
+int run_tail(iris_handle* h, int it_first, int it_last);
+
+// iris_step with snapshots: before tail_t0 the steps are cut at the snapshot
+// boundaries so that the state after step k*ck_seg is whole when it is
+// copied; from tail_t0 on H of every step is recorded
 int step_with_checkpoints(iris_handle* h, int it_first, int it_last) {
   int it = it_first;
   while (it <= it_last) {
-    const int k = (it - 1) / h->ck_seg;
-    if ((it - 1) % h->ck_seg == 0 && k < (int)h->ck.size() && !h->ck_ok[k]) {
-      int rc = state_copy(h, h->ck[k], true);
+    const int cur = it - 1;
+    int rc;
+    if (cur < h->tail_t0) {
+      const int k = cur / h->ck_seg;
+      if (cur % h->ck_seg == 0 && k < (int)h->ck.size() && !h->ck_ok[k]) {
+        if ((rc = state_copy(h, h->ck[k], true))) return rc;
+        h->ck_ok[k] = 1;
+      }
+      const int end = std::min(it_last, std::min((k + 1) * h->ck_seg, h->tail_t0));
+      rc = h->prec == 8 ? run_steps<double>(h, it, end) : run_steps<float>(h, it, end);
       if (rc) return rc;
-      h->ck_ok[k] = 1;
+      it = end + 1;
+    } else {
+      rc = h->prec == 8 ? run_tail<double>(h, it, it_last) : run_tail<float>(h, it, it_last);
+      if (rc) return rc;
+      it = it_last + 1;
     }
-    const int end = std::min(it_last, (k + 1) * h->ck_seg);
-    int rc = h->prec == 8 ? run_steps<double>(h, it, end) : run_steps<float>(h, it, end);
-    if (rc) return rc;
-    it = end + 1;
   }
   return 0;
 }
 
-// H on planes hi-1..hf of field buffer f into record slot r
-int record_h(iris_handle* h, const void* f, int r) {
+size_t rec_step_bytes(const iris_handle* h) { return h->csize * 3 * (size_t)h->rec_plane; }
+const void* rec_at(const iris_handle* h, const void* base, int r) {
+  return (const char*)base + rec_step_bytes(h) * (size_t)r;
+}
+
+// H on planes hi-1..hf of field buffer f into slot r of record buffer base
+int record_h(iris_handle* h, const void* f, void* base, int r) {
   ProfScope ps(h, IRIS_K_CHECKPOINT);
-  const size_t n = h->csize * (size_t)h->rec_plane;
-  for (int c = 0; c < 3; ++c) {
-    const char* src = (const char*)f + h->csize * ((size_t)h->cs * c + (size_t)h->sxy * (h->p.hi - 1));
-    char* dst = (char*)h->hrec + n * (3 * (size_t)r + c);
-    HIP_TRY(h, hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToDevice, h->stream));
-  }
+  // component strides and the plane range are multiples of 16 bytes (x pitch
+  // a multiple of 128 bytes, component stride of 256 elements)
+  const int64_t n16 = (int64_t)(h->csize * (size_t)h->rec_plane / 16);
+  const int64_t cs16 = (int64_t)(h->csize * (size_t)h->cs / 16);
+  const char* src = (const char*)f + h->csize * (size_t)h->sxy * (h->p.hi - 1);
+  char* dst = (char*)base + rec_step_bytes(h) * (size_t)r;
+  hipLaunchKernelGGL(k_record, dim3((unsigned)((3 * n16 + 255) / 256)), dim3(256), 0, h->stream, (const int4*)src,
+                     cs16, n16, (int4*)dst);
+  HIP_TRY(h, hipGetLastError());
   return 0;
 }
 
@@ -876,7 +916,7 @@ inline unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256); }
 // forward step in reverse order; the TF/SF terms do not depend on the state).
 // face_on bit 2j+s: face half s of pair j carries objective weight.
 template <typename T>
-int adjoint_step(iris_handle* h, int n, int nobox, int face_on, int rec_slot) {
+int adjoint_step(iris_handle* h, int n, int nobox, int face_on, const void* rec) {
   using C = cplx<T>;
   const iris_params& p = h->p;
   ProfScope ps(h, IRIS_K_ADJOINT);
@@ -921,7 +961,7 @@ int adjoint_step(iris_handle* h, int n, int nobox, int face_on, int rec_slot) {
   // 5. gradient on the structure cells with H of step n-1
   const int64_t nstruct = (int64_t)p.nx * p.ny * (p.hf - p.hi + 1);
   hipLaunchKernelGGL(k_adj_grad<T>, dim3(blocks_for(nstruct)), dim3(256), 0, h->stream, a, (const C*)ae,
-                     (const C*)h->hrec + 3 * h->rec_plane * (int64_t)rec_slot, h->rec_plane, h->grad);
+                     (const C*)rec, h->rec_plane, h->grad);
   // 6. E update: local part, then H^ += curl_f(.)
   hipLaunchKernelGGL(k_adj_e_local<T>, dim3(blocks_for(ncells)), dim3(256), 0, h->stream, a, ae, (C*)h->ad, tmp);
   hipLaunchKernelGGL(k_adj_curl_f<T>, dim3(blocks_for(ncells1)), dim3(256), 0, h->stream, a, (const C*)tmp, ah);
@@ -951,20 +991,30 @@ void adj_sources(iris_handle* h, const StepArgs<T>& a, int n, int nobox, int fac
   hipLaunchKernelGGL(k_dft_phase, dim3((p.nof + 255) / 256), dim3(256), 0, h->stream, h->fw, p.nof, 1, 1,
                      n + 1 - p.to_fourier, p.sample_fourier, 1.0, h->ph);
   const DeviceBox& B = h->box[nobox - 1];
-  for (int j = 0; j < 3; ++j)
-    for (int s = 0; s < 2; ++s) {
-      if (!((face_on >> (2 * j + s)) & 1)) continue;
-      hipLaunchKernelGGL(k_adj_source_mixed<T>, dim3(blocks_for(B.g.npts[j] / 2)), dim3(256), 0, h->stream, a, B.g,
-                         j, s, (const double2*)h->adj_g, p.nof, (const double2*)h->ph, p.dt);
+  const int lo[3] = {B.g.bxi, B.g.byi, B.g.bzi}, hi[3] = {B.g.bxf, B.g.byf, B.g.bzf};
+  for (int j = 0; j < 3; ++j) {
+    const int m = (face_on >> (2 * j)) & 3;
+    if (!m) continue;
+    if (m == 3 && lo[j] != hi[j]) {
+      // the two faces of the pair share no cell: one launch
+      hipLaunchKernelGGL(k_adj_source_mixed<T>, dim3(blocks_for(B.g.npts[j])), dim3(256), 0, h->stream, a, B.g, j, 2,
+                         (const double2*)h->adj_g, p.nof, (const double2*)h->ph, p.dt);
+      continue;
     }
+    for (int s = 0; s < 2; ++s)
+      if ((m >> s) & 1)
+        hipLaunchKernelGGL(k_adj_source_mixed<T>, dim3(blocks_for(B.g.npts[j] / 2)), dim3(256), 0, h->stream, a,
+                           B.g, j, s, (const double2*)h->adj_g, p.nof, (const double2*)h->ph, p.dt);
+  }
 }
 
 // one fused adjoint step over lists of lean chunks (the forward's lean pass
 // on the plain planes) and general chunks (k_adj_gen)
 template <typename T>
-void adj_lists(iris_handle* h, const StepArgs<T>& a, const int* lean, int nlean, const int* gen, int ngen, int slot) {
+void adj_lists(iris_handle* h, const StepArgs<T>& a, const int* lean, int nlean, const int* gen, int ngen,
+               const void* recp) {
   using C = cplx<T>;
-  const C* rec = (const C*)h->hrec + 3 * h->rec_plane * (int64_t)slot;
+  const C* rec = (const C*)recp;
   TileMap tm{(int)h->grd_fused.x, (int)h->grd_fused.y, 0, (int)h->grd_fused.x * h->xcd_rows};
   if (nlean) {
     tm.nblk = tm.tx * tm.ty * nlean;
@@ -993,11 +1043,11 @@ void adj_lists(iris_handle* h, const StepArgs<T>& a, const int* lean, int nlean,
 
 // adjoint of step n: A -> B, then A and B swap
 template <typename T>
-int adjoint_single(iris_handle* h, int n, int nobox, int face_on, int slot) {
+int adjoint_single(iris_handle* h, int n, int nobox, int face_on, const void* rec) {
   ProfScope ps(h, IRIS_K_ADJOINT);
   const StepArgs<T> a = adj_args<T>(h, h->ae, h->ah, h->ae2, h->ah2, h->ad, h->ab, h->ad2, h->ab2);
   adj_sources<T>(h, a, n, nobox, face_on);
-  adj_lists<T>(h, a, (const int*)h->chunks, h->nbulk, (const int*)h->chunks + h->nbulk, h->ngen, slot);
+  adj_lists<T>(h, a, (const int*)h->chunks, h->nbulk, (const int*)h->chunks + h->nbulk, h->ngen, rec);
   HIP_TRY(h, hipGetLastError());
   std::swap(h->ae, h->ae2);
   std::swap(h->ah, h->ah2);
@@ -1010,43 +1060,46 @@ int adjoint_single(iris_handle* h, int n, int nobox, int face_on, int slot) {
 // chunks (A -> C, exports into B), the other chunks by two single steps
 // (A -> B -> C), the sources of step n-1 into B between them
 template <typename T>
-int adjoint_double(iris_handle* h, int n, int nobox, int face_on, int slot) {
+int adjoint_double(iris_handle* h, int n, int nobox, int face_on, const void* rec_n, const void* rec_n1) {
   using C = cplx<T>;
   ProfScope ps(h, IRIS_K_ADJOINT);
-  const int* rl = h->gl_rest;
-  const int* rg = h->gl_rest + h->gl_nlean;
+  const ListSet& L = h->al;
+  const int* rl = L.rest;
+  const int* rg = L.rest + L.nlean;
   const StepArgs<T> a1 = adj_args<T>(h, h->ae, h->ah, h->ae2, h->ah2, h->ad, h->ab, h->ad2, h->ab2);
   adj_sources<T>(h, a1, n, nobox, face_on);
-  adj_lists<T>(h, a1, rl, h->gl_nlean, rg, h->gl_ngen, slot);
+  adj_lists<T>(h, a1, rl, L.nlean, rg, L.ngen, rec_n);
   {
     const StepArgs<T> a2 = adj_args<T>(h, h->ae, h->ah, h->ae3, h->ah3, h->ad, h->ab, h->ad2, h->ab2);
-    const TileMap tm{h->tx2, h->ty2, h->tx2 * h->ty2 * h->gl_n2, h->tx2 * h->xcd_rows2};
+    const TileMap tm{h->tx2, h->ty2, h->tx2 * h->ty2 * L.n2, h->tx2 * h->xcd_rows2};
     Yee2Samples<T> none{};
     hipLaunchKernelGGL(k_yee2<T>, dim3((unsigned)tm.nblk), dim3(kT2X, Yee2Shape<T>::R), 0, h->stream, a2,
-                       (C*)h->ae2, (C*)h->ah2, (const int*)h->gl_chunks2, tm, none);
+                       (C*)h->ae2, (C*)h->ah2, (const int*)L.chunks2, tm, none);
   }
   const StepArgs<T> a3 = adj_args<T>(h, h->ae2, h->ah2, h->ae3, h->ah3, h->ad2, h->ab2, h->ad, h->ab);
   adj_sources<T>(h, a3, n - 1, nobox, face_on);
-  adj_lists<T>(h, a3, rl, h->gl_nlean, rg, h->gl_ngen, slot - 1);
+  adj_lists<T>(h, a3, rl, L.nlean, rg, L.ngen, rec_n1);
   HIP_TRY(h, hipGetLastError());
   std::swap(h->ae, h->ae3);
   std::swap(h->ah, h->ah3);
   return 0;
 }
 
-// Two-step lists of the gradient sweeps for box nobox with weighted faces
-// face_on (bit 2j+s): two-step chunks need every plane z0-2 .. z1+2 between
-// the UPML slabs, off the structure and record planes hi-1..hf, off the TF/SF
-// planes and off the planes holding weighted face points (the z-faces, or the
-// box's whole z range when an x or y face is weighted).
+// Two-step lists of the gradient runs.  Two-step chunks need every plane
+// z0-2 .. z1+2 between the UPML slabs, off the structure and record planes
+// hi-1..hf and off the TF/SF planes; with face_on (bit 2j+s: face s of pair j
+// of box nobox carries weight) also off the planes holding weighted face
+// points (the z-faces, or the box's whole z range when an x or y face is
+// weighted).  The other chunks are lean (planes z0-1..z1+1 plain, z1 < nz)
+// or general.
 template <typename T>
-int build_grad_lists(iris_handle* h, int nobox, int face_on) {
+int build_lists(iris_handle* h, ListSet& out, int nobox, int face_on) {
   const iris_params& p = h->p;
   const int key = nobox * 64 + face_on;
-  if (h->gl_key == key) return 0;
-  h->gl_key = key;
-  h->gl_on = false;
-  h->gl_n2 = h->gl_nlean = h->gl_ngen = 0;
+  if (out.key == key) return 0;
+  out.key = key;
+  out.on = false;
+  out.n2 = out.nlean = out.ngen = 0;
   const BoxGeom& g = h->box[nobox - 1].g;
   const bool side = (face_on & 0xf) != 0;
   auto eligible = [&](int z) {
@@ -1081,6 +1134,7 @@ int build_grad_lists(iris_handle* h, int nobox, int face_on) {
   const char* z2env = std::getenv("IRIS_ZCHUNK2");
   if (z2env && std::atoi(z2env) > 0) zc2 = std::min(std::atoi(z2env), kChunkLenMask + 1);
   std::vector<int> c2, rlean, rgen;
+  std::vector<unsigned char> zm((size_t)p.nz + 2, 0);
   const int nch = (int)dbl.size();
   for (int k = 0; k < nch;) {
     const int z0 = 1 + k * zc;
@@ -1102,26 +1156,60 @@ int build_grad_lists(iris_handle* h, int nobox, int face_on) {
       if (c0 + len - 1 == r1) v |= kChunkExportAbove;
       c2.push_back(v);
     }
+    for (int z = r0; z <= r1; ++z) zm[z] = 1;
     k = k1 + 1;
   }
   std::vector<int> rest(rlean);
   rest.insert(rest.end(), rgen.begin(), rgen.end());
   int rc;
-  if (!h->gl_chunks2) {
+  if (!out.chunks2) {
     // sized for the most chunks any face mask can give
     const size_t cap = (size_t)nch + 1;
-    if ((rc = adj_alloc(h, (void**)&h->gl_chunks2, sizeof(int) * cap)) ||
-        (rc = adj_alloc(h, (void**)&h->gl_rest, sizeof(int) * cap)))
+    if ((rc = adj_alloc(h, (void**)&out.chunks2, sizeof(int) * cap)) ||
+        (rc = adj_alloc(h, (void**)&out.rest, sizeof(int) * cap)) || (rc = adj_alloc(h, (void**)&out.zmask, zm.size())))
       return rc;
   }
-  HIP_TRY(h, hipMemcpyAsync(h->gl_chunks2, c2.data(), sizeof(int) * c2.size(), hipMemcpyHostToDevice, h->stream));
+  HIP_TRY(h, hipMemcpyAsync(out.chunks2, c2.data(), sizeof(int) * c2.size(), hipMemcpyHostToDevice, h->stream));
   if (!rest.empty())
-    HIP_TRY(h, hipMemcpyAsync(h->gl_rest, rest.data(), sizeof(int) * rest.size(), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(h, hipMemcpyAsync(out.rest, rest.data(), sizeof(int) * rest.size(), hipMemcpyHostToDevice, h->stream));
+  HIP_TRY(h, hipMemcpyAsync(out.zmask, zm.data(), zm.size(), hipMemcpyHostToDevice, h->stream));
   HIP_TRY(h, hipStreamSynchronize(h->stream));
-  h->gl_n2 = (int)c2.size();
-  h->gl_nlean = (int)rlean.size();
-  h->gl_ngen = (int)rgen.size();
-  h->gl_on = true;
+  out.n2 = (int)c2.size();
+  out.nlean = (int)rlean.size();
+  out.ngen = (int)rgen.size();
+  out.on = true;
+  return 0;
+}
+
+// Steps it_first..it_last of the recorded tail (all after step tail_t0): H of
+// every step goes to its record slot; two steps per pass on the record
+// lists (their single-step chunks hold the record planes, so H of the first
+// step is whole there in the second buffer), the DFT as in run_steps.
+template <typename T>
+int run_tail(iris_handle* h, int it_first, int it_last) {
+  const iris_params& p = h->p;
+  int it = it_first;
+  auto rec = [&](const void* f, int step) -> int {
+    const int r = step - h->tail_t0;
+    if (r < 0 || r >= h->tail_n || h->tail_ok[r]) return 0;
+    h->tail_ok[r] = 1;
+    return record_h(h, f, h->tail_rec, r);
+  };
+  int rc = rec(h->fh, it - 1);
+  if (rc) return rc;
+  const PassLists L = h->rl.view();
+  while (h->rl.on && it_last - it + 1 >= 3) {
+    if ((rc = launch_double<T>(h, it, L, true))) return rc;
+    if ((rc = rec(h->gh, it)) || (rc = rec(h->fh, it + 1))) return rc;
+    it += 2;
+  }
+  for (; it <= it_last; ++it) {
+    if ((rc = launch_step<T>(h, it))) return rc;
+    if (it >= p.to_fourier && (it % p.sample_fourier) == 0 && (rc = dft_sample<T>(h, it + 1 - p.to_fourier)))
+      return rc;
+    h->it = it;
+    if ((rc = rec(h->fh, it))) return rc;
+  }
   return 0;
 }
 
@@ -1129,31 +1217,28 @@ template <typename T>
 int adjoint_impl(iris_handle* h, int nobox, const double* weights, double* grad_out) {
   const iris_params& p = h->p;
   const int nt = p.nt, seg = h->ck_seg;
-  const int nseg = (int)h->ck.size();
+  const int nhead = (int)h->ck.size();
   const size_t fb = field_bytes(h), abytes = aux_bytes(h);
+  const bool fused = h->fused && h->adj_fused;
   int rc;
   if ((rc = flush(h))) return rc;
   // buffers
   if (!h->ae) {
-    h->rec_plane = h->sxy * (p.hf - p.hi + 2);
-    h->rec_cap = seg;
     const size_t gbytes = sizeof(double2) * (size_t)p.nx * p.ny * (p.hf - p.hi + 1);
     if ((rc = adj_alloc(h, &h->ae, fb)) || (rc = adj_alloc(h, &h->ah, fb)) || (rc = adj_alloc(h, &h->atmp, fb)) ||
         (rc = adj_alloc(h, &h->ad, abytes)) || (rc = adj_alloc(h, &h->ab, abytes)) ||
-        (rc = adj_alloc(h, &h->hrec, h->csize * 3 * (size_t)h->rec_plane * h->rec_cap)) ||
         (rc = adj_alloc(h, (void**)&h->grad, gbytes)) ||
         (rc = adj_alloc(h, (void**)&h->wdev, sizeof(double) * 6 * p.nof)) ||
         (rc = adj_alloc(h, &h->ck_final, state_bytes(h))))
       return rc;
   }
-  const bool fused = h->fused && h->adj_fused;
   if (fused && !h->ae2) {
     if ((rc = adj_alloc(h, &h->ae2, fb)) || (rc = adj_alloc(h, &h->ah2, fb)) || (rc = adj_alloc(h, &h->ad2, abytes)) ||
         (rc = adj_alloc(h, &h->ab2, abytes)))
       return rc;
   }
-  if (h->rec_cap < seg) {
-    if ((rc = adj_alloc(h, &h->hrec, h->csize * 3 * (size_t)h->rec_plane * seg))) return rc;
+  if (nhead > 0 && h->rec_cap < seg) {
+    if ((rc = adj_alloc(h, &h->hrec, rec_step_bytes(h) * seg))) return rc;
     h->rec_cap = seg;
   }
   const DeviceBox& B = h->box[nobox - 1];
@@ -1178,8 +1263,8 @@ int adjoint_impl(iris_handle* h, int nobox, const double* weights, double* grad_
       hipLaunchKernelGGL(k_adj_g, dim3(blocks_for(B.g.npts[j])), dim3(256), 0, h->stream, B.g, j,
                          (const double2*)B.emg, p.nof, (const double*)h->wdev, sflux, h->adj_g);
   HIP_TRY(h, hipGetLastError());
-  if (fused && (rc = build_grad_lists<T>(h, nobox, face_on))) return rc;
-  if (fused && h->gl_on && !h->ae3) {
+  if (fused && (rc = build_lists<T>(h, h->al, nobox, face_on))) return rc;
+  if (fused && h->al.on && !h->ae3) {
     if ((rc = adj_alloc(h, &h->ae3, fb)) || (rc = adj_alloc(h, &h->ah3, fb))) return rc;
   }
   // zero adjoint state (ghosts included) and gradient; keep the state at nt
@@ -1194,43 +1279,46 @@ int adjoint_impl(iris_handle* h, int nobox, const double* weights, double* grad_
   const size_t gbytes = sizeof(double2) * (size_t)p.nx * p.ny * (p.hf - p.hi + 1);
   HIP_TRY(h, hipMemsetAsync(h->grad, 0, gbytes, h->stream));
   if ((rc = state_copy(h, h->ck_final, true))) return rc;
-  for (int k = nseg - 1; k >= 0; --k) {
-    const int s0 = k * seg, s1 = std::min(nt, s0 + seg);
-    // forward again from the snapshot, recording H of steps s0..s1-1 (two
-    // steps per pass on the gradient lists, whose single-step chunks hold the
-    // record planes: H of the first step is then in the second buffer)
-    if ((rc = state_copy(h, h->ck[k], false))) return rc;
-    if ((rc = record_h(h, h->fh, 0))) return rc;
-    int n = s0 + 1;
-    if (fused && h->gl_on) {
-      PassLists L;
-      L.chunks2 = h->gl_chunks2;
-      L.n2 = h->gl_n2;
-      L.rest = h->gl_rest;
-      L.nlean = h->gl_nlean;
-      L.ngen = h->gl_ngen;
-      for (; n + 1 < s1; n += 2) {
-        if ((rc = launch_double<T>(h, n, L, false))) return rc;
-        if ((rc = record_h(h, h->gh, n - s0)) || (rc = record_h(h, h->fh, n + 1 - s0))) return rc;
-      }
-    }
-    for (; n < s1; ++n) {
-      if ((rc = launch_step<T>(h, n))) return rc;
-      if ((rc = record_h(h, h->fh, n - s0))) return rc;
-    }
-    for (int m = s1; m > s0;) {
+
+  // adjoint of steps lo+1..hi_, H of step m-1 in slot m-1-lo of `base`
+  auto sweep = [&](int lo, int hi_, const void* base) -> int {
+    for (int m = hi_; m > lo;) {
+      int r;
       if (!fused) {
-        rc = adjoint_step<T>(h, m, nobox, face_on, m - 1 - s0);
+        r = adjoint_step<T>(h, m, nobox, face_on, rec_at(h, base, m - 1 - lo));
         m -= 1;
-      } else if (h->gl_on && m - s0 >= 2) {
-        rc = adjoint_double<T>(h, m, nobox, face_on, m - 1 - s0);
+      } else if (h->al.on && m - lo >= 2) {
+        r = adjoint_double<T>(h, m, nobox, face_on, rec_at(h, base, m - 1 - lo), rec_at(h, base, m - 2 - lo));
         m -= 2;
       } else {
-        rc = adjoint_single<T>(h, m, nobox, face_on, m - 1 - s0);
+        r = adjoint_single<T>(h, m, nobox, face_on, rec_at(h, base, m - 1 - lo));
         m -= 1;
       }
-      if (rc) return rc;
+      if (r) return r;
     }
+    return 0;
+  };
+  // the recorded tail needs no recomputation
+  if (h->tail_n && (rc = sweep(h->tail_t0, nt, h->tail_rec))) return rc;
+  const PassLists RL = h->rl.view();
+  for (int k = nhead - 1; k >= 0; --k) {
+    const int s0 = k * seg, s1 = std::min(h->tail_t0, s0 + seg);
+    // forward again from the snapshot, recording H of steps s0..s1-1 (two
+    // steps per pass on the record lists: H of the first step is then whole
+    // on the record planes in the second buffer)
+    if ((rc = state_copy(h, h->ck[k], false))) return rc;
+    if ((rc = record_h(h, h->fh, h->hrec, 0))) return rc;
+    int n = s0 + 1;
+    if (fused && h->rl.on)
+      for (; n + 1 < s1; n += 2) {
+        if ((rc = launch_double<T>(h, n, RL, false))) return rc;
+        if ((rc = record_h(h, h->gh, h->hrec, n - s0)) || (rc = record_h(h, h->fh, h->hrec, n + 1 - s0))) return rc;
+      }
+    for (; n < s1; ++n) {
+      if ((rc = launch_step<T>(h, n))) return rc;
+      if ((rc = record_h(h, h->fh, h->hrec, n - s0))) return rc;
+    }
+    if ((rc = sweep(s0, s1, h->hrec))) return rc;
   }
   if ((rc = state_copy(h, h->ck_final, false))) return rc;
   h->it = nt;
@@ -1250,6 +1338,8 @@ void release(iris_handle* h) {
   for (void* p : h->ck) (void)hipFree(p);
   h->ck.clear();
   h->ck_ok.clear();
+  if (h->tail_rec) (void)hipFree(h->tail_rec);
+  h->tail_rec = nullptr;
   h->ck_final = nullptr;  // one of adj_allocs
   for (void* p : h->adj_allocs) (void)hipFree(p);
   h->adj_allocs.clear();
@@ -1325,30 +1415,84 @@ int iris_step(iris_handle* h, int32_t it_first, int32_t it_last) {
   return h->prec == 8 ? run_steps<double>(h, it_first, it_last) : run_steps<float>(h, it_first, it_last);
 }
 
-int iris_checkpoint(iris_handle* h, int32_t segment) {
+int iris_checkpoint(iris_handle* h, int32_t segment, int64_t record_bytes) {
   if (!h) return fail(nullptr, IRIS_EINVAL, "handle is NULL");
   if (h->it != 0) return fail(h, IRIS_ESTATE, "iris_checkpoint: call it before the first step (now at step %d)", h->it);
   (void)hipSetDevice(h->device);
   HIP_TRY(h, hipStreamSynchronize(h->stream));
-  for (void* p : h->ck) (void)hipFree(p);
+  for (void* q : h->ck) (void)hipFree(q);
   h->ck.clear();
   h->ck_ok.clear();
+  if (h->tail_rec) (void)hipFree(h->tail_rec);
+  h->tail_rec = nullptr;
+  h->tail_ok.clear();
   h->ck_seg = 0;
+  h->tail_t0 = 0;
+  h->tail_n = 0;
   if (segment <= 0) return 0;
-  const int n = (h->p.nt + segment - 1) / segment;
-  const size_t bytes = state_bytes(h);
-  for (int k = 0; k < std::max(n, 1); ++k) {
-    void* p = nullptr;
-    if (hipMalloc(&p, bytes) != hipSuccess) {
+  const iris_params& p = h->p;
+  const int nt = p.nt;
+  h->rec_plane = h->sxy * (p.hf - p.hi + 2);
+  const size_t rstep = rec_step_bytes(h), sbytes = state_bytes(h);
+  int64_t nrec = 0;
+  if (record_bytes > 0) {
+    nrec = std::min<int64_t>(nt, record_bytes / (int64_t)rstep);
+  } else if (record_bytes < 0) {
+    // as many recorded steps as fit in 90% of the free device memory beside
+    // the head snapshots and the adjoint's own buffers
+    size_t freeb = 0, total = 0;
+    HIP_TRY(h, hipMemGetInfo(&freeb, &total));
+    const size_t fb = field_bytes(h), ab = aux_bytes(h);
+    int64_t ntot = 0;
+    for (int b = 0; b < p.nbox; ++b) ntot = std::max<int64_t>(ntot, h->box[b].g.ntot);
+    const double reserve = 7.0 * fb + 4.0 * ab + (double)segment * rstep + sbytes + 16.0 * ntot * p.nof +
+                           16.0 * p.nx * p.ny * (p.hf - p.hi + 1) + (double)(1u << 30);
+    const double avail = 0.9 * (double)freeb - reserve;
+    nrec = std::min<int64_t>(nt, avail > 0 ? (int64_t)(avail / rstep) : 0);
+    while (nrec > 0 && (double)nrec * rstep + (double)((nt - nrec + segment - 1) / segment) * sbytes > avail)
+      nrec -= std::max<int64_t>(1, nrec / 64);
+    nrec = std::max<int64_t>(nrec, 0);
+  }
+  const int t0 = nt - (int)nrec;
+  const int nhead = (t0 + segment - 1) / segment;
+  if (nrec > 0 && hipMalloc(&h->tail_rec, rstep * (size_t)nrec) != hipSuccess) {
+    (void)hipGetLastError();
+    h->tail_rec = nullptr;
+    return fail(h, IRIS_ENOMEM, "iris_checkpoint: records of %lld steps (%zu bytes each) do not fit", (long long)nrec,
+                rstep);
+  }
+  for (int k = 0; k < nhead; ++k) {
+    void* q = nullptr;
+    if (hipMalloc(&q, sbytes) != hipSuccess) {
       (void)hipGetLastError();
-      for (void* q : h->ck) (void)hipFree(q);
+      for (void* r : h->ck) (void)hipFree(r);
       h->ck.clear();
-      return fail(h, IRIS_ENOMEM, "iris_checkpoint: %d snapshots of %zu bytes do not fit", n, bytes);
+      if (h->tail_rec) (void)hipFree(h->tail_rec);
+      h->tail_rec = nullptr;
+      return fail(h, IRIS_ENOMEM, "iris_checkpoint: %d snapshots of %zu bytes do not fit", nhead, sbytes);
     }
-    h->ck.push_back(p);
+    h->ck.push_back(q);
   }
   h->ck_ok.assign(h->ck.size(), 0);
+  h->tail_ok.assign((size_t)nrec, 0);
+  h->tail_t0 = t0;
+  h->tail_n = (int)nrec;
   h->ck_seg = segment;
+  if (h->two_step) {
+    const int rc = h->prec == 8 ? build_lists<double>(h, h->rl, 1, 0) : build_lists<float>(h, h->rl, 1, 0);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+int iris_checkpoint_info(const iris_handle* h, int64_t info[6]) {
+  if (!h || !info) return IRIS_EINVAL;
+  info[0] = h->ck_seg;
+  info[1] = (int64_t)h->ck.size();
+  info[2] = h->tail_n;
+  info[3] = h->tail_t0;
+  info[4] = (int64_t)(h->csize * 3 * (size_t)h->sxy * (h->p.hf - h->p.hi + 2));
+  info[5] = (int64_t)state_bytes(h);
   return 0;
 }
 
@@ -1359,6 +1503,8 @@ int iris_adjoint(iris_handle* h, int32_t nobox, const double* weights, double* g
   if (h->it != h->p.nt) return fail(h, IRIS_ESTATE, "iris_adjoint: the forward run is at step %d, not nt=%d", h->it, h->p.nt);
   for (size_t k = 0; k < h->ck_ok.size(); ++k)
     if (!h->ck_ok[k]) return fail(h, IRIS_ESTATE, "iris_adjoint: snapshot %zu was not taken", k);
+  for (size_t k = 0; k < h->tail_ok.size(); ++k)
+    if (!h->tail_ok[k]) return fail(h, IRIS_ESTATE, "iris_adjoint: H of step %zu was not recorded", h->tail_t0 + k);
   (void)hipSetDevice(h->device);
   return h->prec == 8 ? adjoint_impl<double>(h, nobox, weights, grad) : adjoint_impl<float>(h, nobox, weights, grad);
 }

@@ -10,7 +10,8 @@ from Jabs -- e.g. the transmission T = J6/(J5+J6) of the reference's
 post-processing -- is differentiated this way.
 
 The engine holds the forward snapshots (iris_checkpoint, default every
-ceil(sqrt(nt)) steps) on the device until the backward has run.
+ceil(sqrt(nt)) steps) and the H records of the run's last steps (by default
+as many as fit in device memory) until the backward has run.
 """
 import math
 
@@ -50,11 +51,11 @@ class FluxFunction(torch.autograd.Function):
     """Jabs (nof, 6) of box `nobox` as a function of the structure's eps_inv."""
 
     @staticmethod
-    def forward(ctx, w, setup, precision, nobox, segment):
+    def forward(ctx, w, setup, precision, nobox, segment, record_bytes):
         s = with_structure(setup, w.detach().cpu().numpy())
         eng = Engine(s, precision=precision)
         try:
-            eng.checkpoint(segment or default_segment(s.params.nt))
+            eng.checkpoint(segment or default_segment(s.params.nt), record_bytes)
             eng.run_to(s.params.nt)
             jabs = eng.flux(nobox)
         except Exception:
@@ -76,14 +77,16 @@ class FluxFunction(torch.autograd.Function):
             eng.close()
             ctx.eng = None
         out = torch.from_numpy(grad if ctx.complex_in else grad.real.copy())
-        return out.to(device=ctx.in_device, dtype=ctx.in_dtype), None, None, None, None
+        return out.to(device=ctx.in_device, dtype=ctx.in_dtype), None, None, None, None, None
 
 
-def flux(w, setup, precision=4, nobox=1, segment=None):
+def flux(w, setup, precision=4, nobox=1, segment=None, record_bytes=-1):
     """Differentiable Jabs (nof, 6): w is the structure's eps_inv as a tensor
     (hf-hi+1, ny, nx), real or complex; precision 4 (FP32 fields, FP64 DFT)
-    or 8; segment = steps between forward snapshots (default ceil(sqrt(nt)))."""
-    return FluxFunction.apply(w, setup, precision, nobox, segment)
+    or 8; segment = steps between forward snapshots (default ceil(sqrt(nt)));
+    record_bytes = device memory for H records of the run's last steps
+    (-1: as much as fits, 0: none)."""
+    return FluxFunction.apply(w, setup, precision, nobox, segment, record_bytes)
 
 
 def transmission(jabs):

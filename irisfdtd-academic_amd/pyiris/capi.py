@@ -95,8 +95,10 @@ def load_library(path=None):
     lib.iris_diagnostics.restype = ct.c_int
     lib.iris_pass_info.argtypes = [H, ct.c_void_p]
     lib.iris_pass_info.restype = ct.c_int
-    lib.iris_checkpoint.argtypes = [H, ct.c_int32]
+    lib.iris_checkpoint.argtypes = [H, ct.c_int32, ct.c_int64]
     lib.iris_checkpoint.restype = ct.c_int
+    lib.iris_checkpoint_info.argtypes = [H, ct.c_void_p]
+    lib.iris_checkpoint_info.restype = ct.c_int
     lib.iris_adjoint.argtypes = [H, ct.c_int32, ct.c_void_p, ct.c_void_p]
     lib.iris_adjoint.restype = ct.c_int
     lib.iris_current_step.argtypes = [H, ct.POINTER(ct.c_int32)]
@@ -118,7 +120,7 @@ def load_library(path=None):
 EXPORTED = ["iris_create", "iris_step", "iris_sync", "iris_flux", "iris_dft_points", "iris_get_dft",
             "iris_get_field", "iris_set_field", "iris_get_plane", "iris_diagnostics", "iris_current_step",
             "iris_destroy", "iris_last_error", "iris_version", "iris_profile", "iris_profile_read",
-            "iris_pass_info", "iris_checkpoint", "iris_adjoint"]
+            "iris_pass_info", "iris_checkpoint", "iris_checkpoint_info", "iris_adjoint"]
 
 KERNELS = ["epass", "hpass", "tfsf", "dft_sample", "dft_accum", "flux", "yee", "yee2", "yee_rest", "adjoint",
            "checkpoint"]
@@ -220,10 +222,20 @@ class Engine:
         self._check(self.lib.iris_pass_info(self._h, _p(v)), "iris_pass_info")
         return dict(zchunk=int(v[0]), two_step_planes=int(v[1]), two_step_chunks=int(v[2]), two_step=bool(v[3]))
 
-    def checkpoint(self, segment):
-        """Keep a snapshot of the state every `segment` steps of the coming
-        forward run (call before the first step); segment <= 0 turns it off."""
-        self._check(self.lib.iris_checkpoint(self._h, int(segment)), "iris_checkpoint")
+    def checkpoint(self, segment, record_bytes=0):
+        """Prepare the coming forward run for adjoint(): H of its last steps
+        recorded in up to record_bytes of device memory (-1: as much as fits),
+        a snapshot of the state every `segment` steps before them.  Call
+        before the first step; segment <= 0 turns it off."""
+        self._check(self.lib.iris_checkpoint(self._h, int(segment), int(record_bytes)), "iris_checkpoint")
+
+    def checkpoint_info(self):
+        """{segment, snapshots, recorded_steps, first_recorded,
+        record_step_bytes, snapshot_bytes}."""
+        v = np.zeros(6, dtype=np.int64)
+        self._check(self.lib.iris_checkpoint_info(self._h, _p(v)), "iris_checkpoint_info")
+        return dict(segment=int(v[0]), snapshots=int(v[1]), recorded_steps=int(v[2]), first_recorded=int(v[3]),
+                    record_step_bytes=int(v[4]), snapshot_bytes=int(v[5]))
 
     def adjoint(self, nobox, weights):
         """dJ/d eps_inv on the structure cells for J = sum(weights * Jabs)

@@ -81,7 +81,7 @@ def torch_gradient(s, w):
 
 def engine_gradient(s, w, precision=8, segment=7):
     with Engine(s, precision=precision) as eng:
-        eng.checkpoint(segment)
+        eng.checkpoint(segment, 0)
         eng.run_to(s.params.nt)
         j = eng.flux(1)
         g = eng.adjoint(1, w)
@@ -275,6 +275,37 @@ def test_two_step_adjoint_sweep(tmp_path, monkeypatch, precision):
     assert np.array_equal(g1, g_single)
     assert np.abs(g1).max() > 0
     assert rel(g1, g_part) < (1e-12 if precision == 8 else 1e-5)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [8, 4])
+def test_recorded_tail(tmp_path, precision):
+    """H recorded during the forward run for its last steps (some, or all)
+    gives bitwise the gradient of pure segment-wise recomputation, with the
+    forward flux unchanged."""
+    print(f"[gpu-test] recorded tail precision={precision}", flush=True)
+    s = make_setup(tmp_path, "two_step")
+    p = s.params
+    w = np.zeros((p.nof, 6))
+    w[:, 4:] = weights_for(p)[:, 4:]
+    g0, j0 = engine_gradient(s, w, precision, 17)
+    for steps in (1, 58, 120, p.nt, -1):
+        with Engine(s, precision=precision) as eng:
+            if steps > 0:
+                eng.checkpoint(17, 0)
+                rstep = eng.checkpoint_info()["record_step_bytes"]
+                eng.checkpoint(17, steps * rstep)
+            else:
+                eng.checkpoint(17, -1)        # as many as fit: the whole run here
+            info = eng.checkpoint_info()
+            want = p.nt if steps < 0 else steps
+            assert info["recorded_steps"] == want
+            assert info["first_recorded"] == p.nt - want
+            assert info["snapshots"] == -(-(p.nt - want) // 17)
+            eng.run_to(p.nt)
+            assert np.array_equal(eng.flux(1), j0)
+            g = eng.adjoint(1, w)
+        assert np.array_equal(g, g0), steps
 
 
 @pytest.mark.gpu
