@@ -85,6 +85,27 @@ module iris_c_api
       type(c_ptr), value :: out
     end function
 
+    ! gradient of a flux objective (no counterpart in the reference)
+    integer(c_int) function iris_checkpoint(h, segment, record_bytes) bind(C, name="iris_checkpoint")
+      import :: c_int, c_ptr, c_int32_t, c_int64_t
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: segment
+      integer(c_int64_t), value :: record_bytes
+    end function
+
+    integer(c_int) function iris_checkpoint_info(h, info) bind(C, name="iris_checkpoint_info")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      type(c_ptr), value :: info
+    end function
+
+    integer(c_int) function iris_adjoint(h, nobox, weights, grad) bind(C, name="iris_adjoint")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: nobox
+      type(c_ptr), value :: weights, grad
+    end function
+
     integer(c_int) function iris_destroy(h) bind(C, name="iris_destroy")
       import :: c_int, c_ptr
       type(c_ptr), value :: h
