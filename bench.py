@@ -16,7 +16,11 @@ steps (inputs resident in HBM); the FP64-field rate, per-kernel HIP-event
 timings, the HBM roofline of the dominant kernel and a CPU baseline (the
 reference binary on a bounded sample) ride along.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-fp64] [--no-cpu]
+A second object, "adjoint", times the gradient workload (see ADJ_WORKLOAD):
+the plain forward run against the gradient of its summed transmission.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-fp64] [--no-cpu] [--no-adjoint]
+  python bench.py --adjoint-only [--adjoint-steps N]
 """
 import argparse
 import json
@@ -150,7 +154,7 @@ def run_gpu(precision, steps, warmup):
                 step_two_pass_frac=ab["step_two_pass"] / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS)
 
 
-def run_adjoint(precision=4):
+def run_adjoint(precision=4, nt=None):
     """Wall time of the plain forward run and of the gradient (forward run
     recording H of its last steps in the free device memory and keeping
     snapshots every ceil(sqrt(N)) steps before them, then the adjoint sweep
@@ -161,7 +165,8 @@ def run_adjoint(precision=4):
     import numpy as np
     from pyiris import Engine
     from pyiris.adjoint import structure_of, with_structure
-    s0 = make_setup(precision, ADJ_WORKLOAD)
+    over = {} if nt is None else dict(nt=nt, no_downloads=nt)
+    s0 = make_setup(precision, ADJ_WORKLOAD, **over)
     rng = np.random.default_rng(4)
     s = with_structure(s0, 1.0 / rng.uniform(1.0, 4.0, size=structure_of(s0).shape))
     p = s.params
@@ -274,12 +279,19 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="run the gradient workload only and print its JSON object (profiling)")
+    ap.add_argument("--adjoint-steps", type=int, default=None,
+                    help="steps of the gradient workload (default 12000; fewer for profiling)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this engine runs on one GPU; a run on several GPUs (one z-slab per GPU with "
               "plane exchanges) is not implemented", file=sys.stderr)
         sys.exit(2)
 
+    if a.adjoint_only:
+        print(json.dumps(run_adjoint(4, a.adjoint_steps)), flush=True)
+        return
     r32 = run_gpu(4, a.steps, a.warmup)
     d32 = r32["kernels"][r32["dominant"]]
     log(f"fp32: {r32['gcells']:.2f} Gcell/s, {r32['ms_per_step']:.3f} ms/step, {r32['dominant']} "
@@ -290,7 +302,7 @@ def main():
         d64 = r64["kernels"][r64["dominant"]]
         log(f"fp64: {r64['gcells']:.2f} Gcell/s, {r64['ms_per_step']:.3f} ms/step, {r64['dominant']} "
             f"{d64.get('gbs', 0):.0f} GB/s")
-    adj = None if a.no_adjoint else run_adjoint(4)
+    adj = None if a.no_adjoint else run_adjoint(4, a.adjoint_steps)
     cpu = None if a.no_cpu else cpu_baseline()
     if cpu:
         log(f"cpu baseline: {cpu['value'] * 1e3:.2f} Mcell/s")

@@ -18,7 +18,10 @@
 Kernel classes: "yee2" = the two-step pass over the plain z-chunks (one
 launch per two steps); "yee" = both instantiations of the single-step fused
 E+H pass (the lean one and the general one), "dft_fold",
-"dft_sample", "epass", "hpass", "tfsf", "flux", "copy" (the calibration).
+"dft_sample", "epass", "hpass", "tfsf", "flux", "adjoint" (the gradient's
+own kernels), "record" (its H records), "copy" (the calibration).
+--no-per-step leaves out the per-step table (for runs whose step count is
+not the general pass's launch count, such as a gradient).
 """
 import argparse
 import csv
@@ -42,6 +45,8 @@ CLASSES = [
     ("dft_phase", r"k_dft_phase"),
     ("dft_sample", r"k_dft_sample"),
     ("flux", r"k_flux"),
+    ("adjoint", r"k_adj_"),
+    ("record", r"k_record"),
     ("copy", r"copyBuffer"),
 ]
 
@@ -88,7 +93,7 @@ def stats(args):
             per_class[key] = per_class.get(key, 0.0) + tot
             if cls == "yee" and re.search(r"k_yee_fused<(float|double), true>", r["Name"]):
                 nsteps[prec] = calls
-    if nsteps:
+    if nsteps and not args.no_per_step:
         out.append("")
         out.append("Per step (steps = launches of the general fused-pass instantiation; warm-up included):")
         out.append("")
@@ -174,6 +179,7 @@ def main():
     a = sub.add_parser("stats")
     a.add_argument("csv")
     a.add_argument("--command", default="")
+    a.add_argument("--no-per-step", action="store_true")
     b = sub.add_parser("pmc")
     b.add_argument("fetch")
     b.add_argument("write")
