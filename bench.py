@@ -17,10 +17,13 @@ timings, the HBM roofline of the dominant kernel and a CPU baseline (the
 reference binary on a bounded sample) ride along.
 
 A second object, "adjoint", times the gradient workload (see ADJ_WORKLOAD):
-the plain forward run against the gradient of its summed transmission.
+the plain forward run against the gradient of its summed transmission.  A
+third, "batch", times the batched Bloch sweep (run_batch) against the same
+runs one after another.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-fp64] [--no-cpu] [--no-adjoint]
   python bench.py --adjoint-only [--adjoint-steps N]
+  python bench.py --batch-only [--batch-steps N]
 """
 import argparse
 import json
@@ -216,6 +219,62 @@ def run_adjoint(precision=4, nt=None):
     return out
 
 
+BATCH_N = 64
+BATCH_STEPS = 1000
+
+
+def run_batch(precision=4, n=BATCH_N, steps=BATCH_STEPS):
+    """The batched Bloch sweep: n random 2D-periodic gratings (nx, ny drawn
+    from 32..128, nz = 768, a 30-cell design layer with eps_r per column
+    U[1,4] smoothed by a 3-cell box filter, incidence angle U[0, 60] degrees
+    with the matching Bloch phase, seed 3, DFT at 50 wavelengths on a full
+    box), `steps` steps each.  Timed once as n runs one after another and
+    once as one batch; the fluxes of the two must be bitwise equal."""
+    import numpy as np
+    from pyiris import Batch, Engine
+    from pyiris.sweep import grating_sweep
+    setups, drawn = grating_sweep(n=n, seed=3, nz=768, nt=steps, nof=50, precision=precision)
+    cells = sum(s.params.nx * s.params.ny * s.params.nz for s in setups)
+    log(f"batch: {n} gratings, {cells / 1e6:.1f} M cells in all, {steps} steps")
+    with Engine(setups[0], precision=precision) as eng:   # warm-up: module load, first launches
+        eng.run_to(min(20, steps))
+        eng.sync()
+    t_seq, f_seq = 0.0, []
+    for s in setups:
+        with Engine(s, precision=precision) as eng:
+            eng.sync()
+            t0 = time.perf_counter()
+            eng.run_to(steps)
+            eng.sync()
+            t_seq += time.perf_counter() - t0
+            f_seq.append(eng.flux(1))
+    engines = [Engine(s, precision=precision) for s in setups]
+    try:
+        b = Batch(engines)
+        info = b.info()
+        for e in engines:
+            e.sync()
+        t0 = time.perf_counter()
+        b.step(1, steps)
+        b.sync()
+        t_bat = time.perf_counter() - t0
+        f_bat = [e.flux(1) for e in engines]
+        b.close()
+    finally:
+        for e in engines:
+            e.close()
+    same = all(np.array_equal(a, c) for a, c in zip(f_seq, f_bat))
+    log(f"batch: one after another {t_seq:.2f} s, batched {t_bat:.2f} s, speed-up {t_seq / t_bat:.2f}, "
+        f"bitwise equal {same}")
+    return {"workload": f"{n} random 2D-periodic gratings (nx, ny in 32..128, nz 768, 30-cell layer eps_r U[1,4] "
+                        f"box-filtered, angle U[0,60] deg with Bloch phase, seed 3), DFT at 50 wavelengths",
+            "instances": n, "steps": steps, "cells": cells, "fields": "fp32" if precision == 4 else "fp64",
+            "sequential_s": round(t_seq, 4), "batched_s": round(t_bat, 4), "speedup": round(t_seq / t_bat, 3),
+            "batched_gcells": round(cells * steps / t_bat / 1e9, 3),
+            "sequential_gcells": round(cells * steps / t_seq / 1e9, 3),
+            "batch_zchunk": info["zchunk"], "flux_bitwise_equal": bool(same)}
+
+
 def measured_traffic(kernel, precision):
     """HBM bytes per launch from the committed rocprofv3 PMC summary, if any."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_fp{32 if precision == 4 else 64}_pmc.json")
@@ -283,6 +342,10 @@ def main():
                     help="run the gradient workload only and print its JSON object (profiling)")
     ap.add_argument("--adjoint-steps", type=int, default=None,
                     help="steps of the gradient workload (default 12000; fewer for profiling)")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="run the batched sweep only and print its JSON object")
+    ap.add_argument("--batch-steps", type=int, default=BATCH_STEPS)
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this engine runs on one GPU; a run on several GPUs (one z-slab per GPU with "
@@ -291,6 +354,9 @@ def main():
 
     if a.adjoint_only:
         print(json.dumps(run_adjoint(4, a.adjoint_steps)), flush=True)
+        return
+    if a.batch_only:
+        print(json.dumps(run_batch(4, BATCH_N, a.batch_steps)), flush=True)
         return
     r32 = run_gpu(4, a.steps, a.warmup)
     d32 = r32["kernels"][r32["dominant"]]
@@ -303,6 +369,7 @@ def main():
         log(f"fp64: {r64['gcells']:.2f} Gcell/s, {r64['ms_per_step']:.3f} ms/step, {r64['dominant']} "
             f"{d64.get('gbs', 0):.0f} GB/s")
     adj = None if a.no_adjoint else run_adjoint(4, a.adjoint_steps)
+    bat = None if a.no_batch else run_batch(4, BATCH_N, a.batch_steps)
     cpu = None if a.no_cpu else cpu_baseline()
     if cpu:
         log(f"cpu baseline: {cpu['value'] * 1e3:.2f} Mcell/s")
@@ -350,6 +417,8 @@ def main():
                         "kernels_us": {n: round(v["avg_us"], 2) for n, v in r64["kernels"].items()}}
     if adj:
         line["adjoint"] = adj
+    if bat:
+        line["batch"] = bat
     if cpu:
         line["speedup_vs_cpu"] = round(r32["gcells"] / cpu["value"], 1)
     print(json.dumps(line), flush=True)

@@ -247,6 +247,36 @@ int iris_checkpoint_info(const iris_handle* h, int64_t info[6]);
  * counterpart in the reference. */
 int iris_adjoint(iris_handle* h, int32_t nobox, const double* weights, double* grad);
 
+/* ---- Batches: many independent runs advanced together.  No counterpart
+ * in the reference, which runs one structure at a time (its <<LOOP>> flow,
+ * code/program.f90:34-57, runs them one after another).  A sweep over
+ * incidence angles, Bloch phases or structures puts one handle per run in a
+ * batch; each step is then one launch of the fused pass over the tiles of
+ * every run (plus one launch for the DFT samples), with z-chunks as long as
+ * the whole batch allows.  Results are bitwise those of stepping each handle
+ * alone. ---- */
+typedef struct iris_batch iris_batch;
+
+/* Group n handles (created with iris_create, same precision and device, the
+ * same to_fourier and sample_fourier; no gradient snapshots).  The handles
+ * stay usable on their own; they must outlive the batch. */
+int iris_batch_create(iris_handle* const* handles, int32_t n, iris_batch** out);
+
+/* Steps it_first..it_last of every handle of the batch (all at step
+ * it_first-1, it_last within each nt).  Queued on the batch's stream; later
+ * calls on a handle wait for it. */
+int iris_batch_step(iris_batch* b, int32_t it_first, int32_t it_last);
+
+/* info[0] = handles, info[1] = planes per z-chunk of the batched pass,
+ * info[2] / info[3] = workgroups of its lean / general launch. */
+int iris_batch_info(const iris_batch* b, int64_t info[4]);
+
+int iris_batch_destroy(iris_batch* b);
+
+/* Message for the last failure of the batch (or of the last failed
+ * iris_batch_create when b is NULL).  Never NULL. */
+const char* iris_batch_last_error(const iris_batch* b);
+
 /* Library version string. */
 const char* iris_version(void);
 

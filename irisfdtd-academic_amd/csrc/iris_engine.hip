@@ -181,6 +181,27 @@ struct iris_handle {
   std::vector<void*> adj_allocs;
 };
 
+// A batch of handles stepped together by the batched fused pass.
+struct iris_batch {
+  std::vector<iris_handle*> hs;
+  int prec = 4;
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  std::string err = "no error";
+  int zc = 32;                      // planes per z-chunk of the batched passes
+  std::vector<void*> allocs;        // device memory of the batch
+  std::vector<int*> lean, gen;      // per instance: chunk lists
+  std::vector<int> nlean, ngen;
+  int* wg[2] = {nullptr, nullptr};  // per class (lean, general): workgroup prefix over the instances
+  int nwg[2] = {0, 0};
+  void* inst = nullptr;             // BatchInst<T>[n]
+  void* ent = nullptr;              // BatchSamp<T>[nent]
+  int64_t* ent_start = nullptr;
+  int nent = 0;
+  int64_t npts = 0;
+  std::vector<char> host_inst, host_ent;
+};
+
 namespace {
 
 int fail(iris_handle* h, int code, const char* fmt, ...) {
@@ -566,6 +587,26 @@ int check_params(const iris_params* p) {
   return 0;
 }
 
+// Single-step z-chunks of zc planes: lean when every plane z0-1 .. z1+1 lies
+// strictly between the UPML slabs, outside the structure and away from the
+// TF/SF planes, and the chunk is not the top one; general otherwise.
+void single_lists(const iris_params& p, int zc, std::vector<int>& lean, std::vector<int>& gen) {
+  auto is_bulk_plane = [&](int z) {
+    if (z <= p.pmlzi || z >= p.pmlzf) return false;
+    if (z >= p.hi && z <= p.hf) return false;
+    if (p.em_source && (z == p.iz_min - 1 || z == p.iz_min)) return false;
+    return true;
+  };
+  lean.clear();
+  gen.clear();
+  for (int z0 = 1; z0 <= p.nz; z0 += zc) {
+    const int z1 = std::min(z0 + zc - 1, p.nz);
+    bool ok = z1 < p.nz;
+    for (int z = z0 - 1; ok && z <= z1 + 1; ++z) ok = is_bulk_plane(z);
+    (ok ? lean : gen).push_back(z0);
+  }
+}
+
 template <typename T>
 void fill_pml(const iris_params& p, const double* fpmn, const double* fmmn, const double* fpmx, const double* fmmx,
               std::vector<PmlPlane<T>>& tab) {
@@ -728,18 +769,13 @@ int create_impl(iris_handle* h, const double* eps_inv, const double* fpmn, const
     // structure and away from the TF/SF planes, and it is not the top chunk
     std::vector<int> bulk, gen;
     const int zc = h->zchunk;
+    single_lists(p, zc, bulk, gen);
     auto is_bulk_plane = [&](int z) {
       if (z <= p.pmlzi || z >= p.pmlzf) return false;
       if (z >= p.hi && z <= p.hf) return false;
       if (p.em_source && (z == p.iz_min - 1 || z == p.iz_min)) return false;
       return true;
     };
-    for (int z0 = 1; z0 <= p.nz; z0 += zc) {
-      const int z1 = std::min(z0 + zc - 1, p.nz);
-      bool ok = z1 < p.nz;
-      for (int z = z0 - 1; ok && z <= z1 + 1; ++z) ok = is_bulk_plane(z);
-      (ok ? bulk : gen).push_back(z0);
-    }
     h->nbulk = (int)bulk.size();
     h->ngen = (int)gen.size();
     std::vector<int> all(bulk);
@@ -1327,6 +1363,203 @@ int adjoint_impl(iris_handle* h, int nobox, const double* weights, double* grad_
   return 0;
 }
 
+// ---- batches ------------------------------------------------------------------
+
+int bfail(iris_batch* b, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (b) b->err = buf; else g_create_error = buf;
+  return code;
+}
+
+#define BHIP_TRY(b, call)                                                                      \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess) return bfail((b), IRIS_EDEVICE, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+int balloc(iris_batch* b, void** p, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  if (hipMalloc(p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    return bfail(b, IRIS_ENOMEM, "hipMalloc(%zu bytes) for the batch failed", bytes);
+  }
+  b->allocs.push_back(*p);
+  return 0;
+}
+
+// argument tables of the current buffers of every instance (parity 0:
+// current -> other; 1: the reverse), uploaded before a run of steps
+template <typename T>
+int batch_tables(iris_batch* b) {
+  const int n = (int)b->hs.size();
+  b->host_inst.assign(sizeof(BatchInst<T>) * n, 0);
+  BatchInst<T>* I = (BatchInst<T>*)b->host_inst.data();
+  std::vector<BatchSamp<T>> E;
+  for (int i = 0; i < n; ++i) {
+    iris_handle* h = b->hs[i];
+    StepArgs<T> a0 = step_args_on<T>(h, h->fe, h->fh, h->ge, h->gh, h->d, h->b, h->dn, h->bn);
+    StepArgs<T> a1 = step_args_on<T>(h, h->ge, h->gh, h->fe, h->fh, h->dn, h->bn, h->d, h->b);
+    a0.zchunk = a1.zchunk = b->zc;
+    I[i].a[0] = a0;
+    I[i].a[1] = a1;
+    I[i].tp = (const TfsfConst*)h->tfc;
+    I[i].chunks[0] = b->lean[i];
+    I[i].chunks[1] = b->gen[i];
+    const int tx = (int)h->grd_fused.x, ty = (int)h->grd_fused.y;
+    I[i].tm[0] = TileMap{tx, ty, tx * ty * b->nlean[i], tx * h->xcd_rows};
+    I[i].tm[1] = TileMap{tx, ty, tx * ty * b->ngen[i], tx * h->xcd_rows};
+    for (int bx = 0; bx < h->p.nbox; ++bx) {
+      if (h->box[bx].g.ntot == 0) continue;
+      BatchSamp<T> e{};
+      e.a[0] = a0;
+      e.a[1] = a1;
+      e.g = h->box[bx].g;
+      e.samp = (cplx<T>*)h->box[bx].samp;
+      E.push_back(e);
+    }
+  }
+  b->host_ent.assign((const char*)E.data(), (const char*)(E.data() + E.size()));
+  BHIP_TRY(b, hipMemcpy(b->inst, b->host_inst.data(), b->host_inst.size(), hipMemcpyHostToDevice));
+  if (!E.empty()) BHIP_TRY(b, hipMemcpy(b->ent, b->host_ent.data(), b->host_ent.size(), hipMemcpyHostToDevice));
+  return 0;
+}
+
+template <typename T>
+int batch_create_impl(iris_batch* b) {
+  const int n = (int)b->hs.size();
+  int rc;
+  // z-chunks as long as the whole batch still gives every CU many workgroups
+  int64_t tiles = 0;
+  int nzmax = 0;
+  for (iris_handle* h : b->hs) {
+    tiles += (int64_t)h->grd_fused.x * h->grd_fused.y;
+    nzmax = std::max(nzmax, h->p.nz);
+  }
+  int zc = 32;
+  while (zc > 1 && tiles * ((nzmax + zc - 1) / zc) < 16 * 256) zc /= 2;
+  const char* zenv = std::getenv("IRIS_BATCH_ZCHUNK");
+  if (zenv && std::atoi(zenv) > 0) zc = std::atoi(zenv);
+  b->zc = zc;
+  std::vector<int> wg[2] = {{0}, {0}};
+  std::vector<int64_t> start = {0};
+  for (int i = 0; i < n; ++i) {
+    iris_handle* h = b->hs[i];
+    std::vector<int> lean, gen;
+    single_lists(h->p, std::min(zc, h->p.nz), lean, gen);
+    int* dl = nullptr;
+    int* dg = nullptr;
+    if ((rc = balloc(b, (void**)&dl, sizeof(int) * std::max<size_t>(lean.size(), 1))) ||
+        (rc = balloc(b, (void**)&dg, sizeof(int) * std::max<size_t>(gen.size(), 1))))
+      return rc;
+    if (!lean.empty()) BHIP_TRY(b, hipMemcpy(dl, lean.data(), sizeof(int) * lean.size(), hipMemcpyHostToDevice));
+    if (!gen.empty()) BHIP_TRY(b, hipMemcpy(dg, gen.data(), sizeof(int) * gen.size(), hipMemcpyHostToDevice));
+    b->lean.push_back(dl);
+    b->gen.push_back(dg);
+    b->nlean.push_back((int)lean.size());
+    b->ngen.push_back((int)gen.size());
+    const int t = (int)(h->grd_fused.x * h->grd_fused.y);
+    wg[0].push_back(wg[0].back() + t * (int)lean.size());
+    wg[1].push_back(wg[1].back() + t * (int)gen.size());
+    for (int bx = 0; bx < h->p.nbox; ++bx)
+      if (h->box[bx].g.ntot) start.push_back(start.back() + h->box[bx].g.ntot);
+  }
+  for (int k = 0; k < 2; ++k) {
+    if ((rc = balloc(b, (void**)&b->wg[k], sizeof(int) * wg[k].size()))) return rc;
+    BHIP_TRY(b, hipMemcpy(b->wg[k], wg[k].data(), sizeof(int) * wg[k].size(), hipMemcpyHostToDevice));
+    b->nwg[k] = wg[k].back();
+  }
+  b->nent = (int)start.size() - 1;
+  b->npts = start.back();
+  if ((rc = balloc(b, (void**)&b->ent_start, sizeof(int64_t) * start.size())) ||
+      (rc = balloc(b, &b->inst, sizeof(BatchInst<T>) * n)) ||
+      (rc = balloc(b, &b->ent, sizeof(BatchSamp<T>) * std::max(b->nent, 1))))
+    return rc;
+  BHIP_TRY(b, hipMemcpy(b->ent_start, start.data(), sizeof(int64_t) * start.size(), hipMemcpyHostToDevice));
+  return 0;
+}
+
+template <typename T>
+int batch_step_impl(iris_batch* b, int it_first, int it_last) {
+  const int n = (int)b->hs.size();
+  iris_handle* h0 = b->hs[0];
+  const iris_params& p0 = h0->p;
+  int rc;
+  // the instances' own work and the batch's previous run first (the argument
+  // tables are rewritten); the instances' launches then go to the batch stream
+  BHIP_TRY(b, hipStreamSynchronize(b->stream));
+  for (iris_handle* h : b->hs) BHIP_TRY(b, hipStreamSynchronize(h->stream));
+  if ((rc = batch_tables<T>(b))) return rc;
+  std::vector<hipStream_t> own(n);
+  for (int i = 0; i < n; ++i) {
+    own[i] = b->hs[i]->stream;
+    b->hs[i]->stream = b->stream;
+  }
+  int par = 0;
+  rc = 0;
+  for (int it = it_first; it <= it_last && !rc; ++it) {
+    for (int k = 0; k < 2; ++k) {
+      if (!b->nwg[k]) continue;
+      if (k == 0)
+        hipLaunchKernelGGL((k_yee_fused_batch<T, false>), dim3(b->nwg[0]), dim3(kFTX, kFTY + 1), 0, b->stream,
+                           (const BatchInst<T>*)b->inst, (const int*)b->wg[0], n, par, it);
+      else
+        hipLaunchKernelGGL((k_yee_fused_batch<T, true>), dim3(b->nwg[1]), dim3(kFTX, kFTY + 1), 0, b->stream,
+                           (const BatchInst<T>*)b->inst, (const int*)b->wg[1], n, par, it);
+    }
+    par ^= 1;
+    if (hipGetLastError() != hipSuccess) {
+      rc = bfail(b, IRIS_EDEVICE, "batched pass launch failed");
+      break;
+    }
+    if (it >= p0.to_fourier && (it % p0.sample_fourier) == 0 && b->nent) {
+      // the step just taken is in the buffers of parity `par`'s source side
+      hipLaunchKernelGGL(k_dft_sample_batch<T>, dim3((unsigned)((b->npts + 255) / 256)), dim3(256), 0, b->stream,
+                         (const BatchSamp<T>*)b->ent, (const int64_t*)b->ent_start, b->nent, par, h0->nsamp);
+      for (iris_handle* h : b->hs) {
+        if (h->nsamp == 0) h->samp_n0 = it + 1 - h->p.to_fourier;
+        h->nsamp++;
+        if (h->nsamp == dft_slots<T>() && (rc = dft_flush<T>(h))) {
+          b->err = h->err;
+          break;
+        }
+      }
+    }
+  }
+  const int steps = par;  // parity of the number of steps taken
+  for (int i = 0; i < n; ++i) {
+    iris_handle* h = b->hs[i];
+    if (steps) {
+      std::swap(h->fe, h->ge);
+      std::swap(h->fh, h->gh);
+      std::swap(h->d, h->dn);
+      std::swap(h->b, h->bn);
+    }
+    if (!rc) h->it = it_last;
+    h->stream = own[i];
+  }
+  if (rc) return rc;
+  // later work on each instance's own stream waits for the batch
+  BHIP_TRY(b, hipEventRecord(b->done, b->stream));
+  for (iris_handle* h : b->hs) BHIP_TRY(b, hipStreamWaitEvent(h->stream, b->done, 0));
+  return 0;
+}
+
+void batch_release(iris_batch* b) {
+  if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (void* q : b->allocs) (void)hipFree(q);
+  b->allocs.clear();
+  if (b->done) (void)hipEventDestroy(b->done);
+  b->done = nullptr;
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  b->stream = nullptr;
+}
+
 void release(iris_handle* h) {
   if (!h) return;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
@@ -1742,6 +1975,77 @@ int iris_profile_read(iris_handle* h, double* ms, int64_t* launches, int32_t res
 }
 
 const char* iris_last_error(const iris_handle* h) { return h ? h->err.c_str() : g_create_error.c_str(); }
+
+int iris_batch_create(iris_handle* const* handles, int32_t n, iris_batch** out) {
+  if (!out) return bfail(nullptr, IRIS_EINVAL, "out is NULL");
+  *out = nullptr;
+  if (!handles || n < 1) return bfail(nullptr, IRIS_EINVAL, "iris_batch_create: no handles");
+  for (int i = 0; i < n; ++i) {
+    const iris_handle* h = handles[i];
+    if (!h) return bfail(nullptr, IRIS_EINVAL, "iris_batch_create: handle %d is NULL", i);
+    if (!h->fused) return bfail(nullptr, IRIS_EUNSUP, "iris_batch_create: handle %d runs the in-place passes", i);
+    if (h->prec != handles[0]->prec || h->device != handles[0]->device)
+      return bfail(nullptr, IRIS_EINVAL, "iris_batch_create: handle %d differs in precision or device", i);
+    if (h->p.to_fourier != handles[0]->p.to_fourier || h->p.sample_fourier != handles[0]->p.sample_fourier)
+      return bfail(nullptr, IRIS_EINVAL, "iris_batch_create: handle %d differs in the DFT cadence", i);
+    if (h->ck_seg > 0) return bfail(nullptr, IRIS_EUNSUP, "iris_batch_create: handle %d keeps gradient snapshots", i);
+    for (int j = 0; j < i; ++j)
+      if (handles[j] == h) return bfail(nullptr, IRIS_EINVAL, "iris_batch_create: handle %d appears twice", i);
+  }
+  iris_batch* b = new iris_batch();
+  b->hs.assign(handles, handles + n);
+  b->prec = handles[0]->prec;
+  (void)hipSetDevice(handles[0]->device);
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess) {
+    (void)hipGetLastError();
+    batch_release(b);
+    delete b;
+    return bfail(nullptr, IRIS_EDEVICE, "iris_batch_create: stream or event");
+  }
+  const int rc = b->prec == 8 ? batch_create_impl<double>(b) : batch_create_impl<float>(b);
+  if (rc) {
+    g_create_error = b->err;
+    batch_release(b);
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return 0;
+}
+
+int iris_batch_step(iris_batch* b, int32_t it_first, int32_t it_last) {
+  if (!b) return bfail(nullptr, IRIS_EINVAL, "batch is NULL");
+  for (size_t i = 0; i < b->hs.size(); ++i) {
+    const iris_handle* h = b->hs[i];
+    if (h->it != it_first - 1)
+      return bfail(b, IRIS_ESTATE, "iris_batch_step: handle %zu is at step %d, not %d", i, h->it, it_first - 1);
+    if (it_last > h->p.nt) return bfail(b, IRIS_ESTATE, "iris_batch_step: it_last=%d is past nt=%d of handle %zu",
+                                        it_last, h->p.nt, i);
+    if (h->nsamp != b->hs[0]->nsamp) return bfail(b, IRIS_ESTATE, "iris_batch_step: DFT rings out of step");
+  }
+  if (it_last < it_first) return 0;
+  (void)hipSetDevice(b->hs[0]->device);
+  return b->prec == 8 ? batch_step_impl<double>(b, it_first, it_last) : batch_step_impl<float>(b, it_first, it_last);
+}
+
+int iris_batch_info(const iris_batch* b, int64_t info[4]) {
+  if (!b || !info) return IRIS_EINVAL;
+  info[0] = (int64_t)b->hs.size();
+  info[1] = b->zc;
+  info[2] = b->nwg[0];
+  info[3] = b->nwg[1];
+  return 0;
+}
+
+int iris_batch_destroy(iris_batch* b) {
+  if (!b) return 0;
+  batch_release(b);
+  delete b;
+  return 0;
+}
+
+const char* iris_batch_last_error(const iris_batch* b) { return b ? b->err.c_str() : g_create_error.c_str(); }
 
 const char* iris_version(void) { return "iris-mi355x 0.1 (gfx950)"; }
 

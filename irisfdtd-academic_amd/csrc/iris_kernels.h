@@ -57,6 +57,34 @@ __global__ __launch_bounds__(256) void k_dft_sample(StepArgs<T> a, BoxGeom g, cp
   samp[(int64_t)slot * g.ntot + i] = (c < 3) ? a.e[idx + c * a.cs] : a.h[idx + (c - 3) * a.cs];
 }
 
+// Batched sampling: entry e (one box of one instance of a batch) owns the
+// concatenated point indices start[e] .. start[e+1]-1 and reads the fields of
+// its instance's state a[par].
+template <typename T> struct BatchSamp {
+  StepArgs<T> a[2];
+  BoxGeom g;
+  cplx<T>* samp;
+};
+template <typename T>
+__global__ __launch_bounds__(256) void k_dft_sample_batch(const BatchSamp<T>* __restrict__ ent,
+                                                          const int64_t* __restrict__ start, int nent, int par,
+                                                          int slot) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= start[nent]) return;
+  int lo = 0, hi = nent;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (start[mid] <= i) lo = mid; else hi = mid;
+  }
+  const BatchSamp<T>& E = ent[lo];
+  const StepArgs<T>& a = E.a[par];
+  const int64_t r = i - start[lo];
+  int c, x, y, z;
+  box_point(E.g, r, &c, &x, &y, &z);
+  const int64_t idx = (int64_t)(x + a.xoff) + a.sx * y + a.sxy * z;
+  E.samp[(int64_t)slot * E.g.ntot + r] = (c < 3) ? a.e[idx + c * a.cs] : a.h[idx + (c - 3) * a.cs];
+}
+
 // phase table ph[k][iw] = scale*exp(i*fw(iw)*n_k), n_k = n0 + k*stride
 // (reference code/dataprocess.f90:145: cdexp(ci*w*it) = (cos, sin)(w*it));
 // scale is 1 for the exact fold and dt for the fused-multiply-add fold.

@@ -419,16 +419,17 @@ template <typename C> __device__ inline void phase3(C ph, C v[3]) {
 // z1 < nz); GEN = true handles every other chunk.  chunks[] lists the first
 // plane of each chunk this launch processes; the grid is one-dimensional,
 // tm.nblk workgroups, and tm gives each its tile.
+// The work of workgroup `blk` of a fused launch (tile and chunk from tm).
 template <typename T, bool GEN>
-__global__ __launch_bounds__(kFTX * (kFTY + 1), IRIS_FUSED_MINBLOCKS) void k_yee_fused(StepArgs<T> a, const TfsfConst* tp, int it,
-                                                                                      const int* chunks, TileMap tm) {
+__device__ __forceinline__ void yee_fused_tile(const StepArgs<T>& a, const TfsfConst* tp, int it, const int* chunks,
+                                               const TileMap& tm, int blk) {
   using C = cplx<T>;
   __shared__ C se[3][kFTY + 1][kFTX + 1];
   const int lane = threadIdx.x, wy = threadIdx.y;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const int64_t cs = a.cs;
   int tbx, tby, tbz;
-  tile_of(tm, blockIdx.x, &tbx, &tby, &tbz);
+  tile_of(tm, blk, &tbx, &tby, &tbz);
   const int x0 = tbx * kFTX + 1, y0 = tby * kFTY + 1;
   const int z0 = chunks[tbz];
   const int z1 = min(z0 + a.zchunk - 1, nz);
@@ -539,6 +540,44 @@ __global__ __launch_bounds__(kFTX * (kFTY + 1), IRIS_FUSED_MINBLOCKS) void k_yee
     __syncthreads();
     if (owna) { ep[0] = na[0]; ep[1] = na[1]; ep[2] = na[2]; }
   }
+}
+
+template <typename T, bool GEN>
+__global__ __launch_bounds__(kFTX * (kFTY + 1), IRIS_FUSED_MINBLOCKS) void k_yee_fused(StepArgs<T> a, const TfsfConst* tp, int it,
+                                                                                      const int* chunks, TileMap tm) {
+  yee_fused_tile<T, GEN>(a, tp, it, chunks, tm, blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Batched fused pass: one launch advances many independent grids (a sweep
+// over angles, Bloch phases or structures) by one step.  Instance i owns
+// workgroups wg_start[i] .. wg_start[i+1]-1; its arguments for the two
+// buffer parities sit in device memory.  Per workgroup the work is exactly
+// that of the single-grid pass.
+// ---------------------------------------------------------------------------
+template <typename T> struct BatchInst {
+  StepArgs<T> a[2];         // parity 0: current buffers -> other; parity 1: the reverse
+  const TfsfConst* tp;
+  const int* chunks[2];     // lean chunks, general chunks
+  TileMap tm[2];
+};
+
+__device__ inline int batch_instance(const int* __restrict__ wg_start, int ninst, int blk) {
+  int lo = 0, hi = ninst;  // wg_start[0] = 0 <= blk < wg_start[ninst]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (wg_start[mid] <= blk) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+template <typename T, bool GEN>
+__global__ __launch_bounds__(kFTX * (kFTY + 1), IRIS_FUSED_MINBLOCKS) void k_yee_fused_batch(
+    const BatchInst<T>* __restrict__ inst, const int* __restrict__ wg_start, int ninst, int par, int it) {
+  const int blk = blockIdx.x;
+  const int i = batch_instance(wg_start, ninst, blk);
+  const BatchInst<T>& I = inst[i];
+  yee_fused_tile<T, GEN>(I.a[par], I.tp, it, I.chunks[GEN ? 1 : 0], I.tm[GEN ? 1 : 0], blk - wg_start[i]);
 }
 
 }  // namespace iris

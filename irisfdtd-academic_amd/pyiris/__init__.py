@@ -5,6 +5,6 @@
   inputs  inputFDTD.dat writer for workloads
   runner  the reference program flow (output files) driven from Python
 """
-from .capi import Engine, IrisError, IrisParams, load_library  # noqa: F401
+from .capi import Batch, Engine, IrisError, IrisParams, load_library  # noqa: F401
 from .host import Setup, load_input  # noqa: F401
 from . import inputs  # noqa: F401

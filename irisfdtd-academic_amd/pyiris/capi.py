@@ -101,6 +101,16 @@ def load_library(path=None):
     lib.iris_checkpoint_info.restype = ct.c_int
     lib.iris_adjoint.argtypes = [H, ct.c_int32, ct.c_void_p, ct.c_void_p]
     lib.iris_adjoint.restype = ct.c_int
+    lib.iris_batch_create.argtypes = [ct.POINTER(H), ct.c_int32, ct.POINTER(H)]
+    lib.iris_batch_create.restype = ct.c_int
+    lib.iris_batch_step.argtypes = [H, ct.c_int32, ct.c_int32]
+    lib.iris_batch_step.restype = ct.c_int
+    lib.iris_batch_info.argtypes = [H, ct.c_void_p]
+    lib.iris_batch_info.restype = ct.c_int
+    lib.iris_batch_destroy.argtypes = [H]
+    lib.iris_batch_destroy.restype = ct.c_int
+    lib.iris_batch_last_error.argtypes = [H]
+    lib.iris_batch_last_error.restype = ct.c_char_p
     lib.iris_current_step.argtypes = [H, ct.POINTER(ct.c_int32)]
     lib.iris_current_step.restype = ct.c_int
     lib.iris_destroy.argtypes = [H]
@@ -120,7 +130,8 @@ def load_library(path=None):
 EXPORTED = ["iris_create", "iris_step", "iris_sync", "iris_flux", "iris_dft_points", "iris_get_dft",
             "iris_get_field", "iris_set_field", "iris_get_plane", "iris_diagnostics", "iris_current_step",
             "iris_destroy", "iris_last_error", "iris_version", "iris_profile", "iris_profile_read",
-            "iris_pass_info", "iris_checkpoint", "iris_checkpoint_info", "iris_adjoint"]
+            "iris_pass_info", "iris_checkpoint", "iris_checkpoint_info", "iris_adjoint", "iris_batch_create",
+            "iris_batch_step", "iris_batch_info", "iris_batch_destroy", "iris_batch_last_error"]
 
 KERNELS = ["epass", "hpass", "tfsf", "dft_sample", "dft_accum", "flux", "yee", "yee2", "yee_rest", "adjoint",
            "checkpoint"]
@@ -266,6 +277,55 @@ class Engine:
         if self._h:
             self.lib.iris_destroy(self._h)
             self._h = ct.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class Batch:
+    """Engines stepped together (iris_batch_*): one launch of the fused pass
+    per step over the tiles of every engine.  The engines stay usable on
+    their own and must outlive the batch."""
+
+    def __init__(self, engines):
+        self.lib = load_library()
+        self.engines = list(engines)
+        arr = (ct.c_void_p * len(self.engines))(*[e._h.value for e in self.engines])
+        self._b = ct.c_void_p()
+        rc = self.lib.iris_batch_create(ct.cast(arr, ct.POINTER(ct.c_void_p)), len(self.engines), ct.byref(self._b))
+        if rc != 0:
+            raise IrisError(f"iris_batch_create failed ({rc}): {self.lib.iris_batch_last_error(None).decode()}")
+
+    def step(self, it_first, it_last):
+        rc = self.lib.iris_batch_step(self._b, int(it_first), int(it_last))
+        if rc != 0:
+            raise IrisError(f"iris_batch_step failed ({rc}): {self.lib.iris_batch_last_error(self._b).decode()}")
+
+    def run_to(self, it_last):
+        self.step(self.engines[0].it + 1, it_last)
+
+    def info(self):
+        v = np.zeros(4, dtype=np.int64)
+        self.lib.iris_batch_info(self._b, _p(v))
+        return dict(instances=int(v[0]), zchunk=int(v[1]), lean_workgroups=int(v[2]), general_workgroups=int(v[3]))
+
+    def sync(self):
+        for e in self.engines:
+            e.sync()
+
+    def close(self):
+        if self._b:
+            self.lib.iris_batch_destroy(self._b)
+            self._b = ct.c_void_p()
 
     def __del__(self):
         try:
