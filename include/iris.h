@@ -253,8 +253,9 @@ int iris_adjoint(iris_handle* h, int32_t nobox, const double* weights, double* g
  * incidence angles, Bloch phases or structures puts one handle per run in a
  * batch; each step is then one launch of the fused pass over the tiles of
  * every run (plus one launch for the DFT samples), with z-chunks as long as
- * the whole batch allows.  Results are bitwise those of stepping each handle
- * alone. ---- */
+ * the whole batch allows.  Runs that the two-step pass covers keep their own
+ * two-step passes (on the batch's stream).  Results are bitwise those of
+ * stepping each handle alone. ---- */
 typedef struct iris_batch iris_batch;
 
 /* Group n handles (created with iris_create, same precision and device, the
@@ -268,8 +269,9 @@ int iris_batch_create(iris_handle* const* handles, int32_t n, iris_batch** out);
 int iris_batch_step(iris_batch* b, int32_t it_first, int32_t it_last);
 
 /* info[0] = handles, info[1] = planes per z-chunk of the batched pass,
- * info[2] / info[3] = workgroups of its lean / general launch. */
-int iris_batch_info(const iris_batch* b, int64_t info[4]);
+ * info[2] / info[3] = workgroups of its lean / general launch, info[4] =
+ * handles that take their own two-step passes. */
+int iris_batch_info(const iris_batch* b, int64_t info[5]);
 
 int iris_batch_destroy(iris_batch* b);
 

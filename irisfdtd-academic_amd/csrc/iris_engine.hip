@@ -109,6 +109,7 @@ struct iris_handle {
   // two steps per pass (k_yee2) over the chunks whose planes z0-1..z1+2 are
   // plain; the others take two single steps (lean ones, then general ones)
   bool two_step = false;
+  bool bloch = false;       // Bloch phase in x or y (the two-step pass's image lanes carry it)
   void* ke = nullptr;       // third E and H buffers (step n+2 of a two-step pass)
   void* kh = nullptr;
   int* chunks2 = nullptr;   // two-step chunks (z0 | export flags)
@@ -183,7 +184,9 @@ struct iris_handle {
 
 // A batch of handles stepped together by the batched fused pass.
 struct iris_batch {
-  std::vector<iris_handle*> hs;
+  std::vector<iris_handle*> hs;     // all handles of the batch
+  std::vector<iris_handle*> one;    // those the batched single-step pass advances
+  std::vector<iris_handle*> own;    // those that take their own two-step passes
   int prec = 4;
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
@@ -411,8 +414,12 @@ int launch_double(iris_handle* h, int it, const PassLists& L, bool dft) {
     ProfScope ps(h, IRIS_K_YEE2);
     const StepArgs<T> a2 = step_args_on<T>(h, h->fe, h->fh, h->ke, h->kh, h->d, h->b, h->dn, h->bn);
     const TileMap tm{h->tx2, h->ty2, h->tx2 * h->ty2 * L.n2, h->tx2 * h->xcd_rows2};
-    hipLaunchKernelGGL(k_yee2<T>, dim3((unsigned)tm.nblk), dim3(kT2X, Yee2Shape<T>::R), 0, h->stream, a2, (C*)h->ge,
-                       (C*)h->gh, L.chunks2, tm, smp);
+    if (h->bloch)
+      hipLaunchKernelGGL((k_yee2<T, true>), dim3((unsigned)tm.nblk), dim3(kT2X, Yee2Shape<T>::R), 0, h->stream, a2,
+                         (C*)h->ge, (C*)h->gh, L.chunks2, tm, smp);
+    else
+      hipLaunchKernelGGL((k_yee2<T, false>), dim3((unsigned)tm.nblk), dim3(kT2X, Yee2Shape<T>::R), 0, h->stream, a2,
+                         (C*)h->ge, (C*)h->gh, L.chunks2, tm, smp);
   }
   if (L.nlean + L.ngen) {
     ProfScope ps(h, IRIS_K_YEE_REST);
@@ -793,14 +800,27 @@ int create_impl(iris_handle* h, const double* eps_inv, const double* fpmn, const
       if (p.em_source && (z == p.iz_min - 1 || z == p.iz_min)) return false;
       return true;
     };
+    // IRIS_TWO_STEP=0: off, =1: on wherever it applies; unset: on when its
+    // tiles waste fewer lanes than the single-step pass's, as estimated by
+    // the lanes per cell-step of both (64 x R footprint lanes per 61 x (R-3)
+    // cells and two steps, against 64 x 9 lanes per 64 x 8 cells and one
+    // step).  Measured on MI355X with 768-plane grids, the two-step pass is
+    // slower only where this ratio is above 1 (one 64-wide row of tiles
+    // against two).
     const char* env = std::getenv("IRIS_TWO_STEP");
-    const bool want = !(env && env[0] == '0');
-    const bool no_phase = p.ak[0][0] == 0.0 && p.ak[0][1] == 0.0 && p.ak[1][0] == 0.0 && p.ak[1][1] == 0.0;
+    bool want = !(env && env[0] == '0');
+    if (!(env && env[0] == '1')) {
+      constexpr int R = Yee2Shape<T>::R, OY = Yee2Shape<T>::OY;
+      const double two = (double)((p.nx + kT2OX - 1) / kT2OX) * ((p.ny + OY - 1) / OY) * R / 2.0;
+      const double one = (double)((p.nx + kFTX - 1) / kFTX) * ((p.ny + kFTY - 1) / kFTY) * (kFTY + 1);
+      if (two > 0.95 * one) want = false;
+    }
+    h->bloch = !(p.ak[0][0] == 0.0 && p.ak[0][1] == 0.0 && p.ak[1][0] == 0.0 && p.ak[1][1] == 0.0);
     std::vector<char> dbl;
     int ndbl = 0;
     for (int z0 = 1; z0 <= p.nz; z0 += zc) {
       const int z1 = std::min(z0 + zc - 1, p.nz);
-      bool ok = h->fused && want && no_phase && p.nx >= kT2X && p.ny >= Yee2Shape<T>::R && z1 + 2 <= p.nz &&
+      bool ok = h->fused && want && p.nx >= kT2X && p.ny >= Yee2Shape<T>::R && z1 + 2 <= p.nz &&
                 z1 - z0 + 1 == zc;
       for (int z = z0 - 1; ok && z <= z1 + 2; ++z) ok = is_two_step_plane(z);
       dbl.push_back(ok ? 1 : 0);
@@ -1109,8 +1129,12 @@ int adjoint_double(iris_handle* h, int n, int nobox, int face_on, const void* re
     const StepArgs<T> a2 = adj_args<T>(h, h->ae, h->ah, h->ae3, h->ah3, h->ad, h->ab, h->ad2, h->ab2);
     const TileMap tm{h->tx2, h->ty2, h->tx2 * h->ty2 * L.n2, h->tx2 * h->xcd_rows2};
     Yee2Samples<T> none{};
-    hipLaunchKernelGGL(k_yee2<T>, dim3((unsigned)tm.nblk), dim3(kT2X, Yee2Shape<T>::R), 0, h->stream, a2,
-                       (C*)h->ae2, (C*)h->ah2, (const int*)L.chunks2, tm, none);
+    if (h->bloch)
+      hipLaunchKernelGGL((k_yee2<T, true>), dim3((unsigned)tm.nblk), dim3(kT2X, Yee2Shape<T>::R), 0, h->stream, a2,
+                         (C*)h->ae2, (C*)h->ah2, (const int*)L.chunks2, tm, none);
+    else
+      hipLaunchKernelGGL((k_yee2<T, false>), dim3((unsigned)tm.nblk), dim3(kT2X, Yee2Shape<T>::R), 0, h->stream, a2,
+                         (C*)h->ae2, (C*)h->ah2, (const int*)L.chunks2, tm, none);
   }
   const StepArgs<T> a3 = adj_args<T>(h, h->ae2, h->ah2, h->ae3, h->ah3, h->ad2, h->ab2, h->ad, h->ab);
   adj_sources<T>(h, a3, n - 1, nobox, face_on);
@@ -1153,8 +1177,7 @@ int build_lists(iris_handle* h, ListSet& out, int nobox, int face_on) {
     if (p.em_source && (z == p.iz_min - 1 || z == p.iz_min)) return false;
     return true;
   };
-  const bool no_phase = p.ak[0][0] == 0.0 && p.ak[0][1] == 0.0 && p.ak[1][0] == 0.0 && p.ak[1][1] == 0.0;
-  const bool shape = h->two_step && no_phase && p.nx >= kT2X && p.ny >= Yee2Shape<T>::R;
+  const bool shape = h->two_step && p.nx >= kT2X && p.ny >= Yee2Shape<T>::R;
   const int zc = h->zchunk;
   std::vector<char> dbl;
   int ndbl = 0;
@@ -1396,12 +1419,13 @@ int balloc(iris_batch* b, void** p, size_t bytes) {
 // current -> other; 1: the reverse), uploaded before a run of steps
 template <typename T>
 int batch_tables(iris_batch* b) {
-  const int n = (int)b->hs.size();
+  const int n = (int)b->one.size();
+  if (n == 0) return 0;
   b->host_inst.assign(sizeof(BatchInst<T>) * n, 0);
   BatchInst<T>* I = (BatchInst<T>*)b->host_inst.data();
   std::vector<BatchSamp<T>> E;
   for (int i = 0; i < n; ++i) {
-    iris_handle* h = b->hs[i];
+    iris_handle* h = b->one[i];
     StepArgs<T> a0 = step_args_on<T>(h, h->fe, h->fh, h->ge, h->gh, h->d, h->b, h->dn, h->bn);
     StepArgs<T> a1 = step_args_on<T>(h, h->ge, h->gh, h->fe, h->fh, h->dn, h->bn, h->d, h->b);
     a0.zchunk = a1.zchunk = b->zc;
@@ -1431,12 +1455,15 @@ int batch_tables(iris_batch* b) {
 
 template <typename T>
 int batch_create_impl(iris_batch* b) {
-  const int n = (int)b->hs.size();
+  // handles the two-step pass covers keep it (their own passes, as alone);
+  // the batched single-step pass takes the others
+  for (iris_handle* h : b->hs) (h->two_step ? b->own : b->one).push_back(h);
+  const int n = (int)b->one.size();
   int rc;
   // z-chunks as long as the whole batch still gives every CU many workgroups
   int64_t tiles = 0;
   int nzmax = 0;
-  for (iris_handle* h : b->hs) {
+  for (iris_handle* h : b->one) {
     tiles += (int64_t)h->grd_fused.x * h->grd_fused.y;
     nzmax = std::max(nzmax, h->p.nz);
   }
@@ -1448,7 +1475,7 @@ int batch_create_impl(iris_batch* b) {
   std::vector<int> wg[2] = {{0}, {0}};
   std::vector<int64_t> start = {0};
   for (int i = 0; i < n; ++i) {
-    iris_handle* h = b->hs[i];
+    iris_handle* h = b->one[i];
     std::vector<int> lean, gen;
     single_lists(h->p, std::min(zc, h->p.nz), lean, gen);
     int* dl = nullptr;
@@ -1476,7 +1503,7 @@ int batch_create_impl(iris_batch* b) {
   b->nent = (int)start.size() - 1;
   b->npts = start.back();
   if ((rc = balloc(b, (void**)&b->ent_start, sizeof(int64_t) * start.size())) ||
-      (rc = balloc(b, &b->inst, sizeof(BatchInst<T>) * n)) ||
+      (rc = balloc(b, &b->inst, sizeof(BatchInst<T>) * std::max(n, 1))) ||
       (rc = balloc(b, &b->ent, sizeof(BatchSamp<T>) * std::max(b->nent, 1))))
     return rc;
   BHIP_TRY(b, hipMemcpy(b->ent_start, start.data(), sizeof(int64_t) * start.size(), hipMemcpyHostToDevice));
@@ -1485,23 +1512,26 @@ int batch_create_impl(iris_batch* b) {
 
 template <typename T>
 int batch_step_impl(iris_batch* b, int it_first, int it_last) {
-  const int n = (int)b->hs.size();
-  iris_handle* h0 = b->hs[0];
-  const iris_params& p0 = h0->p;
+  const int n = (int)b->one.size();
+  const int nall = (int)b->hs.size();
   int rc;
   // the instances' own work and the batch's previous run first (the argument
   // tables are rewritten); the instances' launches then go to the batch stream
   BHIP_TRY(b, hipStreamSynchronize(b->stream));
   for (iris_handle* h : b->hs) BHIP_TRY(b, hipStreamSynchronize(h->stream));
   if ((rc = batch_tables<T>(b))) return rc;
-  std::vector<hipStream_t> own(n);
-  for (int i = 0; i < n; ++i) {
+  std::vector<hipStream_t> own(nall);
+  for (int i = 0; i < nall; ++i) {
     own[i] = b->hs[i]->stream;
     b->hs[i]->stream = b->stream;
   }
-  int par = 0;
   rc = 0;
-  for (int it = it_first; it <= it_last && !rc; ++it) {
+  // the two-step handles: their own passes, on the batch's stream
+  for (iris_handle* h : b->own)
+    if (!rc && (rc = run_steps<T>(h, it_first, it_last))) b->err = h->err;
+  iris_handle* h0 = n ? b->one[0] : nullptr;
+  int par = 0;
+  for (int it = it_first; n && it <= it_last && !rc; ++it) {
     for (int k = 0; k < 2; ++k) {
       if (!b->nwg[k]) continue;
       if (k == 0)
@@ -1516,11 +1546,11 @@ int batch_step_impl(iris_batch* b, int it_first, int it_last) {
       rc = bfail(b, IRIS_EDEVICE, "batched pass launch failed");
       break;
     }
-    if (it >= p0.to_fourier && (it % p0.sample_fourier) == 0 && b->nent) {
+    if (it >= h0->p.to_fourier && (it % h0->p.sample_fourier) == 0 && b->nent) {
       // the step just taken is in the buffers of parity `par`'s source side
       hipLaunchKernelGGL(k_dft_sample_batch<T>, dim3((unsigned)((b->npts + 255) / 256)), dim3(256), 0, b->stream,
                          (const BatchSamp<T>*)b->ent, (const int64_t*)b->ent_start, b->nent, par, h0->nsamp);
-      for (iris_handle* h : b->hs) {
+      for (iris_handle* h : b->one) {
         if (h->nsamp == 0) h->samp_n0 = it + 1 - h->p.to_fourier;
         h->nsamp++;
         if (h->nsamp == dft_slots<T>() && (rc = dft_flush<T>(h))) {
@@ -1532,7 +1562,7 @@ int batch_step_impl(iris_batch* b, int it_first, int it_last) {
   }
   const int steps = par;  // parity of the number of steps taken
   for (int i = 0; i < n; ++i) {
-    iris_handle* h = b->hs[i];
+    iris_handle* h = b->one[i];
     if (steps) {
       std::swap(h->fe, h->ge);
       std::swap(h->fh, h->gh);
@@ -1540,8 +1570,8 @@ int batch_step_impl(iris_batch* b, int it_first, int it_last) {
       std::swap(h->b, h->bn);
     }
     if (!rc) h->it = it_last;
-    h->stream = own[i];
   }
+  for (int i = 0; i < nall; ++i) b->hs[i]->stream = own[i];
   if (rc) return rc;
   // later work on each instance's own stream waits for the batch
   BHIP_TRY(b, hipEventRecord(b->done, b->stream));
@@ -2022,19 +2052,21 @@ int iris_batch_step(iris_batch* b, int32_t it_first, int32_t it_last) {
       return bfail(b, IRIS_ESTATE, "iris_batch_step: handle %zu is at step %d, not %d", i, h->it, it_first - 1);
     if (it_last > h->p.nt) return bfail(b, IRIS_ESTATE, "iris_batch_step: it_last=%d is past nt=%d of handle %zu",
                                         it_last, h->p.nt, i);
-    if (h->nsamp != b->hs[0]->nsamp) return bfail(b, IRIS_ESTATE, "iris_batch_step: DFT rings out of step");
+    if (!h->two_step && h->nsamp != b->one[0]->nsamp)
+      return bfail(b, IRIS_ESTATE, "iris_batch_step: DFT rings out of step");
   }
   if (it_last < it_first) return 0;
   (void)hipSetDevice(b->hs[0]->device);
   return b->prec == 8 ? batch_step_impl<double>(b, it_first, it_last) : batch_step_impl<float>(b, it_first, it_last);
 }
 
-int iris_batch_info(const iris_batch* b, int64_t info[4]) {
+int iris_batch_info(const iris_batch* b, int64_t info[5]) {
   if (!b || !info) return IRIS_EINVAL;
   info[0] = (int64_t)b->hs.size();
   info[1] = b->zc;
   info[2] = b->nwg[0];
   info[3] = b->nwg[1];
+  info[4] = (int64_t)b->own.size();
   return 0;
 }
 

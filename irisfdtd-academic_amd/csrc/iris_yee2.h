@@ -2,12 +2,13 @@
 //
 // k_yee2 advances E and H from step n to n+2 in one pass over z-chunks whose
 // planes z0-1 .. z1+2 all lie between the UPML slabs and away from the TF/SF
-// planes (regions I and III and the structure), for grids without a Bloch
-// phase (exp(+-i ak) = 1: the x/y wraps are plain copies).  Per cell it
-// performs exactly the operations of two single steps (reference
-// code/fdtd.f90:57-248 with the periodic wraps of code/boundary.f90:163-348),
-// so its results are those of two launches of k_yee_fused; what it saves is
-// HBM traffic: step n is read and step n+2 written once.
+// planes (regions I and III and the structure).  Per cell it performs the
+// operations of two single steps (reference code/fdtd.f90:57-248 with the
+// periodic wraps of code/boundary.f90:163-348); without a Bloch phase
+// (exp(+-i ak) = 1, the x/y wraps are plain copies) its results are exactly
+// those of two launches of k_yee_fused, with one they agree to rounding (see
+// PH below).  What it saves is HBM traffic: step n is read and step n+2
+// written once.
 //
 // Work decomposition.  A workgroup is kT2X lanes (x) by kT2R waves (y) and
 // covers the footprint x0-1 .. x0+kT2X-2, y0-1 .. y0+kT2R-2 of an output tile
@@ -116,7 +117,15 @@ template <typename T> struct Yee2Samples {
   int nbox;
 };
 
-template <typename T>
+// PH = true: grids with a Bloch phase in x or y.  A footprint lane outside
+// 1..n then works in the frame of the cell it stands for: every value it
+// loads is multiplied by that cell's Bloch factor (exp(-i ak) below 1,
+// exp(+i ak) above n, the product at corners), so what it computes is the
+// periodic image with its phase and its LDS neighbours need no correction.
+// The results equal those of single steps up to rounding in the cells next
+// to the period boundary (the single-step pass applies the factor after
+// forming an image instead of before).
+template <typename T, bool PH>
 __global__ __launch_bounds__(kT2X * Yee2Shape<T>::R, Yee2Shape<T>::WAVES_PER_EU) void k_yee2(StepArgs<T> a, cplx<T>* __restrict__ eb,
                                                       cplx<T>* __restrict__ hb, const int* __restrict__ chunks,
                                                       TileMap tm, Yee2Samples<T> smp) {
@@ -148,14 +157,26 @@ __global__ __launch_bounds__(kT2X * Yee2Shape<T>::R, Yee2Shape<T>::WAVES_PER_EU)
   const int wp = min(wy + 1, kT2R - 1), wm = max(wy - 1, 0);
   const C* __restrict__ e0 = a.e;
   const C* __restrict__ h0 = a.h;
+  // Bloch factor of the cell this lane stands for (1 inside); every lane
+  // multiplies, so the loads carry no per-lane branch (a factor of exactly 1
+  // leaves a value unchanged)
+  C phi = mk<T>((T)1, (T)0);
+  if constexpr (PH) {
+    if (xg < 1) phi = a.fxm; else if (xg > nx) phi = a.fxp;
+    if (yg < 1) phi = cmul(phi, a.fym); else if (yg > ny) phi = cmul(phi, a.fyp);
+  }
+  auto ld = [&](const C* __restrict__ f, int64_t i) -> C {
+    if constexpr (PH) return cmul(phi, f[i]);
+    return f[i];
+  };
 
   // pipeline registers of this lane: H0(z-1), E1(z-1), H1(z-2), E2(z-2)
   C hz1[3], e1p[3] = {}, h1pp[3] = {}, e2p[3] = {};
   {
     const int64_t i = col + sxy * (z0 - 2);
-    hz1[0] = h0[i];
-    hz1[1] = h0[i + cs];
-    hz1[2] = h0[i + 2 * cs];
+    hz1[0] = ld(h0, i);
+    hz1[1] = ld(h0, i + cs);
+    hz1[2] = ld(h0, i + 2 * cs);
   }
 
   for (int z = z0 - 1; z <= z1 + 2; ++z) {
@@ -163,15 +184,15 @@ __global__ __launch_bounds__(kT2X * Yee2Shape<T>::R, Yee2Shape<T>::WAVES_PER_EU)
     // ---- E1(z) from H0(z), H0(z-1), E0(z) (code/fdtd.f90:78-96, :137-147)
     C h0z[3], e1[3];
     {
-      h0z[0] = h0[iz];
-      h0z[1] = h0[iz + cs];
-      h0z[2] = h0[iz + 2 * cs];
-      const C h3ym = h0[colym + sxy * z + 2 * cs], h1ym = h0[colym + sxy * z];
-      const C h3xm = h0[colxm + sxy * z + 2 * cs], h2xm = h0[colxm + sxy * z + cs];
+      h0z[0] = ld(h0, iz);
+      h0z[1] = ld(h0, iz + cs);
+      h0z[2] = ld(h0, iz + 2 * cs);
+      const C h3ym = ld(h0, colym + sxy * z + 2 * cs), h1ym = ld(h0, colym + sxy * z);
+      const C h3xm = ld(h0, colxm + sxy * z + 2 * cs), h2xm = ld(h0, colxm + sxy * z + cs);
       const C c[3] = {cadd(csub(csub(h0z[2], h3ym), h0z[1]), hz1[1]),
                       cadd(csub(csub(h0z[0], hz1[0]), h0z[2]), h3xm),
                       cadd(csub(csub(h0z[1], h2xm), h0z[0]), h1ym)};
-      const C eo[3] = {e0[iz], e0[iz + cs], e0[iz + 2 * cs]};
+      const C eo[3] = {ld(e0, iz), ld(e0, iz + cs), ld(e0, iz + 2 * cs)};
       e_update_plain<T>(a, col, z, eo, c, e1);
     }
     // ---- H1(z-1) from H0(z-1), E1(z-1) (own and LDS +x/+y), E1(z)

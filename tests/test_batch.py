@@ -29,8 +29,8 @@ def mixed_setups(tmp_path):
                      box=(1, 8, 1, 6, 12, 50)), 25.0),
         make(tmp_path, "b", i_pol=1, ixmax=7, iymax=6, izmax=64, pmlzi=8, pmlzf=57, hi=26, dh=6, dx=2,
              eps_bbox=3.0 + 0.4j, eps3=1.8, box=(2, 6, 1, 5, 12, 48), mbox=(1, 7, 1, 6, 20, 40), nof=2),
-        make(tmp_path, "c", ixmax=64, iymax=16, izmax=96, pmlzi=10, pmlzf=86, hi=44, dh=10, dx=8, eps_bbox=2.25,
-             tfsf_o=20, box=(1, 64, 1, 16, 15, 80), nof=4),
+        make(tmp_path, "c", ixmax=70, iymax=21, izmax=96, pmlzi=10, pmlzf=86, hi=44, dh=10, dx=8, eps_bbox=2.25,
+             tfsf_o=20, box=(1, 70, 1, 21, 15, 80), nof=4),
         make(tmp_path, "d", ixmax=20, iymax=1, izmax=80, pmlzi=10, pmlzf=70, hi=36, dh=8, dx=5, eps_bbox=4.0,
              box=(1, 20, 1, 1, 14, 60)),
         oblique(make(tmp_path, "e", ixmax=33, iymax=9, izmax=70, pmlzi=9, pmlzf=62, hi=30, dh=7, dx=0, eps_bbox=1.7,
