@@ -2054,6 +2054,8 @@ int iris_batch_step(iris_batch* b, int32_t it_first, int32_t it_last) {
                                         it_last, h->p.nt, i);
     if (!h->two_step && h->nsamp != b->one[0]->nsamp)
       return bfail(b, IRIS_ESTATE, "iris_batch_step: DFT rings out of step");
+    if (h->ck_seg > 0)
+      return bfail(b, IRIS_EUNSUP, "iris_batch_step: handle %zu keeps gradient snapshots (step it alone)", i);
   }
   if (it_last < it_first) return 0;
   (void)hipSetDevice(b->hs[0]->device);
