@@ -78,7 +78,7 @@ def make_setup(precision, base=None, **over):
         return load_input(path, precision=precision)
 
 
-def algorithmic_bytes(p, precision, two_step_planes=0):
+def algorithmic_bytes(p, precision, two_step_planes=0, upml_two_step_planes=0, upml_in_two_step=0):
     """Compulsory HBM bytes per launch (c = bytes per complex value, r = per
     real):
       yee (fused E+H pass, one step): read E and H, write E and H (12
@@ -86,6 +86,9 @@ def algorithmic_bytes(p, precision, two_step_planes=0):
         structure cell eps_inv;
       yee2 (two-step pass over the plain z-chunks): read E and H of step n,
         write E and H of step n+2 (12 complex per cell of those planes);
+      yee2_pml (its variant over the chunks reaching into the UPML): as yee2,
+        plus read D and B of step n and write them for step n+2 (12 complex
+        per UPML cell);
       yee_rest (one step of the other z-chunks inside a two-step pass): as
         yee over the remaining planes;
       in-place E or H pass: read E and H, write one of them (9 complex per
@@ -98,9 +101,11 @@ def algorithmic_bytes(p, precision, two_step_planes=0):
     npml = p.pmlzi + (p.nz - p.pmlzf + 1)
     nxy = p.nx * p.ny
     nstr = nxy * (p.hf - p.hi + 1)
+    rest = p.nz - two_step_planes - upml_two_step_planes
     return {"yee": 12 * c * ncell + 12 * c * nxy * npml + r * nstr,
             "yee2": 12 * c * nxy * two_step_planes,
-            "yee_rest": 12 * c * nxy * (p.nz - two_step_planes) + 12 * c * nxy * npml + r * nstr,
+            "yee2_pml": 12 * c * nxy * (upml_two_step_planes + upml_in_two_step),
+            "yee_rest": 12 * c * nxy * rest + 12 * c * nxy * (npml - upml_in_two_step),
             "epass": 9 * c * ncell + 6 * c * nxy * npml + r * nstr,
             "hpass": 9 * c * ncell + 6 * c * nxy * npml,
             "step_two_pass": 18 * c * ncell + 12 * c * nxy * npml + r * nstr}
@@ -141,7 +146,8 @@ def run_gpu(precision, steps, warmup):
     eng.close()
     secs = t1 - t0
     gcells = p.nx * p.ny * p.nz * steps / secs / 1e9
-    ab = algorithmic_bytes(p, precision, info["two_step_planes"])
+    ab = algorithmic_bytes(p, precision, info["two_step_planes"], info["upml_two_step_planes"],
+                           info["upml_planes_in_two_step"])
     kern = {}
     for k, (ms, n) in prof.items():
         if n:
@@ -151,7 +157,7 @@ def run_gpu(precision, steps, warmup):
                 kern[k]["gbs"] = ab[k] / (1e-3 * ms / n) / 1e9
     dom = max(kern, key=lambda k: kern[k]["total_ms"])
     ms_step = 1e3 * secs / steps
-    yee_ms = sum(kern[k]["total_ms"] for k in ("yee", "yee2", "yee_rest") if k in kern) / steps
+    yee_ms = sum(kern[k]["total_ms"] for k in ("yee", "yee2", "yee2_pml", "yee_rest") if k in kern) / steps
     return dict(gcells=gcells, ms_per_step=ms_step, secs=secs, kernels=kern, dominant=dom, passes=info,
                 yee_ms_per_step=yee_ms,
                 step_two_pass_frac=ab["step_two_pass"] / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS)

@@ -192,7 +192,9 @@ const char* iris_last_error(const iris_handle* h);
                             /* the other z-chunks                           */
 #define IRIS_K_ADJOINT 9    /* one adjoint step (all its kernels)           */
 #define IRIS_K_CHECKPOINT 10 /* state snapshots, restores and H records     */
-#define IRIS_NKERNEL 11
+#define IRIS_K_YEE2_PML 11  /* two-step pass over chunks reaching into the   */
+                            /* UPML slabs                                    */
+#define IRIS_NKERNEL 12
 
 /* Enable (1) or disable (0) per-kernel timing with HIP events recorded
  * around every launch on the handle's stream.  No counterpart in the
@@ -205,11 +207,13 @@ int iris_profile(iris_handle* h, int32_t enable);
 int iris_profile_read(iris_handle* h, double* ms, int64_t* launches, int32_t reset);
 
 /* Layout of the step passes: info[0] = planes per z-chunk; info[1] = planes
- * advanced two steps per pass (0 when the two-step pass is off: Bloch phase
- * in x or y, a grid narrower than a tile, IRIS_TWO_STEP=0); info[2] = z-chunks
- * of the two-step pass; info[3] = 1 when it is on.  No counterpart in the
+ * advanced two steps per pass by the plain two-step kernel (0 when the
+ * two-step pass is off: a grid narrower than a tile, or one where it wastes
+ * more lanes than single steps, or IRIS_TWO_STEP=0); info[2] = its z-chunks;
+ * info[3] = 1 when the pass is on; info[4] = planes advanced by its UPML
+ * variant; info[5] = UPML planes among those.  No counterpart in the
  * reference; used to account the algorithmic bytes of the passes. */
-int iris_pass_info(const iris_handle* h, int64_t info[4]);
+int iris_pass_info(const iris_handle* h, int64_t info[6]);
 
 /* ---- Gradient of a face-flux objective.  The reference has no adjoint;
  * the differentiable form stands beside the plain solver on the same handle.

@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <utility>
 #include <string>
 #include <vector>
@@ -46,14 +47,25 @@ struct DeviceBox {
   void* samp = nullptr;
 };
 
+// single-step chunks of one length inside the rest list of a two-step pass
+struct RestGroup {
+  int len;          // planes per chunk (the launches' zchunk)
+  int off;          // first entry in the rest list
+  int nlean, ngen;  // lean entries, then general ones
+};
+
 // chunk lists of a two-step pass: two-step chunks (z0 | length | export
 // flags) and the chunks left to single steps, lean ones first
 struct PassLists {
   const int* chunks2 = nullptr;
   int n2 = 0;
+  const int* chunks2p = nullptr;  // two-step chunks reaching into the UPML slabs
+  int n2p = 0;
   const int* rest = nullptr;
   int nlean = 0, ngen = 0;
   const unsigned char* zmask = nullptr;  // 1 on the planes of two-step chunks
+  const RestGroup* groups = nullptr;      // rest chunks by length (null: one group of zchunk planes)
+  int ngroups = 0;
 };
 
 // device copy of the lists of one two-step classification
@@ -115,8 +127,15 @@ struct iris_handle {
   int* chunks2 = nullptr;   // two-step chunks (z0 | export flags)
   int n2 = 0;
   int64_t planes2 = 0;      // planes advanced by the two-step pass
-  int* chunks_rest = nullptr; // the other chunks: lean ones, then general ones
+  int* chunks2p = nullptr;  // two-step chunks reaching into the UPML slabs (k_yee2<..., PML>)
+  int n2p = 0;
+  int64_t planes2p = 0, upml2p = 0;  // their planes, and the UPML planes among them
+  int ty2p = 0;             // tile rows of the UPML variant
+  void* kd = nullptr;       // third D and B buffers (the UPML variant's output)
+  void* kb = nullptr;
+  int* chunks_rest = nullptr; // the other chunks: by length, lean ones then general ones
   int nrest_lean = 0, nrest_gen = 0;
+  std::vector<RestGroup> rest_groups;
   unsigned char* zmask = nullptr; // 1 on the planes of two-step chunks
   int tx2 = 0, ty2 = 0;
   int xcd_rows2 = 4;        // tile rows per XCD run of the two-step pass
@@ -403,28 +422,47 @@ int launch_double(iris_handle* h, int it, const PassLists& L, bool dft) {
     smp.g[b] = h->box[b].g;
     smp.samp[b] = (s1 && h->box[b].g.ntot) ? (C*)h->box[b].samp + (int64_t)h->nsamp * h->box[b].g.ntot : nullptr;
   }
-  const int* rl = L.rest;
-  const int* rg = L.rest + L.nlean;
+  const RestGroup one{h->zchunk, 0, L.nlean, L.ngen};
+  const RestGroup* groups = L.groups ? L.groups : &one;
+  const int ngroups = L.groups ? L.ngroups : 1;
+  auto rest = [&](StepArgs<T> a, int step) {
+    for (int g = 0; g < ngroups; ++g) {
+      const RestGroup& G = groups[g];
+      if (G.nlean + G.ngen == 0) continue;
+      a.zchunk = G.len;
+      launch_fused_lists<T>(h, a, step, L.rest + G.off, G.nlean, L.rest + G.off + G.nlean, G.ngen);
+    }
+  };
   if (L.nlean + L.ngen) {
     ProfScope ps(h, IRIS_K_YEE_REST);
-    const StepArgs<T> a1 = step_args_on<T>(h, h->fe, h->fh, h->ge, h->gh, h->d, h->b, h->dn, h->bn);
-    launch_fused_lists<T>(h, a1, it, rl, L.nlean, rg, L.ngen);
+    rest(step_args_on<T>(h, h->fe, h->fh, h->ge, h->gh, h->d, h->b, h->dn, h->bn), it);
   }
-  {
+  if (L.n2) {
     ProfScope ps(h, IRIS_K_YEE2);
     const StepArgs<T> a2 = step_args_on<T>(h, h->fe, h->fh, h->ke, h->kh, h->d, h->b, h->dn, h->bn);
     const TileMap tm{h->tx2, h->ty2, h->tx2 * h->ty2 * L.n2, h->tx2 * h->xcd_rows2};
     if (h->bloch)
       hipLaunchKernelGGL((k_yee2<T, true>), dim3((unsigned)tm.nblk), dim3(kT2X, Yee2Shape<T>::R), 0, h->stream, a2,
-                         (C*)h->ge, (C*)h->gh, L.chunks2, tm, smp);
+                         (C*)h->ge, (C*)h->gh, nullptr, L.chunks2, tm, smp);
     else
       hipLaunchKernelGGL((k_yee2<T, false>), dim3((unsigned)tm.nblk), dim3(kT2X, Yee2Shape<T>::R), 0, h->stream, a2,
-                         (C*)h->ge, (C*)h->gh, L.chunks2, tm, smp);
+                         (C*)h->ge, (C*)h->gh, nullptr, L.chunks2, tm, smp);
+  }
+  if (L.n2p) {
+    // the UPML variant: D and B of step n+2 into the third aux buffers
+    ProfScope ps(h, IRIS_K_YEE2_PML);
+    const StepArgs<T> a2 = step_args_on<T>(h, h->fe, h->fh, h->ke, h->kh, h->d, h->b, h->kd, h->kb);
+    const TileMap tm{h->tx2, h->ty2p, h->tx2 * h->ty2p * L.n2p, h->tx2 * h->xcd_rows2};
+    if (h->bloch)
+      hipLaunchKernelGGL((k_yee2<T, true, true>), dim3((unsigned)tm.nblk), dim3(kT2X, Yee2Shape<T, true>::R), 0,
+                         h->stream, a2, (C*)h->ge, (C*)h->gh, (C*)h->dn, L.chunks2p, tm, smp);
+    else
+      hipLaunchKernelGGL((k_yee2<T, false, true>), dim3((unsigned)tm.nblk), dim3(kT2X, Yee2Shape<T, true>::R), 0,
+                         h->stream, a2, (C*)h->ge, (C*)h->gh, (C*)h->dn, L.chunks2p, tm, smp);
   }
   if (L.nlean + L.ngen) {
     ProfScope ps(h, IRIS_K_YEE_REST);
-    const StepArgs<T> a3 = step_args_on<T>(h, h->ge, h->gh, h->ke, h->kh, h->dn, h->bn, h->d, h->b);
-    launch_fused_lists<T>(h, a3, it + 1, rl, L.nlean, rg, L.ngen);
+    rest(step_args_on<T>(h, h->ge, h->gh, h->ke, h->kh, h->dn, h->bn, h->kd, h->kb), it + 1);
   }
   HIP_TRY(h, hipGetLastError());
   if (s1) {
@@ -447,6 +485,8 @@ int launch_double(iris_handle* h, int it, const PassLists& L, bool dft) {
   }
   std::swap(h->fe, h->ke);
   std::swap(h->fh, h->kh);
+  std::swap(h->d, h->kd);
+  std::swap(h->b, h->kb);
   h->it = it;
   if (s2) {
     int rc = dft_sample<T>(h, it + 2 - p.to_fourier);
@@ -548,6 +588,10 @@ int run_steps(iris_handle* h, int it_first, int it_last) {
   L.nlean = h->nrest_lean;
   L.ngen = h->nrest_gen;
   L.zmask = h->zmask;
+  L.chunks2p = h->chunks2p;
+  L.n2p = h->n2p;
+  L.groups = h->rest_groups.data();
+  L.ngroups = (int)h->rest_groups.size();
   while (h->two_step && it_last - it + 1 >= 3) {
     int rc = launch_double<T>(h, it, L, true);
     if (rc) return rc;
@@ -791,15 +835,13 @@ int create_impl(iris_handle* h, const double* eps_inv, const double* fpmn, const
     HIP_TRY(h, hipMemcpyAsync(h->chunks, all.data(), sizeof(int) * all.size(), hipMemcpyHostToDevice, h->stream));
     HIP_TRY(h, hipStreamSynchronize(h->stream));
 
-    // two steps per pass: chunks whose planes z0-1 .. z1+2 lie between the
-    // UPML slabs and away from the TF/SF planes (the structure is allowed), on
-    // grids without a Bloch phase in x and y and wide enough that a tile's
-    // periodic images stay inside the grid
-    auto is_two_step_plane = [&](int z) {
-      if (z <= p.pmlzi || z >= p.pmlzf) return false;
-      if (p.em_source && (z == p.iz_min - 1 || z == p.iz_min)) return false;
-      return true;
-    };
+    // Two steps per pass.  An output plane z of a two-step chunk needs
+    // 2 <= z <= nz-2 (the wrap in z stays with single steps) and no TF/SF plane
+    // in z-1 .. z+2 (the corrections belong to single steps); the plain kernel
+    // also needs z-1 .. z+2 outside the UPML slabs, the others take its UPML
+    // variant.  Runs of such planes are cut into chunks of up to 2*zc planes
+    // (plain) or one chunk per UPML segment; the remaining planes take single
+    // steps in chunks of up to zc planes with their own lengths.
     // IRIS_TWO_STEP=0: off, =1: on wherever it applies; unset: on when its
     // tiles waste fewer lanes than the single-step pass's, as estimated by
     // the lanes per cell-step of both (64 x R footprint lanes per 61 x (R-3)
@@ -816,62 +858,105 @@ int create_impl(iris_handle* h, const double* eps_inv, const double* fpmn, const
       if (two > 0.95 * one) want = false;
     }
     h->bloch = !(p.ak[0][0] == 0.0 && p.ak[0][1] == 0.0 && p.ak[1][0] == 0.0 && p.ak[1][1] == 0.0);
-    std::vector<char> dbl;
-    int ndbl = 0;
-    for (int z0 = 1; z0 <= p.nz; z0 += zc) {
-      const int z1 = std::min(z0 + zc - 1, p.nz);
-      bool ok = h->fused && want && p.nx >= kT2X && p.ny >= Yee2Shape<T>::R && z1 + 2 <= p.nz &&
-                z1 - z0 + 1 == zc;
-      for (int z = z0 - 1; ok && z <= z1 + 2; ++z) ok = is_two_step_plane(z);
-      dbl.push_back(ok ? 1 : 0);
-      ndbl += ok;
+    const bool shape = h->fused && want && p.nx >= kT2X && p.ny >= Yee2Shape<T>::R && p.nz >= 6;
+    auto tfsf_plane = [&](int z) { return p.em_source && (z == p.iz_min - 1 || z == p.iz_min); };
+    auto upml_plane = [&](int z) { return z <= p.pmlzi || z >= p.pmlzf; };
+    // 0: single step; 1: plain two-step output plane; 2: UPML-variant output plane
+    std::vector<char> kind((size_t)p.nz + 2, 0);
+    int nout = 0;
+    for (int z = 2; shape && z <= p.nz - 2; ++z) {
+      bool ok = true;
+      for (int q = z - 1; ok && q <= z + 2; ++q) ok = !tfsf_plane(q);
+      if (!ok) continue;
+      bool plain = true;
+      for (int q = z - 1; plain && q <= z + 2; ++q) plain = !upml_plane(q);
+      kind[z] = plain ? 1 : 2;
+      ++nout;
     }
-    if (ndbl > 0) {
-      // runs of consecutive two-step chunks are cut into chunks of up to
-      // 2*zc planes (less recomputed halo per plane); the other chunks keep zc
-      std::vector<int> c2, rlean, rgen;
-      std::vector<unsigned char> zm((size_t)p.nz + 2, 0);
-      const int nch = (int)dbl.size();
+    const char* vpml = std::getenv("IRIS_TWO_STEP_UPML");  // 0: leave the UPML planes to single steps
+    if (vpml && vpml[0] == '0')
+      for (int z = 1; z <= p.nz; ++z)
+        if (kind[z] == 2) { kind[z] = 0; --nout; }
+    // a run of two-step planes needs at least two of them of each kind
+    for (int z = 1; z <= p.nz; ++z) {
+      if (!kind[z]) continue;
+      int e = z;
+      while (e + 1 <= p.nz && kind[e + 1] == kind[z]) ++e;
+      if (e == z) { kind[z] = 0; --nout; }
+      z = e;
+    }
+    if (nout > 0) {
       int zc2 = std::min(2 * zc, kChunkLenMask + 1);
       const char* z2env = std::getenv("IRIS_ZCHUNK2");  // planes per two-step chunk (tuning)
       if (z2env && std::atoi(z2env) > 0) zc2 = std::min(std::atoi(z2env), kChunkLenMask + 1);
-      for (int k = 0; k < nch;) {
-        const int z0 = 1 + k * zc;
-        if (!dbl[k]) {
-          const int z1 = std::min(z0 + zc - 1, p.nz);
-          bool lean = z1 < p.nz;
-          for (int z = z0 - 1; lean && z <= z1 + 1; ++z) lean = is_bulk_plane(z);
-          (lean ? rlean : rgen).push_back(z0);
-          ++k;
+      std::vector<int> c2, c2p;
+      std::map<int, std::pair<std::vector<int>, std::vector<int>>> byl;  // length -> (lean, general)
+      std::vector<unsigned char> zm((size_t)p.nz + 2, 0);
+      for (int z = 1; z <= p.nz;) {
+        if (!kind[z]) {
+          // single-step planes z .. e: chunks of zc planes and one shorter
+          int e = z;
+          while (e + 1 <= p.nz && !kind[e + 1]) ++e;
+          for (int c0 = z; c0 <= e; c0 += zc) {
+            const int c1 = std::min(c0 + zc - 1, e);
+            bool lean = c1 < p.nz;
+            for (int q = c0 - 1; lean && q <= c1 + 1; ++q) lean = is_bulk_plane(q);
+            auto& g = byl[c1 - c0 + 1];
+            (lean ? g.first : g.second).push_back(c0);
+          }
+          z = e + 1;
           continue;
         }
-        int k1 = k;
-        while (k1 + 1 < nch && dbl[k1 + 1]) ++k1;
-        const int r0 = z0, r1 = 1 + (k1 + 1) * zc - 1;  // the run's planes
-        for (int c0 = r0; c0 <= r1; c0 += zc2) {
-          const int len = std::min(zc2, r1 - c0 + 1);
-          int v = c0 | ((len - 1) << kChunkLenShift);
-          if (c0 == r0) v |= kChunkExportBelow;
-          if (c0 + len - 1 == r1) v |= kChunkExportAbove;
-          c2.push_back(v);
+        // a run of two-step planes z .. e (both kinds); exports at its ends
+        int e = z;
+        while (e + 1 <= p.nz && kind[e + 1]) ++e;
+        for (int s0 = z; s0 <= e;) {
+          int s1 = s0;
+          while (s1 + 1 <= e && kind[s1 + 1] == kind[s0]) ++s1;
+          const int step = kind[s0] == 1 ? zc2 : std::min(s1 - s0 + 1, kChunkLenMask + 1);
+          for (int c0 = s0; c0 <= s1; c0 += step) {
+            const int len = std::min(step, s1 - c0 + 1);
+            int v = c0 | ((len - 1) << kChunkLenShift);
+            if (c0 == z) v |= kChunkExportBelow;
+            if (c0 + len - 1 == e) v |= kChunkExportAbove;
+            (kind[s0] == 1 ? c2 : c2p).push_back(v);
+          }
+          for (int q = s0; q <= s1; ++q) {
+            zm[q] = 1;
+            if (kind[s0] == 1) h->planes2 += 1;
+            else { h->planes2p += 1; if (upml_plane(q)) h->upml2p += 1; }
+          }
+          s0 = s1 + 1;
         }
-        for (int z = r0; z <= r1; ++z) zm[z] = 1;
-        h->planes2 += r1 - r0 + 1;
-        k = k1 + 1;
+        z = e + 1;
+      }
+      std::vector<int> rest;
+      for (auto& kv : byl) {
+        RestGroup G{kv.first, (int)rest.size(), (int)kv.second.first.size(), (int)kv.second.second.size()};
+        rest.insert(rest.end(), kv.second.first.begin(), kv.second.first.end());
+        rest.insert(rest.end(), kv.second.second.begin(), kv.second.second.end());
+        h->rest_groups.push_back(G);
+        h->nrest_lean += G.nlean;
+        h->nrest_gen += G.ngen;
       }
       h->two_step = true;
       h->n2 = (int)c2.size();
-      h->nrest_lean = (int)rlean.size();
-      h->nrest_gen = (int)rgen.size();
+      h->n2p = (int)c2p.size();
       h->tx2 = (p.nx + kT2OX - 1) / kT2OX;
       h->ty2 = (p.ny + Yee2Shape<T>::OY - 1) / Yee2Shape<T>::OY;
-      std::vector<int> rest(rlean);
-      rest.insert(rest.end(), rgen.begin(), rgen.end());
-      if ((rc = dalloc(h, (void**)&h->chunks2, sizeof(int) * c2.size()))) return rc;
-      if ((rc = dalloc(h, (void**)&h->chunks_rest, sizeof(int) * std::max<size_t>(rest.size(), 1)))) return rc;
-      if ((rc = dalloc(h, (void**)&h->zmask, zm.size()))) return rc;
+      h->ty2p = (p.ny + Yee2Shape<T, true>::OY - 1) / Yee2Shape<T, true>::OY;
+      if ((rc = dalloc(h, (void**)&h->chunks2, sizeof(int) * std::max<size_t>(c2.size(), 1))) ||
+          (rc = dalloc(h, (void**)&h->chunks2p, sizeof(int) * std::max<size_t>(c2p.size(), 1))) ||
+          (rc = dalloc(h, (void**)&h->chunks_rest, sizeof(int) * std::max<size_t>(rest.size(), 1))) ||
+          (rc = dalloc(h, (void**)&h->zmask, zm.size())))
+        return rc;
       if ((rc = dalloc(h, &h->ke, fbytes)) || (rc = dalloc(h, &h->kh, fbytes))) return rc;
-      HIP_TRY(h, hipMemcpyAsync(h->chunks2, c2.data(), sizeof(int) * c2.size(), hipMemcpyHostToDevice, h->stream));
+      if ((rc = dalloc(h, &h->kd, abytes)) || (rc = dalloc(h, &h->kb, abytes))) return rc;
+      if (!c2.empty())
+        HIP_TRY(h, hipMemcpyAsync(h->chunks2, c2.data(), sizeof(int) * c2.size(), hipMemcpyHostToDevice, h->stream));
+      if (!c2p.empty())
+        HIP_TRY(h, hipMemcpyAsync(h->chunks2p, c2p.data(), sizeof(int) * c2p.size(), hipMemcpyHostToDevice,
+                                  h->stream));
       if (!rest.empty())
         HIP_TRY(h, hipMemcpyAsync(h->chunks_rest, rest.data(), sizeof(int) * rest.size(), hipMemcpyHostToDevice,
                                   h->stream));
@@ -1131,10 +1216,10 @@ int adjoint_double(iris_handle* h, int n, int nobox, int face_on, const void* re
     Yee2Samples<T> none{};
     if (h->bloch)
       hipLaunchKernelGGL((k_yee2<T, true>), dim3((unsigned)tm.nblk), dim3(kT2X, Yee2Shape<T>::R), 0, h->stream, a2,
-                         (C*)h->ae2, (C*)h->ah2, (const int*)L.chunks2, tm, none);
+                         (C*)h->ae2, (C*)h->ah2, nullptr, (const int*)L.chunks2, tm, none);
     else
       hipLaunchKernelGGL((k_yee2<T, false>), dim3((unsigned)tm.nblk), dim3(kT2X, Yee2Shape<T>::R), 0, h->stream, a2,
-                         (C*)h->ae2, (C*)h->ah2, (const int*)L.chunks2, tm, none);
+                         (C*)h->ae2, (C*)h->ah2, nullptr, (const int*)L.chunks2, tm, none);
   }
   const StepArgs<T> a3 = adj_args<T>(h, h->ae2, h->ah2, h->ae3, h->ah3, h->ad2, h->ab2, h->ad, h->ab);
   adj_sources<T>(h, a3, n - 1, nobox, face_on);
@@ -1963,12 +2048,14 @@ int iris_diagnostics(iris_handle* h, int32_t zi, int32_t zf, double out[4]) {
   return 0;
 }
 
-int iris_pass_info(const iris_handle* h, int64_t info[4]) {
+int iris_pass_info(const iris_handle* h, int64_t info[6]) {
   if (!h || !info) return IRIS_EINVAL;
   info[0] = h->zchunk;
   info[1] = h->two_step ? h->planes2 : 0;
   info[2] = h->two_step ? h->n2 : 0;
   info[3] = h->two_step ? 1 : 0;
+  info[4] = h->two_step ? h->planes2p : 0;
+  info[5] = h->two_step ? h->upml2p : 0;
   return 0;
 }
 

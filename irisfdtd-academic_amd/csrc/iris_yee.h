@@ -387,6 +387,14 @@ __device__ inline void e_item(const StepArgs<T>& a, int xs, int ys, int zs, cplx
 // columns overlap, share an L2, while every chunk is still spread over all
 // XCDs.  Tiles past the last whole round of runs, and run = 0, keep the
 // identity order t = L.
+// chunks[] word of the two-step pass: first plane (bits 0-19), planes - 1
+// (bits 20-27), export flags (bits 28-29).  A single-step chunk is its first
+// plane; it spans a.zchunk planes (up to nz).
+constexpr int kChunkZMask = (1 << 20) - 1;
+constexpr int kChunkLenShift = 20, kChunkLenMask = 0xff;
+constexpr int kChunkExportBelow = 1 << 28;
+constexpr int kChunkExportAbove = 1 << 29;
+
 struct TileMap {
   int tx, ty;  // tiles along x and y
   int nblk;    // workgroups in the launch (tx * ty * chunks)

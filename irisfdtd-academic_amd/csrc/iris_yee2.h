@@ -38,17 +38,13 @@ constexpr int kT2OX = kT2X - 3;  // output cells per row of a tile
 // Waves per workgroup (footprint rows) and the waves per SIMD the register
 // allocation must allow: FP32 fields 16 rows, one workgroup per CU (98 KB of
 // LDS); FP64 fields 12 rows, one workgroup per CU (147 KB of LDS).
-template <typename T> struct Yee2Shape {
-  static constexpr int R = sizeof(T) == 4 ? 16 : 12;
+// The UPML variant carries D and B through the pipeline as well (twelve
+// more complex registers per lane) and uses 12 / 8 rows.
+template <typename T, bool PML = false> struct Yee2Shape {
+  static constexpr int R = sizeof(T) == 4 ? (PML ? 12 : 16) : (PML ? 8 : 12);
   static constexpr int OY = R - 3;  // output rows per tile
   static constexpr int WAVES_PER_EU = (R + 3) / 4;
 };
-// chunks[] word of the two-step pass: first plane (bits 0-19), planes - 1
-// (bits 20-27), export flags (bits 28-29)
-constexpr int kChunkZMask = (1 << 20) - 1;
-constexpr int kChunkLenShift = 20, kChunkLenMask = 0xff;
-constexpr int kChunkExportBelow = 1 << 28;
-constexpr int kChunkExportAbove = 1 << 29;
 
 // Index in the concatenated face-point list of box g (inverse of box_point,
 // reference code/dataprocess.f90:813-901): store E (base 0) or H (base 3) of
@@ -109,6 +105,29 @@ __device__ inline void e_update_plain(const StepArgs<T>& a, int64_t col, int z, 
   }
 }
 
+// z-UPML updates of one cell with the operations of e_cell / h_cell
+// (reference code/boundary_II.f90:84-99, :170-183): from the old aux value
+// and the curl, the new aux value and the new field.
+template <typename T>
+__device__ inline void e_upml3(const PmlPlane<T>& P, T e02, const cplx<T> e[3], const cplx<T> c[3],
+                               const cplx<T> dold[3], cplx<T> dnew[3], cplx<T> n[3]) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const cplx<T> cc = cscale(e02, c[k]);
+    dnew[k] = cadd(cscale(P.cd[k], dold[k]), cscale(P.gd[k], cc));
+    n[k] = cadd(cscale(P.ce[k], e[k]), cscale(P.ge[k], csub(cscale(P.fp[k], dnew[k]), cscale(P.fm[k], dold[k]))));
+  }
+}
+template <typename T>
+__device__ inline void h_upml3(const PmlPlane<T>& P, const cplx<T> h[3], const cplx<T> c[3], const cplx<T> bold[3],
+                               cplx<T> bnew[3], cplx<T> n[3]) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    bnew[k] = csub(cscale(P.cd[k], bold[k]), cscale(P.gb[k], c[k]));
+    n[k] = cadd(cscale(P.ce[k], h[k]), cscale(P.gh[k], csub(cscale(P.fp[k], bnew[k]), cscale(P.fm[k], bold[k]))));
+  }
+}
+
 // Sampling of step n+1 by k_yee2: up to two boxes; samp[b] already points at
 // the ring slot of that step, or is null when the step is not sampled.
 template <typename T> struct Yee2Samples {
@@ -125,12 +144,19 @@ template <typename T> struct Yee2Samples {
 // The results equal those of single steps up to rounding in the cells next
 // to the period boundary (the single-step pass applies the factor after
 // forming an image instead of before).
-template <typename T, bool PH>
-__global__ __launch_bounds__(kT2X * Yee2Shape<T>::R, Yee2Shape<T>::WAVES_PER_EU) void k_yee2(StepArgs<T> a, cplx<T>* __restrict__ eb,
-                                                      cplx<T>* __restrict__ hb, const int* __restrict__ chunks,
-                                                      TileMap tm, Yee2Samples<T> smp) {
+// PML = true: chunks that reach into the z-UPML slabs (planes 2 .. nz-2; the
+// wrap in z is left to single steps).  On UPML planes the lane also reads
+// D (of step n) with E and B with H, keeps D and B of step n+1 beside E1 and
+// H1, and stores D and B of step n+2 into a.dn / a.bn (a third aux buffer:
+// other workgroups still read the step-n values for their halos).  With
+// the export below, D of step n+1 on plane z0 also goes to db: the
+// single-step pass under the chunk recomputes E of step n+2 there.
+template <typename T, bool PH, bool PML = false>
+__global__ __launch_bounds__((kT2X * Yee2Shape<T, PML>::R), (Yee2Shape<T, PML>::WAVES_PER_EU)) void k_yee2(
+    StepArgs<T> a, cplx<T>* __restrict__ eb, cplx<T>* __restrict__ hb, cplx<T>* __restrict__ db,
+    const int* __restrict__ chunks, TileMap tm, Yee2Samples<T> smp) {
   using C = cplx<T>;
-  constexpr int kT2R = Yee2Shape<T>::R, kT2OY = Yee2Shape<T>::OY;
+  constexpr int kT2R = Yee2Shape<T, PML>::R, kT2OY = Yee2Shape<T, PML>::OY;
   __shared__ C s1[2][3][kT2R][kT2X];  // E1 of planes z (z & 1) and z-1
   __shared__ C sh[3][kT2R][kT2X];  // H1 of plane z-1
   __shared__ C s2[3][kT2R][kT2X];  // E2 of plane z-1
@@ -170,8 +196,16 @@ __global__ __launch_bounds__(kT2X * Yee2Shape<T>::R, Yee2Shape<T>::WAVES_PER_EU)
     return f[i];
   };
 
-  // pipeline registers of this lane: H0(z-1), E1(z-1), H1(z-2), E2(z-2)
+  auto upml = [&](int z) { return PML && (z <= a.pmlzi || z >= a.pmlzf); };
+  auto auxi = [&](int z, int k) -> int64_t {  // aux index of component k of this lane's cell on UPML plane z
+    const int top = z >= a.pmlzf;
+    return a.aux(top, k, col + sxy * (top ? (z - a.pmlzf) : (z - 1)));
+  };
+
+  // pipeline registers of this lane: H0(z-1), E1(z-1), H1(z-2), E2(z-2), and
+  // on UPML planes D1(z-1), B1(z-2)
   C hz1[3], e1p[3] = {}, h1pp[3] = {}, e2p[3] = {};
+  C d1p[3] = {}, b1pp[3] = {};
   {
     const int64_t i = col + sxy * (z0 - 2);
     hz1[0] = ld(h0, i);
@@ -182,7 +216,7 @@ __global__ __launch_bounds__(kT2X * Yee2Shape<T>::R, Yee2Shape<T>::WAVES_PER_EU)
   for (int z = z0 - 1; z <= z1 + 2; ++z) {
     const int64_t iz = col + sxy * z;
     // ---- E1(z) from H0(z), H0(z-1), E0(z) (code/fdtd.f90:78-96, :137-147)
-    C h0z[3], e1[3];
+    C h0z[3], e1[3], d1n[3] = {}, b1n[3] = {};
     {
       h0z[0] = ld(h0, iz);
       h0z[1] = ld(h0, iz + cs);
@@ -193,7 +227,12 @@ __global__ __launch_bounds__(kT2X * Yee2Shape<T>::R, Yee2Shape<T>::WAVES_PER_EU)
                       cadd(csub(csub(h0z[0], hz1[0]), h0z[2]), h3xm),
                       cadd(csub(csub(h0z[1], h2xm), h0z[0]), h1ym)};
       const C eo[3] = {ld(e0, iz), ld(e0, iz + cs), ld(e0, iz + 2 * cs)};
-      e_update_plain<T>(a, col, z, eo, c, e1);
+      if (upml(z)) {
+        const C dold[3] = {ld(a.d, auxi(z, 0)), ld(a.d, auxi(z, 1)), ld(a.d, auxi(z, 2))};
+        e_upml3<T>(a.pml[z], a.e02, eo, c, dold, d1n, e1);
+      } else {
+        e_update_plain<T>(a, col, z, eo, c, e1);
+      }
     }
     // ---- H1(z-1) from H0(z-1), E1(z-1) (own and LDS +x/+y), E1(z)
     // (code/fdtd.f90:182-195)
@@ -205,9 +244,15 @@ __global__ __launch_bounds__(kT2X * Yee2Shape<T>::R, Yee2Shape<T>::WAVES_PER_EU)
       const C c0 = cadd(csub(csub(e3yp, e1p[2]), e1[1]), e1p[1]);
       const C c1 = cadd(csub(csub(e1[0], e1p[0]), e3xp), e1p[2]);
       const C c2 = cadd(csub(csub(e2xp, e1p[1]), e1yp), e1p[0]);
-      h1n[0] = csub(hz1[0], cscale(a.dtcq2, c0));
-      h1n[1] = csub(hz1[1], cscale(a.dtcq2, c1));
-      h1n[2] = csub(hz1[2], cscale(a.dtcq2, c2));
+      if (upml(z - 1)) {
+        const C bold[3] = {ld(a.b, auxi(z - 1, 0)), ld(a.b, auxi(z - 1, 1)), ld(a.b, auxi(z - 1, 2))};
+        const C cc[3] = {c0, c1, c2};
+        h_upml3<T>(a.pml[z - 1], hz1, cc, bold, b1n, h1n);
+      } else {
+        h1n[0] = csub(hz1[0], cscale(a.dtcq2, c0));
+        h1n[1] = csub(hz1[1], cscale(a.dtcq2, c1));
+        h1n[2] = csub(hz1[2], cscale(a.dtcq2, c2));
+      }
     }
     // slot pz last held E1(z-2), read before the previous plane's second
     // barrier; sh was last read before the previous plane's second barrier
@@ -219,26 +264,35 @@ __global__ __launch_bounds__(kT2X * Yee2Shape<T>::R, Yee2Shape<T>::WAVES_PER_EU)
     sh[2][wy][lane] = h1n[2];
     __syncthreads();
     // ---- E2(z-1) from E1(z-1), H1(z-1) (own and LDS -x/-y), H1(z-2)
-    C e2n[3] = {};
+    C e2n[3] = {}, d2n[3] = {};
     if (z >= z0 + 1) {
       const C h3ym = sh[2][wm][lane], h1ym = sh[0][wm][lane];
       const C h3xm = sh[2][wy][lm], h2xm = sh[1][wy][lm];
       const C c[3] = {cadd(csub(csub(h1n[2], h3ym), h1n[1]), h1pp[1]),
                       cadd(csub(csub(h1n[0], h1pp[0]), h1n[2]), h3xm),
                       cadd(csub(csub(h1n[1], h2xm), h1n[0]), h1ym)};
-      e_update_plain<T>(a, col, z - 1, e1p, c, e2n);
+      if (upml(z - 1)) {
+        e_upml3<T>(a.pml[z - 1], a.e02, e1p, c, d1p, d2n, e2n);
+      } else {
+        e_update_plain<T>(a, col, z - 1, e1p, c, e2n);
+      }
     }
     // ---- H2(z-2) from H1(z-2), E2(z-2) (own and LDS +x/+y), E2(z-1)
-    C h2n[3] = {};
+    C h2n[3] = {}, b2n[3] = {};
     if (z >= z0 + 2) {
       const C e3yp = s2[2][wp][lane], e1yp = s2[0][wp][lane];
       const C e3xp = s2[2][wy][lp], e2xp = s2[1][wy][lp];
       const C c0 = cadd(csub(csub(e3yp, e2p[2]), e2n[1]), e2p[1]);
       const C c1 = cadd(csub(csub(e2n[0], e2p[0]), e3xp), e2p[2]);
       const C c2 = cadd(csub(csub(e2xp, e2p[1]), e1yp), e2p[0]);
-      h2n[0] = csub(h1pp[0], cscale(a.dtcq2, c0));
-      h2n[1] = csub(h1pp[1], cscale(a.dtcq2, c1));
-      h2n[2] = csub(h1pp[2], cscale(a.dtcq2, c2));
+      if (upml(z - 2)) {
+        const C cc[3] = {c0, c1, c2};
+        h_upml3<T>(a.pml[z - 2], h1pp, cc, b1pp, b2n, h2n);
+      } else {
+        h2n[0] = csub(h1pp[0], cscale(a.dtcq2, c0));
+        h2n[1] = csub(h1pp[1], cscale(a.dtcq2, c1));
+        h2n[2] = csub(h1pp[2], cscale(a.dtcq2, c2));
+      }
     }
     __syncthreads();  // the E2(z-2) neighbours and H1(z-1) neighbours have been read
     s2[0][wy][lane] = e2n[0];
@@ -247,15 +301,34 @@ __global__ __launch_bounds__(kT2X * Yee2Shape<T>::R, Yee2Shape<T>::WAVES_PER_EU)
 
     if (own) {
       // step n+2 of the chunk's planes, with H's min-side ghost planes
-      if (z >= z0 + 1 && z - 1 <= z1) store3(a.en, col + sxy * (z - 1), cs, e2n);
+      if (z >= z0 + 1 && z - 1 <= z1) {
+        store3(a.en, col + sxy * (z - 1), cs, e2n);
+        if (upml(z - 1)) {
+          a.dn[auxi(z - 1, 0)] = d2n[0];
+          a.dn[auxi(z - 1, 1)] = d2n[1];
+          a.dn[auxi(z - 1, 2)] = d2n[2];
+        }
+      }
       if (z >= z0 + 2) {
         const int64_t ih = col + sxy * (z - 2);
         store3(a.hn, ih, cs, h2n);
+        if (upml(z - 2)) {
+          a.bn[auxi(z - 2, 0)] = b2n[0];
+          a.bn[auxi(z - 2, 1)] = b2n[1];
+          a.bn[auxi(z - 2, 2)] = b2n[2];
+        }
         if (xs == nx) store3_phase(a.hn, ih - nx, cs, a.fxm, h2n);
         if (ys == ny) store3_phase(a.hn, ih - a.sx * ny, cs, a.fym, h2n);
       }
       // step n+1 planes read by the single-step passes of the neighbouring chunks
-      if (exp_below && z == z0) store3(eb, iz, cs, e1);
+      if (exp_below && z == z0) {
+        store3(eb, iz, cs, e1);
+        if (upml(z)) {
+          db[auxi(z, 0)] = d1n[0];
+          db[auxi(z, 1)] = d1n[1];
+          db[auxi(z, 2)] = d1n[2];
+        }
+      }
       if (exp_below && z - 1 == z0) {
         const int64_t ih = col + sxy * z0;
         store3(hb, ih, cs, h1n);
@@ -276,6 +349,10 @@ __global__ __launch_bounds__(kT2X * Yee2Shape<T>::R, Yee2Shape<T>::WAVES_PER_EU)
       e1p[k] = e1[k];
       if (z >= z0) h1pp[k] = h1n[k];
       e2p[k] = e2n[k];
+      if constexpr (PML) {
+        d1p[k] = d1n[k];
+        if (z >= z0) b1pp[k] = b1n[k];
+      }
     }
   }
 }

@@ -134,7 +134,7 @@ EXPORTED = ["iris_create", "iris_step", "iris_sync", "iris_flux", "iris_dft_poin
             "iris_batch_step", "iris_batch_info", "iris_batch_destroy", "iris_batch_last_error"]
 
 KERNELS = ["epass", "hpass", "tfsf", "dft_sample", "dft_accum", "flux", "yee", "yee2", "yee_rest", "adjoint",
-           "checkpoint"]
+           "checkpoint", "yee2_pml"]
 
 
 class Engine:
@@ -227,11 +227,12 @@ class Engine:
         return out
 
     def pass_info(self):
-        """{zchunk, two_step_planes, two_step_chunks, two_step}: the layout of
-        the step passes."""
-        v = np.zeros(4, dtype=np.int64)
+        """{zchunk, two_step_planes, two_step_chunks, two_step, upml_two_step_planes,
+        upml_planes_in_two_step}: the layout of the step passes."""
+        v = np.zeros(6, dtype=np.int64)
         self._check(self.lib.iris_pass_info(self._h, _p(v)), "iris_pass_info")
-        return dict(zchunk=int(v[0]), two_step_planes=int(v[1]), two_step_chunks=int(v[2]), two_step=bool(v[3]))
+        return dict(zchunk=int(v[0]), two_step_planes=int(v[1]), two_step_chunks=int(v[2]), two_step=bool(v[3]),
+                    upml_two_step_planes=int(v[4]), upml_planes_in_two_step=int(v[5]))
 
     def checkpoint(self, segment, record_bytes=0):
         """Prepare the coming forward run for adjoint(): H of its last steps
