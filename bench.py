@@ -278,7 +278,7 @@ def run_batch(precision=4, n=BATCH_N, steps=BATCH_STEPS):
             "sequential_s": round(t_seq, 4), "batched_s": round(t_bat, 4), "speedup": round(t_seq / t_bat, 3),
             "batched_gcells": round(cells * steps / t_bat / 1e9, 3),
             "sequential_gcells": round(cells * steps / t_seq / 1e9, 3),
-            "batch_zchunk": info["zchunk"], "own_two_step": info["own_two_step"],
+            "batch_zchunk": info["zchunk"], "two_step_handles": info["two_step"],
             "flux_bitwise_equal": bool(same)}
 
 

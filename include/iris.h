@@ -257,9 +257,10 @@ int iris_adjoint(iris_handle* h, int32_t nobox, const double* weights, double* g
  * incidence angles, Bloch phases or structures puts one handle per run in a
  * batch; each step is then one launch of the fused pass over the tiles of
  * every run (plus one launch for the DFT samples), with z-chunks as long as
- * the whole batch allows.  Runs that the two-step pass covers keep their own
- * two-step passes (on the batch's stream).  Results are bitwise those of
- * stepping each handle alone. ---- */
+ * the whole batch allows.  Runs that the two-step pass covers take it
+ * batched as well: each pair of steps is one launch per kernel over all of
+ * them (their single-step chunks, plain and UPML two-step chunks, DFT
+ * samples).  Results are bitwise those of stepping each handle alone. ---- */
 typedef struct iris_batch iris_batch;
 
 /* Group n handles (created with iris_create, same precision and device, the
@@ -274,7 +275,7 @@ int iris_batch_step(iris_batch* b, int32_t it_first, int32_t it_last);
 
 /* info[0] = handles, info[1] = planes per z-chunk of the batched pass,
  * info[2] / info[3] = workgroups of its lean / general launch, info[4] =
- * handles that take their own two-step passes. */
+ * handles that take the (batched) two-step pass. */
 int iris_batch_info(const iris_batch* b, int64_t info[5]);
 
 int iris_batch_destroy(iris_batch* b);

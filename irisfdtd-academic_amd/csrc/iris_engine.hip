@@ -222,6 +222,21 @@ struct iris_batch {
   int nent = 0;
   int64_t npts = 0;
   std::vector<char> host_inst, host_ent;
+  // the two-step handles (`own`), batched: pairs of steps by one launch per
+  // kernel over all of them.  Tables per pair parity (the pair runs from the
+  // current buffers to the third ones, the next pair back).
+  std::vector<iris_handle*> own2[2];  // without / with a Bloch phase
+  void* y2[2][2] = {};                // Batch2Inst<T>[2][n]: [bloch][upml variant], pair parity outer
+  int* y2wg[2][2] = {};               // workgroup prefixes
+  int y2nwg[2][2] = {};
+  void* rinst = nullptr;              // BatchInst<T>[2][nr]: their single-step chunks, per (handle, length group)
+  int nr = 0;
+  int* rwg[2] = {nullptr, nullptr};   // per class (lean, general)
+  int rnwg[2] = {0, 0};
+  void* ent2 = nullptr;               // BatchSamp<T>[2][nent2]: step n+1 (masked), step n+2
+  int64_t* ent2_start = nullptr;
+  int nent2 = 0;
+  int64_t npts2 = 0;
 };
 
 namespace {
@@ -839,8 +854,8 @@ int create_impl(iris_handle* h, const double* eps_inv, const double* fpmn, const
     // 2 <= z <= nz-2 (the wrap in z stays with single steps) and no TF/SF plane
     // in z-1 .. z+2 (the corrections belong to single steps); the plain kernel
     // also needs z-1 .. z+2 outside the UPML slabs, the others take its UPML
-    // variant.  Runs of such planes are cut into chunks of up to 2*zc planes
-    // (plain) or one chunk per UPML segment; the remaining planes take single
+    // variant.  Runs of planes of one kind are cut into near-equal chunks of
+    // up to 2*zc planes; the remaining planes take single
     // steps in chunks of up to zc planes with their own lengths.
     // IRIS_TWO_STEP=0: off, =1: on wherever it applies; unset: on when its
     // tiles waste fewer lanes than the single-step pass's, as estimated by
@@ -873,8 +888,32 @@ int create_impl(iris_handle* h, const double* eps_inv, const double* fpmn, const
       kind[z] = plain ? 1 : 2;
       ++nout;
     }
-    const char* vpml = std::getenv("IRIS_TWO_STEP_UPML");  // 0: leave the UPML planes to single steps
-    if (vpml && vpml[0] == '0')
+    // The UPML variant by default only where its launch gives every CU a
+    // workgroup: its slabs are a few dozen planes, and on small grids the
+    // few workgroups of its launch took longer than the single steps they
+    // replace (measured on 64 gratings of 32..128 x 32..128 x 768 cells).
+    // IRIS_TWO_STEP_UPML=0: never, =1: wherever it applies.
+    const char* vpml = std::getenv("IRIS_TWO_STEP_UPML");
+    bool upml_two = !(vpml && vpml[0] == '0');
+    if (upml_two && !(vpml && vpml[0] == '1')) {
+      const int zc2 = std::min(2 * zc, kChunkLenMask + 1);
+      int64_t nch = 0;
+      for (int z = 1; z <= p.nz; ++z) {
+        if (kind[z] != 2) continue;
+        int e = z;
+        while (e + 1 <= p.nz && kind[e + 1] == 2) ++e;
+        nch += (e - z + 1 + zc2 - 1) / zc2;
+        z = e;
+      }
+      const int64_t tiles = (int64_t)((p.nx + kT2OX - 1) / kT2OX) *
+                            ((p.ny + Yee2Shape<T, true>::OY - 1) / Yee2Shape<T, true>::OY);
+      int dev = 0, ncu = 256;
+      if (hipGetDevice(&dev) != hipSuccess ||
+          hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        ncu = 256;
+      if (tiles * nch < ncu) upml_two = false;
+    }
+    if (!upml_two)
       for (int z = 1; z <= p.nz; ++z)
         if (kind[z] == 2) { kind[z] = 0; --nout; }
     // a run of two-step planes needs at least two of them of each kind
@@ -913,13 +952,16 @@ int create_impl(iris_handle* h, const double* eps_inv, const double* fpmn, const
         for (int s0 = z; s0 <= e;) {
           int s1 = s0;
           while (s1 + 1 <= e && kind[s1 + 1] == kind[s0]) ++s1;
-          const int step = kind[s0] == 1 ? zc2 : std::min(s1 - s0 + 1, kChunkLenMask + 1);
-          for (int c0 = s0; c0 <= s1; c0 += step) {
-            const int len = std::min(step, s1 - c0 + 1);
+          // near-equal chunks of at most zc2 planes (a short last chunk
+          // would repeat the pipeline's 3 planes of fill for few planes)
+          const int nseg = s1 - s0 + 1, nch = (nseg + zc2 - 1) / zc2;
+          for (int k = 0, c0 = s0; k < nch; ++k) {
+            const int len = nseg / nch + (k < nseg % nch ? 1 : 0);
             int v = c0 | ((len - 1) << kChunkLenShift);
             if (c0 == z) v |= kChunkExportBelow;
             if (c0 + len - 1 == e) v |= kChunkExportAbove;
             (kind[s0] == 1 ? c2 : c2p).push_back(v);
+            c0 += len;
           }
           for (int q = s0; q <= s1; ++q) {
             zm[q] = 1;
@@ -1538,6 +1580,228 @@ int batch_tables(iris_batch* b) {
   return 0;
 }
 
+// argument tables of the two-step handles for their current buffers: the
+// pair of steps from the current buffers (pair parity 0) or from the third
+// ones (parity 1); step n+1 always goes to the second buffers
+template <typename T>
+int batch_tables2(iris_batch* b) {
+  using C = cplx<T>;
+  if (b->own.empty()) return 0;
+  for (int bl = 0; bl < 2; ++bl) {
+    const std::vector<iris_handle*>& hs = b->own2[bl];
+    const int n = (int)hs.size();
+    if (!n) continue;
+    for (int pm = 0; pm < 2; ++pm) {
+      std::vector<Batch2Inst<T>> I((size_t)2 * n);
+      for (int pp = 0; pp < 2; ++pp)
+        for (int i = 0; i < n; ++i) {
+          iris_handle* h = hs[i];
+          Batch2Inst<T>& B = I[(size_t)pp * n + i];
+          void *A0 = pp ? h->ke : h->fe, *A1 = pp ? h->kh : h->fh, *C0 = pp ? h->fe : h->ke, *C1 = pp ? h->fh : h->kh;
+          void *AD = pp ? h->kd : h->d, *AB = pp ? h->kb : h->b, *CD = pp ? h->d : h->kd, *CB = pp ? h->b : h->kb;
+          B.a = pm ? step_args_on<T>(h, A0, A1, C0, C1, AD, AB, CD, CB)
+                   : step_args_on<T>(h, A0, A1, C0, C1, AD, AB, h->dn, h->bn);
+          B.eb = (C*)h->ge;
+          B.hb = (C*)h->gh;
+          B.db = pm ? (C*)h->dn : nullptr;
+          B.chunks = pm ? h->chunks2p : h->chunks2;
+          const int ty = pm ? h->ty2p : h->ty2;
+          B.tm = TileMap{h->tx2, ty, h->tx2 * ty * (pm ? h->n2p : h->n2), h->tx2 * h->xcd_rows2};
+          B.smp = Yee2Samples<T>{};
+          B.smp.nbox = h->p.nbox;
+          for (int bx = 0; bx < h->p.nbox; ++bx) {
+            B.smp.g[bx] = h->box[bx].g;
+            B.smp.samp[bx] = h->box[bx].g.ntot ? (C*)h->box[bx].samp : nullptr;
+          }
+        }
+      BHIP_TRY(b, hipMemcpy(b->y2[bl][pm], I.data(), sizeof(Batch2Inst<T>) * I.size(), hipMemcpyHostToDevice));
+    }
+  }
+  // single steps: entry (handle, length group); a[0] = step n+1, a[1] = step n+2
+  std::vector<BatchInst<T>> R((size_t)2 * b->nr);
+  std::vector<BatchSamp<T>> E((size_t)2 * b->nent2);
+  for (int pp = 0; pp < 2; ++pp) {
+    int r = 0, e = 0;
+    for (iris_handle* h : b->own) {
+      void *A0 = pp ? h->ke : h->fe, *A1 = pp ? h->kh : h->fh, *C0 = pp ? h->fe : h->ke, *C1 = pp ? h->fh : h->kh;
+      void *AD = pp ? h->kd : h->d, *AB = pp ? h->kb : h->b, *CD = pp ? h->d : h->kd, *CB = pp ? h->b : h->kb;
+      const StepArgs<T> s1 = step_args_on<T>(h, A0, A1, h->ge, h->gh, AD, AB, h->dn, h->bn);
+      const StepArgs<T> s2 = step_args_on<T>(h, h->ge, h->gh, C0, C1, h->dn, h->bn, CD, CB);
+      const int tx = (int)h->grd_fused.x, ty = (int)h->grd_fused.y;
+      for (const RestGroup& G : h->rest_groups) {
+        BatchInst<T>& I = R[(size_t)pp * b->nr + r++];
+        I.a[0] = s1;
+        I.a[1] = s2;
+        I.a[0].zchunk = I.a[1].zchunk = G.len;
+        I.tp = (const TfsfConst*)h->tfc;
+        I.chunks[0] = h->chunks_rest + G.off;
+        I.chunks[1] = h->chunks_rest + G.off + G.nlean;
+        I.tm[0] = TileMap{tx, ty, tx * ty * G.nlean, tx * h->xcd_rows};
+        I.tm[1] = TileMap{tx, ty, tx * ty * G.ngen, tx * h->xcd_rows};
+      }
+      // samples: pp 0 = step n+1 from the second buffers outside the
+      // two-step planes; pp 1 = step n+2, a[par] by the pair's parity
+      for (int bx = 0; bx < h->p.nbox; ++bx) {
+        if (h->box[bx].g.ntot == 0) continue;
+        BatchSamp<T>& S = E[(size_t)pp * b->nent2 + e++];
+        S.g = h->box[bx].g;
+        S.samp = (C*)h->box[bx].samp;
+        if (pp == 0) {
+          S.a[0] = S.a[1] = step_args_on<T>(h, h->ge, h->gh, h->ge, h->gh, h->dn, h->bn, h->dn, h->bn);
+          S.zmask = h->zmask;
+        } else {
+          S.a[0] = step_args_on<T>(h, h->ke, h->kh, h->ke, h->kh, h->kd, h->kb, h->kd, h->kb);
+          S.a[1] = step_args_on<T>(h, h->fe, h->fh, h->fe, h->fh, h->d, h->b, h->d, h->b);
+          S.zmask = nullptr;
+        }
+      }
+    }
+  }
+  if (b->nr) BHIP_TRY(b, hipMemcpy(b->rinst, R.data(), sizeof(BatchInst<T>) * R.size(), hipMemcpyHostToDevice));
+  if (b->nent2) BHIP_TRY(b, hipMemcpy(b->ent2, E.data(), sizeof(BatchSamp<T>) * E.size(), hipMemcpyHostToDevice));
+  return 0;
+}
+
+// workgroup prefixes and table memory of the batched two-step handles
+template <typename T>
+int batch_create2(iris_batch* b) {
+  int rc;
+  for (iris_handle* h : b->own) b->own2[h->bloch ? 1 : 0].push_back(h);
+  for (int bl = 0; bl < 2; ++bl) {
+    const std::vector<iris_handle*>& hs = b->own2[bl];
+    if (hs.empty()) continue;
+    for (int pm = 0; pm < 2; ++pm) {
+      std::vector<int> wg = {0};
+      for (iris_handle* h : hs) {
+        const int64_t nb = (int64_t)h->tx2 * (pm ? h->ty2p : h->ty2) * (pm ? h->n2p : h->n2);
+        wg.push_back(wg.back() + (int)((nb + kNumXcd - 1) / kNumXcd * kNumXcd));
+      }
+      if ((rc = balloc(b, (void**)&b->y2wg[bl][pm], sizeof(int) * wg.size())) ||
+          (rc = balloc(b, &b->y2[bl][pm], sizeof(Batch2Inst<T>) * 2 * hs.size())))
+        return rc;
+      BHIP_TRY(b, hipMemcpy(b->y2wg[bl][pm], wg.data(), sizeof(int) * wg.size(), hipMemcpyHostToDevice));
+      b->y2nwg[bl][pm] = wg.back();
+    }
+  }
+  std::vector<int> wg[2] = {{0}, {0}};
+  std::vector<int64_t> start = {0};
+  for (iris_handle* h : b->own) {
+    const int t = (int)(h->grd_fused.x * h->grd_fused.y);
+    for (const RestGroup& G : h->rest_groups) {
+      wg[0].push_back(wg[0].back() + t * G.nlean);
+      wg[1].push_back(wg[1].back() + t * G.ngen);
+      b->nr++;
+    }
+    for (int bx = 0; bx < h->p.nbox; ++bx)
+      if (h->box[bx].g.ntot) start.push_back(start.back() + h->box[bx].g.ntot);
+  }
+  for (int k = 0; k < 2; ++k) {
+    if ((rc = balloc(b, (void**)&b->rwg[k], sizeof(int) * wg[k].size()))) return rc;
+    BHIP_TRY(b, hipMemcpy(b->rwg[k], wg[k].data(), sizeof(int) * wg[k].size(), hipMemcpyHostToDevice));
+    b->rnwg[k] = wg[k].back();
+  }
+  b->nent2 = (int)start.size() - 1;
+  b->npts2 = start.back();
+  if ((rc = balloc(b, (void**)&b->ent2_start, sizeof(int64_t) * start.size())) ||
+      (rc = balloc(b, &b->rinst, sizeof(BatchInst<T>) * 2 * std::max(b->nr, 1))) ||
+      (rc = balloc(b, &b->ent2, sizeof(BatchSamp<T>) * 2 * std::max(b->nent2, 1))))
+    return rc;
+  BHIP_TRY(b, hipMemcpy(b->ent2_start, start.data(), sizeof(int64_t) * start.size(), hipMemcpyHostToDevice));
+  return 0;
+}
+
+// one DFT sample of every two-step handle of the batch (table half `half`,
+// state a[par]), then each handle's ring bookkeeping as dft_sample does it
+template <typename T>
+int batch_sample2(iris_batch* b, int half, int par, int n) {
+  iris_handle* h0 = b->own[0];
+  if (b->nent2)
+    hipLaunchKernelGGL(k_dft_sample_batch<T>, dim3((unsigned)((b->npts2 + 255) / 256)), dim3(256), 0, b->stream,
+                       (const BatchSamp<T>*)b->ent2 + (size_t)half * b->nent2, (const int64_t*)b->ent2_start,
+                       b->nent2, par, h0->nsamp);
+  for (iris_handle* h : b->own) {
+    if (h->nsamp == 0) h->samp_n0 = n;
+    h->nsamp++;
+    if (h->nsamp == dft_slots<T>()) {
+      const int rc = dft_flush<T>(h);
+      if (rc) {
+        b->err = h->err;
+        return rc;
+      }
+    }
+  }
+  return 0;
+}
+
+// steps it_first..it_last of the two-step handles: pairs of steps while at
+// least three remain (as run_steps takes them alone), then single steps
+template <typename T>
+int batch_run2(iris_batch* b, int it_first, int it_last) {
+  if (b->own.empty()) return 0;
+  const iris_params& p = b->own[0]->p;
+  int it = it_first, pp = 0, rc = 0;
+  auto rest = [&](int k, int it_step) {
+    for (int c = 0; c < 2; ++c) {
+      if (!b->rnwg[c]) continue;
+      const BatchInst<T>* I = (const BatchInst<T>*)b->rinst + (size_t)pp * b->nr;
+      if (c == 0)
+        hipLaunchKernelGGL((k_yee_fused_batch<T, false>), dim3(b->rnwg[0]), dim3(kFTX, kFTY + 1), 0, b->stream, I,
+                           (const int*)b->rwg[0], b->nr, k, it_step);
+      else
+        hipLaunchKernelGGL((k_yee_fused_batch<T, true>), dim3(b->rnwg[1]), dim3(kFTX, kFTY + 1), 0, b->stream, I,
+                           (const int*)b->rwg[1], b->nr, k, it_step);
+    }
+  };
+  while (it_last - it + 1 >= 3) {
+    const bool s1 = it >= p.to_fourier && (it % p.sample_fourier) == 0;
+    const bool s2 = it + 1 >= p.to_fourier && ((it + 1) % p.sample_fourier) == 0;
+    const int slot = s1 ? b->own[0]->nsamp : -1;
+    rest(0, it);
+    for (int bl = 0; bl < 2; ++bl) {
+      const int n = (int)b->own2[bl].size();
+      for (int pm = 0; pm < 2; ++pm) {
+        if (!b->y2nwg[bl][pm]) continue;
+        const Batch2Inst<T>* I = (const Batch2Inst<T>*)b->y2[bl][pm] + (size_t)pp * n;
+        const dim3 g(b->y2nwg[bl][pm]);
+        const int* w = b->y2wg[bl][pm];
+        if (bl == 0 && pm == 0)
+          hipLaunchKernelGGL((k_yee2_batch<T, false, false>), g, dim3(kT2X, Yee2Shape<T>::R), 0, b->stream, I, w, n,
+                             slot);
+        else if (bl == 0)
+          hipLaunchKernelGGL((k_yee2_batch<T, false, true>), g, dim3(kT2X, Yee2Shape<T, true>::R), 0, b->stream, I,
+                             w, n, slot);
+        else if (pm == 0)
+          hipLaunchKernelGGL((k_yee2_batch<T, true, false>), g, dim3(kT2X, Yee2Shape<T>::R), 0, b->stream, I, w, n,
+                             slot);
+        else
+          hipLaunchKernelGGL((k_yee2_batch<T, true, true>), g, dim3(kT2X, Yee2Shape<T, true>::R), 0, b->stream, I,
+                             w, n, slot);
+      }
+    }
+    rest(1, it + 1);
+    if (hipGetLastError() != hipSuccess) return bfail(b, IRIS_EDEVICE, "batched two-step launch failed");
+    if (s1 && (rc = batch_sample2<T>(b, 0, 0, it + 1 - p.to_fourier))) return rc;
+    if (s2 && (rc = batch_sample2<T>(b, 1, pp, it + 2 - p.to_fourier))) return rc;
+    pp ^= 1;
+    it += 2;
+  }
+  for (iris_handle* h : b->own) {
+    if (pp) {
+      std::swap(h->fe, h->ke);
+      std::swap(h->fh, h->kh);
+      std::swap(h->d, h->kd);
+      std::swap(h->b, h->kb);
+    }
+    h->it = it - 1;
+  }
+  for (iris_handle* h : b->own)
+    if ((rc = run_steps<T>(h, it, it_last))) {
+      b->err = h->err;
+      return rc;
+    }
+  return 0;
+}
+
 template <typename T>
 int batch_create_impl(iris_batch* b) {
   // handles the two-step pass covers keep it (their own passes, as alone);
@@ -1592,7 +1856,7 @@ int batch_create_impl(iris_batch* b) {
       (rc = balloc(b, &b->ent, sizeof(BatchSamp<T>) * std::max(b->nent, 1))))
     return rc;
   BHIP_TRY(b, hipMemcpy(b->ent_start, start.data(), sizeof(int64_t) * start.size(), hipMemcpyHostToDevice));
-  return 0;
+  return batch_create2<T>(b);
 }
 
 template <typename T>
@@ -1604,16 +1868,15 @@ int batch_step_impl(iris_batch* b, int it_first, int it_last) {
   // tables are rewritten); the instances' launches then go to the batch stream
   BHIP_TRY(b, hipStreamSynchronize(b->stream));
   for (iris_handle* h : b->hs) BHIP_TRY(b, hipStreamSynchronize(h->stream));
-  if ((rc = batch_tables<T>(b))) return rc;
+  if ((rc = batch_tables<T>(b)) || (rc = batch_tables2<T>(b))) return rc;
   std::vector<hipStream_t> own(nall);
   for (int i = 0; i < nall; ++i) {
     own[i] = b->hs[i]->stream;
     b->hs[i]->stream = b->stream;
   }
   rc = 0;
-  // the two-step handles: their own passes, on the batch's stream
-  for (iris_handle* h : b->own)
-    if (!rc && (rc = run_steps<T>(h, it_first, it_last))) b->err = h->err;
+  // the two-step handles: batched pairs of steps, on the batch's stream
+  rc = batch_run2<T>(b, it_first, it_last);
   iris_handle* h0 = n ? b->one[0] : nullptr;
   int par = 0;
   for (int it = it_first; n && it <= it_last && !rc; ++it) {
@@ -2139,7 +2402,7 @@ int iris_batch_step(iris_batch* b, int32_t it_first, int32_t it_last) {
       return bfail(b, IRIS_ESTATE, "iris_batch_step: handle %zu is at step %d, not %d", i, h->it, it_first - 1);
     if (it_last > h->p.nt) return bfail(b, IRIS_ESTATE, "iris_batch_step: it_last=%d is past nt=%d of handle %zu",
                                         it_last, h->p.nt, i);
-    if (!h->two_step && h->nsamp != b->one[0]->nsamp)
+    if (h->nsamp != (h->two_step ? b->own[0] : b->one[0])->nsamp)
       return bfail(b, IRIS_ESTATE, "iris_batch_step: DFT rings out of step");
     if (h->ck_seg > 0)
       return bfail(b, IRIS_EUNSUP, "iris_batch_step: handle %zu keeps gradient snapshots (step it alone)", i);

@@ -64,6 +64,7 @@ template <typename T> struct BatchSamp {
   StepArgs<T> a[2];
   BoxGeom g;
   cplx<T>* samp;
+  const unsigned char* zmask;  // when set: skip the planes z with zmask[z] != 0
 };
 template <typename T>
 __global__ __launch_bounds__(256) void k_dft_sample_batch(const BatchSamp<T>* __restrict__ ent,
@@ -81,6 +82,7 @@ __global__ __launch_bounds__(256) void k_dft_sample_batch(const BatchSamp<T>* __
   const int64_t r = i - start[lo];
   int c, x, y, z;
   box_point(E.g, r, &c, &x, &y, &z);
+  if (E.zmask && E.zmask[z]) return;
   const int64_t idx = (int64_t)(x + a.xoff) + a.sx * y + a.sxy * z;
   E.samp[(int64_t)slot * E.g.ntot + r] = (c < 3) ? a.e[idx + c * a.cs] : a.h[idx + (c - 3) * a.cs];
 }

@@ -151,10 +151,11 @@ template <typename T> struct Yee2Samples {
 // other workgroups still read the step-n values for their halos).  With
 // the export below, D of step n+1 on plane z0 also goes to db: the
 // single-step pass under the chunk recomputes E of step n+2 there.
-template <typename T, bool PH, bool PML = false>
-__global__ __launch_bounds__((kT2X * Yee2Shape<T, PML>::R), (Yee2Shape<T, PML>::WAVES_PER_EU)) void k_yee2(
-    StepArgs<T> a, cplx<T>* __restrict__ eb, cplx<T>* __restrict__ hb, cplx<T>* __restrict__ db,
-    const int* __restrict__ chunks, TileMap tm, Yee2Samples<T> smp) {
+// The work of one workgroup: tile (bx, by) of chunk bz.
+template <typename T, bool PH, bool PML>
+__device__ __forceinline__ void yee2_tile(const StepArgs<T>& a, cplx<T>* __restrict__ eb, cplx<T>* __restrict__ hb,
+                                          cplx<T>* __restrict__ db, const int* __restrict__ chunks, int bx, int by,
+                                          int bz, const Yee2Samples<T>& smp, int slot) {
   using C = cplx<T>;
   constexpr int kT2R = Yee2Shape<T, PML>::R, kT2OY = Yee2Shape<T, PML>::OY;
   __shared__ C s1[2][3][kT2R][kT2X];  // E1 of planes z (z & 1) and z-1
@@ -163,8 +164,6 @@ __global__ __launch_bounds__((kT2X * Yee2Shape<T, PML>::R), (Yee2Shape<T, PML>::
   const int lane = threadIdx.x, wy = threadIdx.y;
   const int nx = a.nx, ny = a.ny;
   const int64_t cs = a.cs, sxy = a.sxy;
-  int bx, by, bz;
-  tile_of(tm, blockIdx.x, &bx, &by, &bz);
   const int craw = chunks[bz];
   const int z0 = craw & kChunkZMask;
   const int z1 = z0 + ((craw >> kChunkLenShift) & kChunkLenMask);
@@ -337,10 +336,11 @@ __global__ __launch_bounds__((kT2X * Yee2Shape<T, PML>::R), (Yee2Shape<T, PML>::
       }
       if (exp_above && z - 1 == z1) store3(hb, col + sxy * z1, cs, h1n);
       // DFT samples of step n+1
-      for (int b = 0; b < smp.nbox; ++b) {
+      for (int b = 0; slot >= 0 && b < smp.nbox; ++b) {
         if (!smp.samp[b]) continue;
-        if (z <= z1 && z >= z0) sample_cell(smp.g[b], smp.samp[b], xs, ys, z, 0, e1);
-        if (z - 1 >= z0 && z - 1 <= z1) sample_cell(smp.g[b], smp.samp[b], xs, ys, z - 1, 3, h1n);
+        C* const sp = smp.samp[b] + (int64_t)slot * smp.g[b].ntot;
+        if (z <= z1 && z >= z0) sample_cell(smp.g[b], sp, xs, ys, z, 0, e1);
+        if (z - 1 >= z0 && z - 1 <= z1) sample_cell(smp.g[b], sp, xs, ys, z - 1, 3, h1n);
       }
     }
     // rotate the pipeline
@@ -355,6 +355,45 @@ __global__ __launch_bounds__((kT2X * Yee2Shape<T, PML>::R), (Yee2Shape<T, PML>::
       }
     }
   }
+}
+
+template <typename T, bool PH, bool PML = false>
+__global__ __launch_bounds__((kT2X * Yee2Shape<T, PML>::R), (Yee2Shape<T, PML>::WAVES_PER_EU)) void k_yee2(
+    StepArgs<T> a, cplx<T>* __restrict__ eb, cplx<T>* __restrict__ hb, cplx<T>* __restrict__ db,
+    const int* __restrict__ chunks, TileMap tm, Yee2Samples<T> smp) {
+  int bx, by, bz;
+  tile_of(tm, blockIdx.x, &bx, &by, &bz);
+  yee2_tile<T, PH, PML>(a, eb, hb, db, chunks, bx, by, bz, smp, 0);
+}
+
+// Batched two-step pass: the chunks of many grids in one launch.  Entry i of
+// a table holds one grid's arguments for the pass from one of its two
+// buffer sets (the pair of steps alternates between them); workgroups
+// wg_start[i] .. wg_start[i+1]-1 are its tiles, a multiple of the XCD count
+// so that the tile order's XCD dealing holds within every grid (the padding
+// workgroups return at once).  The DFT samples of step n+1 go to ring slot
+// `slot` of the grid's boxes (no sample when slot < 0).
+template <typename T> struct Batch2Inst {
+  StepArgs<T> a;
+  cplx<T>* eb;
+  cplx<T>* hb;
+  cplx<T>* db;
+  const int* chunks;
+  TileMap tm;
+  Yee2Samples<T> smp;  // ring bases (slot 0)
+};
+
+template <typename T, bool PH, bool PML>
+__global__ __launch_bounds__((kT2X * Yee2Shape<T, PML>::R), (Yee2Shape<T, PML>::WAVES_PER_EU)) void k_yee2_batch(
+    const Batch2Inst<T>* __restrict__ inst, const int* __restrict__ wg_start, int ninst, int slot) {
+  const int blk = blockIdx.x;
+  const int i = batch_instance(wg_start, ninst, blk);
+  const Batch2Inst<T>& I = inst[i];
+  const int L = blk - wg_start[i];
+  if (L >= I.tm.nblk) return;  // padding (no barrier has been reached)
+  int bx, by, bz;
+  tile_of(I.tm, L, &bx, &by, &bz);
+  yee2_tile<T, PH, PML>(I.a, I.eb, I.hb, I.db, I.chunks, bx, by, bz, I.smp, slot);
 }
 
 // DFT samples of one step from a buffer that holds that step only on the

@@ -318,7 +318,7 @@ class Batch:
         v = np.zeros(5, dtype=np.int64)
         self.lib.iris_batch_info(self._b, _p(v))
         return dict(instances=int(v[0]), zchunk=int(v[1]), lean_workgroups=int(v[2]), general_workgroups=int(v[3]),
-                    own_two_step=int(v[4]))
+                    two_step=int(v[4]))
 
     def sync(self):
         for e in self.engines:

@@ -229,6 +229,7 @@ def test_adjoint_with_two_step_forward(tmp_path, monkeypatch):
     """On a grid where the forward run takes the two-step pass, the gradient
     equals the one of a single-step forward run bitwise."""
     print("[gpu-test] adjoint two-step forward", flush=True)
+    monkeypatch.setenv("IRIS_TWO_STEP_UPML", "1")   # the UPML slabs in the two-step pass too
     s = make_setup(tmp_path, "two_step")
     w = weights_for(s.params)
     for prec in (4, 8):
@@ -252,6 +253,7 @@ def test_two_step_adjoint_sweep(tmp_path, monkeypatch, precision):
     adjoint.  The result is bitwise that of single steps (snapshot interval 1,
     or no two-step pass at all) and matches the one-kernel-per-part adjoint."""
     print(f"[gpu-test] two-step adjoint sweep precision={precision}", flush=True)
+    monkeypatch.setenv("IRIS_TWO_STEP_UPML", "1")   # the UPML slabs in the two-step pass too
     s = make_setup(tmp_path, "two_step")
     p = s.params
     w = np.zeros((p.nof, 6))
@@ -286,6 +288,7 @@ def test_two_step_adjoint_with_bloch_phase(tmp_path, monkeypatch, precision):
     matches the one-kernel-per-part adjoint."""
     from pyiris.sweep import oblique
     print(f"[gpu-test] two-step adjoint bloch precision={precision}", flush=True)
+    monkeypatch.setenv("IRIS_TWO_STEP_UPML", "1")   # the UPML slabs in the two-step pass too
     s = oblique(make_setup(tmp_path, "two_step"), 30.0)
     p = s.params
     w = np.zeros((p.nof, 6))

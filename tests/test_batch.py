@@ -35,6 +35,13 @@ def mixed_setups(tmp_path):
              box=(1, 20, 1, 1, 14, 60)),
         oblique(make(tmp_path, "e", ixmax=33, iymax=9, izmax=70, pmlzi=9, pmlzf=62, hi=30, dh=7, dx=0, eps_bbox=1.7,
                      box=(3, 30, 2, 8, 13, 55)), 50.0),
+        # two-step grids with a Bloch phase (and one more shape without)
+        oblique(make(tmp_path, "f", ixmax=70, iymax=21, izmax=96, pmlzi=10, pmlzf=86, hi=44, dh=10, dx=8,
+                     eps_bbox=2.25, tfsf_o=20, box=(1, 70, 1, 21, 15, 80), nof=4), 20.0),
+        oblique(make(tmp_path, "g", ixmax=130, iymax=40, izmax=80, pmlzi=9, pmlzf=71, hi=36, dh=8, dx=20,
+                     eps_bbox=3.0, tfsf_o=18, box=(1, 130, 1, 40, 13, 66), nof=3), 40.0),
+        make(tmp_path, "h", ixmax=100, iymax=33, izmax=88, pmlzi=12, pmlzf=76, hi=40, dh=6, dx=10, eps_bbox=2.0,
+             tfsf_o=20, box=(1, 100, 1, 33, 15, 72), nof=3),
     ]
 
 
@@ -65,16 +72,27 @@ def test_grating_sweep_draws(tmp_path):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("upml", ["auto", "1"])
 @pytest.mark.parametrize("precision", [8, 4])
-def test_batch_is_bitwise_each_run_alone(tmp_path, precision):
-    print(f"[gpu-test] batch vs alone precision={precision}", flush=True)
+def test_batch_is_bitwise_each_run_alone(tmp_path, monkeypatch, precision, upml):
+    """Single-step grids, and two-step grids with and without a Bloch phase
+    (with the UPML slabs in the two-step pass where the engine would put them
+    there, or wherever it applies)."""
+    print(f"[gpu-test] batch vs alone precision={precision} upml={upml}", flush=True)
+    if upml != "auto":
+        monkeypatch.setenv("IRIS_TWO_STEP_UPML", upml)
     setups = mixed_setups(tmp_path)
     nt = setups[0].params.nt
     steps = [(1, 37), (38, 38), (39, nt)]
     ref = [alone(s, precision, steps) for s in setups]
     engines = [Engine(s, precision=precision) for s in setups]
+    two = [e.pass_info() for e in engines]
+    assert sum(t["two_step"] for t in two) >= 3
+    if upml == "1":
+        assert sum(t["upml_two_step_planes"] > 0 for t in two) >= 3
     try:
         with Batch(engines) as b:
+            assert b.info()["two_step"] == sum(t["two_step"] for t in two)
             for a, c in steps:
                 b.step(a, c)
         for s, e, r in zip(setups, engines, ref):

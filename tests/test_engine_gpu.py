@@ -379,7 +379,7 @@ TWO_STEP_CASES = {
 
 
 def _run_two_step(setup, precision, two_step, zchunk):
-    keys = {"IRIS_TWO_STEP": "1" if two_step else "0", "IRIS_ZCHUNK": str(zchunk)}
+    keys = {"IRIS_TWO_STEP": "1" if two_step else "0", "IRIS_TWO_STEP_UPML": "1", "IRIS_ZCHUNK": str(zchunk)}
     old = {k: os.environ.get(k) for k in keys}
     os.environ.update(keys)
     try:
@@ -393,9 +393,11 @@ def _run_two_step(setup, precision, two_step, zchunk):
     with eng:
         eng.profile(True)
         res = run_with_cadence(eng, setup.params, setup.params.nbox)
-        used = eng.profile_read()["yee2"][1]
+        prof = eng.profile_read()
+        used = prof["yee2"][1] + prof["yee2_pml"][1]
         p = setup.params
-        out = dict(res=res, e=eng.field(0), h=eng.field(1), used=used,
+        out = dict(res=res, e=eng.field(0), h=eng.field(1), used=used, used_pml=prof["yee2_pml"][1],
+                   upml_planes=eng.pass_info()["upml_two_step_planes"],
                    hp=eng.plane(Engine.PLANE_H_PREV + 1, 3, p.nz // 2, (1, p.nx), (1, p.ny)),
                    dft={(b, j): eng.dft(b, j) for b in range(1, p.nbox + 1) for j in (1, 2, 3)
                         if eng.dft_points(b, j) > 1})
@@ -413,6 +415,7 @@ def test_two_step_pass_is_bitwise_two_single_steps(tmp_path, name, precision, zc
     one = _run_two_step(s, precision, False, zchunk)
     two = _run_two_step(s, precision, True, zchunk)
     assert one["used"] == 0 and two["used"] > 0
+    assert (two["used_pml"] > 0) == (two["upml_planes"] > 0)
     for b in one["res"]:
         assert [it for it, _ in one["res"][b]] == [it for it, _ in two["res"][b]]
         for (_, j1), (_, j2) in zip(one["res"][b], two["res"][b]):
@@ -454,6 +457,7 @@ def test_two_step_pass_with_bloch_phase(tmp_path, name, precision):
     one = _run_two_step(s, precision, False, 5)
     two = _run_two_step(s, precision, True, 5)
     assert one["used"] == 0 and two["used"] > 0
+    assert (two["used_pml"] > 0) == (two["upml_planes"] > 0)
     tol = 1e-12 if precision == 8 else 1e-5
     for b in one["res"]:
         for (_, j1), (_, j2) in zip(one["res"][b], two["res"][b]):
