@@ -21,7 +21,12 @@ the plain forward run against the gradient of its summed transmission.  A
 third, "batch", times the batched Bloch sweep (run_batch) against the same
 runs one after another.
 
+Before the forward runs the engine times candidate layouts of its step
+passes (iris_autotune; all give the same results bit for bit) and keeps the
+fastest; --no-autotune keeps the default layout.
+
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-fp64] [--no-cpu] [--no-adjoint]
+                  [--no-batch] [--no-autotune]
   python bench.py --adjoint-only [--adjoint-steps N]
   python bench.py --batch-only [--batch-steps N]
 """
@@ -111,12 +116,20 @@ def algorithmic_bytes(p, precision, two_step_planes=0, upml_two_step_planes=0, u
             "step_two_pass": 18 * c * ncell + 12 * c * nxy * npml + r * nstr}
 
 
-def run_gpu(precision, steps, warmup):
+AUTOTUNE_STEPS = 8  # steps timed per candidate layout
+
+
+def run_gpu(precision, steps, warmup, autotune=True):
     from pyiris import Engine
     s = make_setup(precision)
     p = s.params
     log(f"precision {precision}: creating engine for {p.nx}x{p.ny}x{p.nz}, nof={p.nof}")
     eng = Engine(s, precision=precision)
+    layout = None
+    if autotune:
+        # layout of the step passes, chosen before the first step (untimed, like the set-up)
+        layout = eng.autotune(AUTOTUNE_STEPS)
+        log(f"precision {precision}: layout {layout}")
     nos = 1
 
     def advance(it_from, it_to, nos):
@@ -158,7 +171,7 @@ def run_gpu(precision, steps, warmup):
     dom = max(kern, key=lambda k: kern[k]["total_ms"])
     ms_step = 1e3 * secs / steps
     yee_ms = sum(kern[k]["total_ms"] for k in ("yee", "yee2", "yee2_pml", "yee_rest") if k in kern) / steps
-    return dict(gcells=gcells, ms_per_step=ms_step, secs=secs, kernels=kern, dominant=dom, passes=info,
+    return dict(gcells=gcells, ms_per_step=ms_step, secs=secs, kernels=kern, dominant=dom, passes=info, layout=layout,
                 yee_ms_per_step=yee_ms,
                 step_two_pass_frac=ab["step_two_pass"] / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS)
 
@@ -344,6 +357,7 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default layout of the step passes")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="run the gradient workload only and print its JSON object (profiling)")
@@ -365,13 +379,13 @@ def main():
     if a.batch_only:
         print(json.dumps(run_batch(4, BATCH_N, a.batch_steps)), flush=True)
         return
-    r32 = run_gpu(4, a.steps, a.warmup)
+    r32 = run_gpu(4, a.steps, a.warmup, not a.no_autotune)
     d32 = r32["kernels"][r32["dominant"]]
     log(f"fp32: {r32['gcells']:.2f} Gcell/s, {r32['ms_per_step']:.3f} ms/step, {r32['dominant']} "
         f"{d32.get('gbs', 0):.0f} GB/s")
     r64 = None
     if not a.no_fp64:
-        r64 = run_gpu(8, a.fp64_steps or max(a.steps // 2, 1), a.warmup)
+        r64 = run_gpu(8, a.fp64_steps or max(a.steps // 2, 1), a.warmup, not a.no_autotune)
         d64 = r64["kernels"][r64["dominant"]]
         log(f"fp64: {r64['gcells']:.2f} Gcell/s, {r64['ms_per_step']:.3f} ms/step, {r64['dominant']} "
             f"{d64.get('gbs', 0):.0f} GB/s")
@@ -410,6 +424,7 @@ def main():
         "kernels_fp32_launches": {n: v["launches"] for n, v in k.items()},
         "yee_ms_per_step_fp32": round(r32["yee_ms_per_step"], 4),
         "passes": r32["passes"],
+        "layout": r32["layout"],
         # per-step bytes of the two-pass algorithm (144 B/cell FP32 + UPML +
         # eps) over the measured step time, against 8 TB/s
         "two_pass_bytes_roofline_frac_fp32": round(r32["step_two_pass_frac"], 4),
@@ -418,7 +433,7 @@ def main():
         line["fp64"] = {"value": round(r64["gcells"], 4), "ms_per_step": round(r64["ms_per_step"], 4),
                         "dominant": r64["dominant"],
                         "dominant_gbs": round(r64["kernels"][r64["dominant"]].get("gbs", 0), 1),
-                        "yee_ms_per_step": round(r64["yee_ms_per_step"], 4),
+                        "yee_ms_per_step": round(r64["yee_ms_per_step"], 4), "layout": r64["layout"],
                         "two_pass_bytes_roofline_frac": round(r64["step_two_pass_frac"], 4),
                         "steps": a.fp64_steps or max(a.steps // 2, 1),
                         "kernels_us": {n: round(v["avg_us"], 2) for n, v in r64["kernels"].items()}}

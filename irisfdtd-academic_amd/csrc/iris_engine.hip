@@ -86,6 +86,19 @@ struct ListSet {
 
 }  // namespace
 
+// Layout of the step passes: chunk lengths, which planes take the two-step
+// pass and its UPML variant, the dealing of tiles to XCDs.  Every layout
+// gives the same results bit for bit, except that with a Bloch phase the
+// two-step pass (and its UPML variant) agree with single steps to rounding.
+struct PassConfig {
+  int zc = 0;         // planes per single-step z-chunk (0: enough chunks for ~4096 workgroups, at most 32 planes)
+  int two = -1;       // two-step pass: 0 off, 1 wherever it applies, -1 where its tiles waste fewer lanes
+  int upml = -1;      // its UPML variant: 0 off, 1 wherever it applies, -1 where its launch fills the CUs
+  int zc2 = 0;        // planes per two-step chunk (0: 2*zc)
+  int xcd_rows = 0;   // tile rows per XCD run of the single-step pass (0: identity order)
+  int xcd_rows2 = 4;  // the same for the two-step pass
+};
+
 struct iris_handle {
   iris_params p{};
   int prec = 8;
@@ -139,6 +152,8 @@ struct iris_handle {
   unsigned char* zmask = nullptr; // 1 on the planes of two-step chunks
   int tx2 = 0, ty2 = 0;
   int xcd_rows2 = 4;        // tile rows per XCD run of the two-step pass
+  int zc2 = 0;              // planes per two-step chunk
+  PassConfig pass;          // the layout the lists were built for
   void* eps_re = nullptr;
   void* eps_im = nullptr;
   void* pml = nullptr;
@@ -764,71 +779,60 @@ void box_geometry(const iris_params& p, int b, DeviceBox& B) {
   g.ntot = s;
 }
 
+// The layout the environment asks for (unset variables: the defaults).
+PassConfig pass_config_env(const iris_params& p) {
+  PassConfig c;
+  auto num = [](const char* name, int* v, int lo) {
+    const char* e = std::getenv(name);
+    if (e && *e) *v = std::max(lo, std::atoi(e));
+  };
+  num("IRIS_ZCHUNK", &c.zc, 0);
+  if (c.zc > p.nz) c.zc = p.nz;
+  num("IRIS_ZCHUNK2", &c.zc2, 0);
+  num("IRIS_XCD_ROWS", &c.xcd_rows, 0);
+  num("IRIS_XCD_ROWS2", &c.xcd_rows2, 0);
+  const char* t = std::getenv("IRIS_TWO_STEP");
+  if (t && (t[0] == '0' || t[0] == '1')) c.two = t[0] - '0';
+  const char* u = std::getenv("IRIS_TWO_STEP_UPML");
+  if (u && (u[0] == '0' || u[0] == '1')) c.upml = u[0] - '0';
+  return c;
+}
+
+void dfree(iris_handle* h, void** p) {
+  if (!*p) return;
+  auto it = std::find(h->allocs.begin(), h->allocs.end(), *p);
+  if (it != h->allocs.end()) h->allocs.erase(it);
+  (void)hipFree(*p);
+  *p = nullptr;
+}
+
+// (Re)builds the chunk lists of the step passes for layout c.  The field
+// state is not touched; the handle's stream is synchronised.
 template <typename T>
-int create_impl(iris_handle* h, const double* eps_inv, const double* fpmn, const double* fmmn, const double* fpmx,
-                const double* fmmx, const double* fw) {
+int build_passes(iris_handle* h, const PassConfig& c) {
   const iris_params& p = h->p;
-  h->csize = sizeof(cplx<T>);
-  // x pitch: ix=1 on a 128-byte boundary
-  const int align = (int)(128 / sizeof(cplx<T>));
-  h->xoff = align - 1;
-  h->sx = ((h->xoff + p.nx + 2 + align - 1) / align) * align;
-  h->sy = p.ny + 2;
-  h->sxy = h->sx * h->sy;
-  h->ncell = h->sxy * (p.nz + 2);
   int rc;
-  // component stride rounded to 256 elements keeps every component aligned
-  h->cs = ((h->ncell + 255) / 256) * 256;
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  h->pass = c;
+  h->xcd_rows = c.xcd_rows;
+  h->xcd_rows2 = c.xcd_rows2;
   const size_t fbytes = h->csize * 3 * (size_t)h->cs;
-  if ((rc = dalloc(h, &h->fe, fbytes)) || (rc = dalloc(h, &h->fh, fbytes))) return rc;
-  if (h->fused && ((rc = dalloc(h, &h->ge, fbytes)) || (rc = dalloc(h, &h->gh, fbytes)))) return rc;
-  h->ps0 = h->sxy * p.pmlzi;
-  h->ps1 = h->sxy * (p.nz - p.pmlzf + 1);
-  h->pb1 = 3 * h->ps0;
   const size_t abytes = h->csize * 3 * (size_t)(h->ps0 + h->ps1);
-  if ((rc = dalloc(h, &h->d, abytes)) || (rc = dalloc(h, &h->b, abytes))) return rc;
-  if (h->fused && ((rc = dalloc(h, &h->dn, abytes)) || (rc = dalloc(h, &h->bn, abytes)))) return rc;
-  if ((rc = upload_tables<T>(h, eps_inv, fpmn, fmmn, fpmx, fmmx))) return rc;
-  // DFT
-  if ((rc = dalloc(h, (void**)&h->fw, sizeof(double) * p.nof))) return rc;
-  HIP_TRY(h, hipMemcpyAsync(h->fw, fw, sizeof(double) * p.nof, hipMemcpyHostToDevice, h->stream));
-  if ((rc = dalloc(h, (void**)&h->ph, sizeof(double2) * (size_t)p.nof * kDftSlotsMax))) return rc;
-  if ((rc = dalloc(h, (void**)&h->flux, sizeof(double) * 6 * (size_t)p.nof))) return rc;
-  for (int b = 0; b < p.nbox; ++b) {
-    DeviceBox& B = h->box[b];
-    box_geometry(p, b, B);
-    if ((rc = dalloc(h, (void**)&B.emg, sizeof(double2) * (size_t)std::max<int64_t>(B.g.ntot, 1) * p.nof))) return rc;
-    if ((rc = dalloc(h, &B.samp, h->csize * (size_t)std::max<int64_t>(B.g.ntot, 1) * kDftSlotsMax))) return rc;
-  }
-  // launch shapes: x-rows of up to 64 (one wavefront), 256 threads per block
-  const int bx = std::min(64, pow2ceil(p.nx));
-  const int by = std::max(1, std::min(256 / bx, pow2ceil(p.ny)));
-  h->blk_yee = dim3(bx, by, 1);
-  const int gx = (p.nx + bx - 1) / bx, gy = (p.ny + by - 1) / by;
-  const int64_t nbxy = (int64_t)gx * gy;
-  int zc = (int)std::min<int64_t>(64, std::max<int64_t>(1, nbxy * p.nz / 4096));
-  h->zchunk = zc;
-  h->grd_yee = dim3(gx, gy, (p.nz + zc - 1) / zc);
   {
     const int fgx = (p.nx + kFTX - 1) / kFTX, fgy = (p.ny + kFTY - 1) / kFTY;
     int fzc = (int)std::min<int64_t>(32, std::max<int64_t>(1, (int64_t)fgx * fgy * p.nz / 4096));
-    // planes per z-chunk can be set for testing the chunk logic on small grids
-    const char* zenv = std::getenv("IRIS_ZCHUNK");
-    if (zenv && std::atoi(zenv) > 0) fzc = std::min(std::atoi(zenv), p.nz);
-    h->blk_fused = dim3(kFTX, kFTY + 1, 1);
+    if (c.zc > 0) fzc = std::min(c.zc, p.nz);
     h->grd_fused = dim3(fgx, fgy, (p.nz + fzc - 1) / fzc);
     if (h->fused) h->zchunk = fzc;
   }
-  const int tbx = std::min(64, pow2ceil(p.nx + 1));
-  const int tby = std::max(1, std::min(256 / tbx, pow2ceil(p.ny + 1)));
-  h->blk_tf = dim3(tbx, tby, 1);
-  h->grd_tf = dim3((p.nx + 1 + tbx - 1) / tbx, (p.ny + 1 + tby - 1) / tby, 1);
-  h->dtcq2 = ((p.dt * p.c0) / p.q) * ((p.dt * p.c0) / p.q);
-  {
-    const TfsfConst t = tfsf_const(h);
-    if ((rc = dalloc(h, (void**)&h->tfc, sizeof(TfsfConst)))) return rc;
-    HIP_TRY(h, hipMemcpyAsync(h->tfc, &t, sizeof t, hipMemcpyHostToDevice, h->stream));
-  }
+  // no two-step pass until the lists below say otherwise
+  h->two_step = false;
+  h->n2 = h->n2p = 0;
+  h->planes2 = h->planes2p = h->upml2p = 0;
+  h->rest_groups.clear();
+  h->nrest_lean = h->nrest_gen = 0;
+  h->zc2 = std::min(c.zc2 > 0 ? c.zc2 : 2 * h->zchunk, kChunkLenMask + 1);
+  const int zc2 = h->zc2;
   {
     // fused z-chunks: a chunk is bulk when every plane it reads or writes,
     // z0-1 .. z1+1, lies strictly between the UPML slabs, outside the
@@ -836,6 +840,7 @@ int create_impl(iris_handle* h, const double* eps_inv, const double* fpmn, const
     std::vector<int> bulk, gen;
     const int zc = h->zchunk;
     single_lists(p, zc, bulk, gen);
+    dfree(h, (void**)&h->chunks);
     auto is_bulk_plane = [&](int z) {
       if (z <= p.pmlzi || z >= p.pmlzf) return false;
       if (z >= p.hi && z <= p.hf) return false;
@@ -857,16 +862,15 @@ int create_impl(iris_handle* h, const double* eps_inv, const double* fpmn, const
     // variant.  Runs of planes of one kind are cut into near-equal chunks of
     // up to 2*zc planes; the remaining planes take single
     // steps in chunks of up to zc planes with their own lengths.
-    // IRIS_TWO_STEP=0: off, =1: on wherever it applies; unset: on when its
-    // tiles waste fewer lanes than the single-step pass's, as estimated by
+    // c.two = 0: off, 1: on wherever it applies, -1: on when its tiles
+    // waste fewer lanes than the single-step pass's, as estimated by
     // the lanes per cell-step of both (64 x R footprint lanes per 61 x (R-3)
     // cells and two steps, against 64 x 9 lanes per 64 x 8 cells and one
     // step).  Measured on MI355X with 768-plane grids, the two-step pass is
     // slower only where this ratio is above 1 (one 64-wide row of tiles
     // against two).
-    const char* env = std::getenv("IRIS_TWO_STEP");
-    bool want = !(env && env[0] == '0');
-    if (!(env && env[0] == '1')) {
+    bool want = c.two != 0;
+    if (c.two < 0) {
       constexpr int R = Yee2Shape<T>::R, OY = Yee2Shape<T>::OY;
       const double two = (double)((p.nx + kT2OX - 1) / kT2OX) * ((p.ny + OY - 1) / OY) * R / 2.0;
       const double one = (double)((p.nx + kFTX - 1) / kFTX) * ((p.ny + kFTY - 1) / kFTY) * (kFTY + 1);
@@ -892,11 +896,9 @@ int create_impl(iris_handle* h, const double* eps_inv, const double* fpmn, const
     // workgroup: its slabs are a few dozen planes, and on small grids the
     // few workgroups of its launch took longer than the single steps they
     // replace (measured on 64 gratings of 32..128 x 32..128 x 768 cells).
-    // IRIS_TWO_STEP_UPML=0: never, =1: wherever it applies.
-    const char* vpml = std::getenv("IRIS_TWO_STEP_UPML");
-    bool upml_two = !(vpml && vpml[0] == '0');
-    if (upml_two && !(vpml && vpml[0] == '1')) {
-      const int zc2 = std::min(2 * zc, kChunkLenMask + 1);
+    // c.upml = 0: never, 1: wherever it applies, -1: by that rule.
+    bool upml_two = c.upml != 0;
+    if (c.upml < 0) {
       int64_t nch = 0;
       for (int z = 1; z <= p.nz; ++z) {
         if (kind[z] != 2) continue;
@@ -925,9 +927,6 @@ int create_impl(iris_handle* h, const double* eps_inv, const double* fpmn, const
       z = e;
     }
     if (nout > 0) {
-      int zc2 = std::min(2 * zc, kChunkLenMask + 1);
-      const char* z2env = std::getenv("IRIS_ZCHUNK2");  // planes per two-step chunk (tuning)
-      if (z2env && std::atoi(z2env) > 0) zc2 = std::min(std::atoi(z2env), kChunkLenMask + 1);
       std::vector<int> c2, c2p;
       std::map<int, std::pair<std::vector<int>, std::vector<int>>> byl;  // length -> (lean, general)
       std::vector<unsigned char> zm((size_t)p.nz + 2, 0);
@@ -987,13 +986,17 @@ int create_impl(iris_handle* h, const double* eps_inv, const double* fpmn, const
       h->tx2 = (p.nx + kT2OX - 1) / kT2OX;
       h->ty2 = (p.ny + Yee2Shape<T>::OY - 1) / Yee2Shape<T>::OY;
       h->ty2p = (p.ny + Yee2Shape<T, true>::OY - 1) / Yee2Shape<T, true>::OY;
+      dfree(h, (void**)&h->chunks2);
+      dfree(h, (void**)&h->chunks2p);
+      dfree(h, (void**)&h->chunks_rest);
+      dfree(h, (void**)&h->zmask);
       if ((rc = dalloc(h, (void**)&h->chunks2, sizeof(int) * std::max<size_t>(c2.size(), 1))) ||
           (rc = dalloc(h, (void**)&h->chunks2p, sizeof(int) * std::max<size_t>(c2p.size(), 1))) ||
           (rc = dalloc(h, (void**)&h->chunks_rest, sizeof(int) * std::max<size_t>(rest.size(), 1))) ||
           (rc = dalloc(h, (void**)&h->zmask, zm.size())))
         return rc;
-      if ((rc = dalloc(h, &h->ke, fbytes)) || (rc = dalloc(h, &h->kh, fbytes))) return rc;
-      if ((rc = dalloc(h, &h->kd, abytes)) || (rc = dalloc(h, &h->kb, abytes))) return rc;
+      if (!h->ke && ((rc = dalloc(h, &h->ke, fbytes)) || (rc = dalloc(h, &h->kh, fbytes)))) return rc;
+      if (!h->kd && ((rc = dalloc(h, &h->kd, abytes)) || (rc = dalloc(h, &h->kb, abytes)))) return rc;
       if (!c2.empty())
         HIP_TRY(h, hipMemcpyAsync(h->chunks2, c2.data(), sizeof(int) * c2.size(), hipMemcpyHostToDevice, h->stream));
       if (!c2p.empty())
@@ -1006,6 +1009,67 @@ int create_impl(iris_handle* h, const double* eps_inv, const double* fpmn, const
       HIP_TRY(h, hipStreamSynchronize(h->stream));
     }
   }
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+template <typename T>
+int create_impl(iris_handle* h, const double* eps_inv, const double* fpmn, const double* fmmn, const double* fpmx,
+                const double* fmmx, const double* fw) {
+  const iris_params& p = h->p;
+  h->csize = sizeof(cplx<T>);
+  // x pitch: ix=1 on a 128-byte boundary
+  const int align = (int)(128 / sizeof(cplx<T>));
+  h->xoff = align - 1;
+  h->sx = ((h->xoff + p.nx + 2 + align - 1) / align) * align;
+  h->sy = p.ny + 2;
+  h->sxy = h->sx * h->sy;
+  h->ncell = h->sxy * (p.nz + 2);
+  int rc;
+  // component stride rounded to 256 elements keeps every component aligned
+  h->cs = ((h->ncell + 255) / 256) * 256;
+  const size_t fbytes = h->csize * 3 * (size_t)h->cs;
+  if ((rc = dalloc(h, &h->fe, fbytes)) || (rc = dalloc(h, &h->fh, fbytes))) return rc;
+  if (h->fused && ((rc = dalloc(h, &h->ge, fbytes)) || (rc = dalloc(h, &h->gh, fbytes)))) return rc;
+  h->ps0 = h->sxy * p.pmlzi;
+  h->ps1 = h->sxy * (p.nz - p.pmlzf + 1);
+  h->pb1 = 3 * h->ps0;
+  const size_t abytes = h->csize * 3 * (size_t)(h->ps0 + h->ps1);
+  if ((rc = dalloc(h, &h->d, abytes)) || (rc = dalloc(h, &h->b, abytes))) return rc;
+  if (h->fused && ((rc = dalloc(h, &h->dn, abytes)) || (rc = dalloc(h, &h->bn, abytes)))) return rc;
+  if ((rc = upload_tables<T>(h, eps_inv, fpmn, fmmn, fpmx, fmmx))) return rc;
+  // DFT
+  if ((rc = dalloc(h, (void**)&h->fw, sizeof(double) * p.nof))) return rc;
+  HIP_TRY(h, hipMemcpyAsync(h->fw, fw, sizeof(double) * p.nof, hipMemcpyHostToDevice, h->stream));
+  if ((rc = dalloc(h, (void**)&h->ph, sizeof(double2) * (size_t)p.nof * kDftSlotsMax))) return rc;
+  if ((rc = dalloc(h, (void**)&h->flux, sizeof(double) * 6 * (size_t)p.nof))) return rc;
+  for (int b = 0; b < p.nbox; ++b) {
+    DeviceBox& B = h->box[b];
+    box_geometry(p, b, B);
+    if ((rc = dalloc(h, (void**)&B.emg, sizeof(double2) * (size_t)std::max<int64_t>(B.g.ntot, 1) * p.nof))) return rc;
+    if ((rc = dalloc(h, &B.samp, h->csize * (size_t)std::max<int64_t>(B.g.ntot, 1) * kDftSlotsMax))) return rc;
+  }
+  // launch shapes: x-rows of up to 64 (one wavefront), 256 threads per block
+  const int bx = std::min(64, pow2ceil(p.nx));
+  const int by = std::max(1, std::min(256 / bx, pow2ceil(p.ny)));
+  h->blk_yee = dim3(bx, by, 1);
+  const int gx = (p.nx + bx - 1) / bx, gy = (p.ny + by - 1) / by;
+  const int64_t nbxy = (int64_t)gx * gy;
+  int zc = (int)std::min<int64_t>(64, std::max<int64_t>(1, nbxy * p.nz / 4096));
+  h->zchunk = zc;
+  h->grd_yee = dim3(gx, gy, (p.nz + zc - 1) / zc);
+  h->blk_fused = dim3(kFTX, kFTY + 1, 1);
+  const int tbx = std::min(64, pow2ceil(p.nx + 1));
+  const int tby = std::max(1, std::min(256 / tbx, pow2ceil(p.ny + 1)));
+  h->blk_tf = dim3(tbx, tby, 1);
+  h->grd_tf = dim3((p.nx + 1 + tbx - 1) / tbx, (p.ny + 1 + tby - 1) / tby, 1);
+  h->dtcq2 = ((p.dt * p.c0) / p.q) * ((p.dt * p.c0) / p.q);
+  {
+    const TfsfConst t = tfsf_const(h);
+    if ((rc = dalloc(h, (void**)&h->tfc, sizeof(TfsfConst)))) return rc;
+    HIP_TRY(h, hipMemcpyAsync(h->tfc, &t, sizeof t, hipMemcpyHostToDevice, h->stream));
+  }
+  if ((rc = build_passes<T>(h, pass_config_env(p)))) return rc;
   HIP_TRY(h, hipStreamSynchronize(h->stream));
   return 0;
 }
@@ -1316,9 +1380,7 @@ int build_lists(iris_handle* h, ListSet& out, int nobox, int face_on) {
     ndbl += ok;
   }
   if (ndbl == 0) return 0;
-  int zc2 = std::min(2 * zc, kChunkLenMask + 1);
-  const char* z2env = std::getenv("IRIS_ZCHUNK2");
-  if (z2env && std::atoi(z2env) > 0) zc2 = std::min(std::atoi(z2env), kChunkLenMask + 1);
+  const int zc2 = h->zc2;
   std::vector<int> c2, rlean, rgen;
   std::vector<unsigned char> zm((size_t)p.nz + 2, 0);
   const int nch = (int)dbl.size();
@@ -1510,6 +1572,165 @@ int adjoint_impl(iris_handle* h, int nobox, const double* weights, double* grad_
   h->it = nt;
   HIP_TRY(h, hipMemcpyAsync(grad_out, h->grad, gbytes, hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+// ---- layout tuning ---------------------------------------------------------------
+
+// Microseconds per step of the current layout: `steps` steps (pairs where
+// the two-step pass applies) after one warm-up pair, without DFT samples,
+// timed with events on the handle's stream.  Changes the field state.
+template <typename T>
+int trial_steps(iris_handle* h, int steps, double* us) {
+  PassLists L;
+  L.chunks2 = h->chunks2;
+  L.n2 = h->n2;
+  L.rest = h->chunks_rest;
+  L.nlean = h->nrest_lean;
+  L.ngen = h->nrest_gen;
+  L.zmask = h->zmask;
+  L.chunks2p = h->chunks2p;
+  L.n2p = h->n2p;
+  L.groups = h->rest_groups.data();
+  L.ngroups = (int)h->rest_groups.size();
+  auto run = [&](int n) -> int {
+    for (int it = 1; it <= n;) {
+      int rc;
+      if (h->two_step && n - it + 1 >= 2) {
+        if ((rc = launch_double<T>(h, it, L, false))) return rc;
+        it += 2;
+      } else {
+        if ((rc = launch_step<T>(h, it))) return rc;
+        ++it;
+      }
+    }
+    return 0;
+  };
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIP_TRY(h, hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return fail(h, IRIS_EDEVICE, "hipEventCreate failed");
+  }
+  int rc = run(2);
+  if (!rc) {
+    (void)hipEventRecord(e0, h->stream);
+    rc = run(steps);
+    (void)hipEventRecord(e1, h->stream);
+  }
+  float ms = 0.f;
+  if (!rc && hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess)
+    *us = 1000.0 * ms / steps;
+  else if (!rc)
+    rc = fail(h, IRIS_EDEVICE, "timing the layout failed");
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
+
+// Picks the fastest layout by timing candidates one parameter at a time
+// (coordinate descent from the current layout; a change is kept when it is
+// at least 2% faster).  Only layouts that give the same results bit for bit
+// are compared: with a Bloch phase the choice between the two-step pass and
+// single steps (and the UPML variant) stays as it is.  The field state is
+// saved before and restored after; the handle stays at step 0.
+template <typename T>
+int autotune_impl(iris_handle* h, int steps, int64_t info[8]) {
+  const size_t bytes = state_bytes(h);
+  void* snap = nullptr;
+  if (hipMalloc(&snap, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(h, IRIS_ENOMEM, "iris_autotune: no device memory for a copy of the state (%zu bytes)", bytes);
+  }
+  const bool prof = h->prof;
+  h->prof = false;
+  int rc = state_copy(h, snap, true);
+  int tried = 0;
+  PassConfig best = h->pass;
+  double tbest = 0.0;
+  auto measure = [&](const PassConfig& c, double* us) -> int {
+    int r = build_passes<T>(h, c);
+    if (!r) r = trial_steps<T>(h, steps, us);
+    ++tried;
+    return r;
+  };
+  if (!rc) rc = measure(best, &tbest);
+  auto consider = [&](PassConfig c) {
+    if (rc) return;
+    double t = 0.0;
+    rc = measure(c, &t);
+    if (!rc && t < 0.98 * tbest) {
+      best = c;
+      tbest = t;
+    }
+  };
+  const iris_params& p = h->p;
+  const int zc0 = h->zchunk;
+  if (!rc && !h->bloch) {
+    for (int v : {0, 1}) {
+      PassConfig c = best;
+      c.two = v;
+      if (c.two != best.two && !(best.two < 0 && (v == 1) == h->two_step)) consider(c);
+    }
+    if (!rc) rc = build_passes<T>(h, best);
+    if (!rc && h->two_step)
+      for (int v : {0, 1}) {
+        PassConfig c = best;
+        c.upml = v;
+        if (c.upml != best.upml && !(best.upml < 0 && (v == 1) == (h->upml2p > 0))) consider(c);
+      }
+  }
+  if (!rc) rc = build_passes<T>(h, best);
+  const bool two = h->two_step;
+  for (int f : {1, 2}) {  // single-step chunks: shorter, longer
+    PassConfig c = best;
+    c.zc = f == 1 ? zc0 / 2 : std::min(2 * zc0, p.nz);
+    if (c.zc >= 1 && c.zc != zc0) consider(c);
+  }
+  for (int v : {2, 4}) {
+    PassConfig c = best;
+    c.xcd_rows = v;
+    if (v != best.xcd_rows) consider(c);
+  }
+  if (two) {
+    if (!rc) rc = build_passes<T>(h, best);
+    const int zc2 = h->zc2;
+    for (int v : {zc2 / 2, 2 * zc2}) {
+      PassConfig c = best;
+      c.zc2 = std::min(v, kChunkLenMask + 1);
+      if (c.zc2 >= 2 && c.zc2 != zc2) consider(c);
+    }
+    for (int v : {0, 2, 8}) {
+      PassConfig c = best;
+      c.xcd_rows2 = v;
+      if (v != best.xcd_rows2) consider(c);
+    }
+  }
+  const int rc_run = rc;
+  rc = build_passes<T>(h, best);
+  if (!rc) rc = state_copy(h, snap, false);
+  // the other buffers as a new handle has them (the previous step's fields
+  // read at step 0 are zero)
+  for (void* q : {h->ge, h->gh})
+    if (!rc && q && hipMemsetAsync(q, 0, field_bytes(h), h->stream) != hipSuccess)
+      rc = fail(h, IRIS_EDEVICE, "hipMemsetAsync failed");
+  for (void* q : {h->dn, h->bn})
+    if (!rc && q && hipMemsetAsync(q, 0, aux_bytes(h), h->stream) != hipSuccess)
+      rc = fail(h, IRIS_EDEVICE, "hipMemsetAsync failed");
+  if (!rc) HIP_TRY(h, hipStreamSynchronize(h->stream));
+  (void)hipFree(snap);
+  h->prof = prof;
+  h->it = 0;
+  if (rc_run) return rc_run;
+  if (rc) return rc;
+  info[0] = h->two_step ? 1 : 0;
+  info[1] = h->two_step ? h->upml2p : 0;
+  info[2] = h->zchunk;
+  info[3] = h->two_step ? h->zc2 : 0;
+  info[4] = h->xcd_rows;
+  info[5] = h->xcd_rows2;
+  info[6] = tried;
+  info[7] = (int64_t)(tbest * 1000.0 + 0.5);
   return 0;
 }
 
@@ -1988,15 +2209,11 @@ int iris_create(const iris_params* p, const double* eps_inv, const double* fpzmn
   {
     const char* env = std::getenv("IRIS_FUSED");
     h->fused = !(env && env[0] == '0');
-    const char* xenv = std::getenv("IRIS_XCD_ROWS");
-    if (xenv && *xenv) h->xcd_rows = std::max(0, std::atoi(xenv));
     const char* aenv = std::getenv("IRIS_ADJ_FUSED");
     h->adj_fused = !(aenv && aenv[0] == '0');
     const char* tyenv = std::getenv("IRIS_ADJ_TY");
     if (tyenv && (std::atoi(tyenv) == 7 || std::atoi(tyenv) == 8 || std::atoi(tyenv) == 15))
       h->adj_ty = std::atoi(tyenv);
-    const char* x2env = std::getenv("IRIS_XCD_ROWS2");
-    if (x2env && *x2env) h->xcd_rows2 = std::max(0, std::atoi(x2env));
   }
   (void)hipGetDevice(&h->device);
   e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
@@ -2320,6 +2537,16 @@ int iris_pass_info(const iris_handle* h, int64_t info[6]) {
   info[4] = h->two_step ? h->planes2p : 0;
   info[5] = h->two_step ? h->upml2p : 0;
   return 0;
+}
+
+int iris_autotune(iris_handle* h, int32_t steps, int64_t info[8]) {
+  if (!h || !info) return fail(h, IRIS_EINVAL, "handle or info is NULL");
+  if (h->it != 0) return fail(h, IRIS_ESTATE, "iris_autotune: call it before the first step (the handle is at step %d)", h->it);
+  if (h->ck_seg > 0) return fail(h, IRIS_ESTATE, "iris_autotune: call it before iris_checkpoint");
+  if (!h->fused) return fail(h, IRIS_EUNSUP, "iris_autotune: the in-place passes have no layout to tune");
+  if (steps < 2 || steps > 1000) return fail(h, IRIS_EINVAL, "iris_autotune: steps must be in 2..1000 (got %d)", steps);
+  (void)hipSetDevice(h->device);
+  return h->prec == 8 ? autotune_impl<double>(h, steps, info) : autotune_impl<float>(h, steps, info);
 }
 
 int iris_current_step(const iris_handle* h, int32_t* it) {

@@ -93,6 +93,13 @@ module iris_c_api
       integer(c_int64_t), value :: record_bytes
     end function
 
+    integer(c_int) function iris_autotune(h, steps, info) bind(C, name="iris_autotune")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: steps
+      type(c_ptr), value :: info
+    end function
+
     integer(c_int) function iris_checkpoint_info(h, info) bind(C, name="iris_checkpoint_info")
       import :: c_int, c_ptr
       type(c_ptr), value :: h

@@ -95,6 +95,8 @@ def load_library(path=None):
     lib.iris_diagnostics.restype = ct.c_int
     lib.iris_pass_info.argtypes = [H, ct.c_void_p]
     lib.iris_pass_info.restype = ct.c_int
+    lib.iris_autotune.argtypes = [H, ct.c_int32, ct.c_void_p]
+    lib.iris_autotune.restype = ct.c_int
     lib.iris_checkpoint.argtypes = [H, ct.c_int32, ct.c_int64]
     lib.iris_checkpoint.restype = ct.c_int
     lib.iris_checkpoint_info.argtypes = [H, ct.c_void_p]
@@ -130,8 +132,8 @@ def load_library(path=None):
 EXPORTED = ["iris_create", "iris_step", "iris_sync", "iris_flux", "iris_dft_points", "iris_get_dft",
             "iris_get_field", "iris_set_field", "iris_get_plane", "iris_diagnostics", "iris_current_step",
             "iris_destroy", "iris_last_error", "iris_version", "iris_profile", "iris_profile_read",
-            "iris_pass_info", "iris_checkpoint", "iris_checkpoint_info", "iris_adjoint", "iris_batch_create",
-            "iris_batch_step", "iris_batch_info", "iris_batch_destroy", "iris_batch_last_error"]
+            "iris_pass_info", "iris_autotune", "iris_checkpoint", "iris_checkpoint_info", "iris_adjoint",
+            "iris_batch_create", "iris_batch_step", "iris_batch_info", "iris_batch_destroy", "iris_batch_last_error"]
 
 KERNELS = ["epass", "hpass", "tfsf", "dft_sample", "dft_accum", "flux", "yee", "yee2", "yee_rest", "adjoint",
            "checkpoint", "yee2_pml"]
@@ -233,6 +235,16 @@ class Engine:
         self._check(self.lib.iris_pass_info(self._h, _p(v)), "iris_pass_info")
         return dict(zchunk=int(v[0]), two_step_planes=int(v[1]), two_step_chunks=int(v[2]), two_step=bool(v[3]),
                     upml_two_step_planes=int(v[4]), upml_planes_in_two_step=int(v[5]))
+
+    def autotune(self, steps=8):
+        """Time candidate layouts of the step passes for `steps` steps each and
+        keep the fastest (before the first step; results unchanged bit for
+        bit).  Returns {two_step, upml_planes, zchunk, zchunk2, xcd_rows,
+        xcd_rows2, layouts, ns_per_step}."""
+        v = np.zeros(8, dtype=np.int64)
+        self._check(self.lib.iris_autotune(self._h, int(steps), _p(v)), "iris_autotune")
+        return dict(two_step=bool(v[0]), upml_planes=int(v[1]), zchunk=int(v[2]), zchunk2=int(v[3]),
+                    xcd_rows=int(v[4]), xcd_rows2=int(v[5]), layouts=int(v[6]), ns_per_step=int(v[7]))
 
     def checkpoint(self, segment, record_bytes=0):
         """Prepare the coming forward run for adjoint(): H of its last steps
