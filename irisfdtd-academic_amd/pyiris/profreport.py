@@ -1,11 +1,10 @@
 #!/usr/bin/env python3
 """Summaries of rocprofv3 output for the engine's kernels.
 
-  profreport.py stats KERNEL_STATS_CSV [--command CMD]
+  profreport.py stats KERNEL_STATS_CSV [--command CMD] [--steps-fp32 N] [--steps-fp64 N]
       Markdown table of a --kernel-trace --stats run (per kernel: calls,
-      average and total time) and the per-step time of each kernel class per
-      field precision; the step count of a precision is the number of launches
-      of its general fused-pass instantiation (one per step).
+      average and total time) and, for the precisions whose step count is
+      given, the time per step of each kernel class.
 
   profreport.py pmc FETCH_DIR WRITE_DIR --steps N --out-dir DIR [--command-fetch CMD --command-write CMD]
       HBM bytes per step of each kernel class from two counter runs of
@@ -16,12 +15,11 @@
       summary into DIR.
 
 Kernel classes: "yee2" = the two-step pass over the plain z-chunks (one
-launch per two steps); "yee" = both instantiations of the single-step fused
-E+H pass (the lean one and the general one), "dft_fold",
+launch per two steps); "yee2_pml" = its UPML variant; "yee" = both
+instantiations of the single-step fused E+H pass (the lean one and the
+general one); "yee2_batch" / "yee_batch" = their batched forms; "dft_fold",
 "dft_sample", "epass", "hpass", "tfsf", "flux", "adjoint" (the gradient's
 own kernels), "record" (its H records), "copy" (the calibration).
---no-per-step leaves out the per-step table (for runs whose step count is
-not the general pass's launch count, such as a gradient).
 """
 import argparse
 import csv
@@ -36,8 +34,11 @@ sys.path.insert(0, HERE)
 from pmcprobe import COPY_BYTES  # noqa: E402
 
 CLASSES = [
-    ("yee2", r"k_yee2"),
-    ("yee", r"k_yee_fused"),
+    ("yee2_batch", r"k_yee2_batch"),
+    ("yee2_pml", r"k_yee2<\w+, \w+, true>"),
+    ("yee2", r"k_yee2<"),
+    ("yee_batch", r"k_yee_fused_batch"),
+    ("yee", r"k_yee_fused<"),
     ("epass", r"k_epass"),
     ("hpass", r"k_hpass"),
     ("tfsf", r"k_tfsf"),
@@ -81,7 +82,7 @@ def stats(args):
         out.append(f"Command: `{args.command}`\n")
     out.append("| kernel | calls | avg (us) | total (ms) |")
     out.append("|---|---:|---:|---:|")
-    per_class, nsteps = {}, {}
+    per_class = {}
     for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"])):
         calls = int(r["Calls"])
         avg = float(r["AverageNs"]) / 1e3
@@ -91,11 +92,11 @@ def stats(args):
         if cls and prec:
             key = (cls, prec)
             per_class[key] = per_class.get(key, 0.0) + tot
-            if cls == "yee" and re.search(r"k_yee_fused<(float|double), true>", r["Name"]):
-                nsteps[prec] = calls
-    if nsteps and not args.no_per_step:
+    nsteps = {4: args.steps_fp32, 8: args.steps_fp64}
+    nsteps = {k: v for k, v in nsteps.items() if v}
+    if nsteps:
         out.append("")
-        out.append("Per step (steps = launches of the general fused-pass instantiation; warm-up included):")
+        out.append("Per step (all timed and warm-up steps of the run):")
         out.append("")
         out.append("| class | fields | steps | ms per step |")
         out.append("|---|---|---:|---:|")
@@ -179,7 +180,8 @@ def main():
     a = sub.add_parser("stats")
     a.add_argument("csv")
     a.add_argument("--command", default="")
-    a.add_argument("--no-per-step", action="store_true")
+    a.add_argument("--steps-fp32", type=int, default=0)
+    a.add_argument("--steps-fp64", type=int, default=0)
     b = sub.add_parser("pmc")
     b.add_argument("fetch")
     b.add_argument("write")
