@@ -152,8 +152,21 @@ struct iris_handle {
   unsigned char* zmask = nullptr; // 1 on the planes of two-step chunks
   int tx2 = 0, ty2 = 0;
   int xcd_rows2 = 4;        // tile rows per XCD run of the two-step pass
-  int zc2 = 0;              // planes per two-step chunk
+  int zc2 = 0;              // planes per plain two-step chunk
+  int ncu = 256;            // compute units of the device
   PassConfig pass;          // the layout the lists were built for
+  // single steps of the two-step layout's other chunks when they come in
+  // several lengths: one batched launch per class (lean, general) over the
+  // length groups, with per-group arguments from a table per buffer
+  // assignment (the buffers rotate between a few assignments)
+  int* rest_wg[2] = {nullptr, nullptr};  // workgroup prefixes over the groups
+  int rest_nwg[2] = {0, 0};
+  struct RestTab {
+    void* key[12];
+    void* dev;
+    std::vector<char> host;  // kept while the upload may still read it
+  };
+  std::vector<RestTab> rest_tabs;
   void* eps_re = nullptr;
   void* eps_im = nullptr;
   void* pml = nullptr;
@@ -436,6 +449,45 @@ int dft_sample(iris_handle* h, int n);
 template <typename T>
 int dft_flush(iris_handle* h);
 
+// Argument table of the handle's single-step length groups for its current
+// buffers (entry g: a[0] = step n+1 from the current into the second
+// buffers, a[1] = step n+2 from the second into the third), made on first
+// use of a buffer assignment.
+template <typename T>
+int rest_table(iris_handle* h, const BatchInst<T>** out) {
+  void* key[12] = {h->fe, h->fh, h->ge, h->gh, h->ke, h->kh, h->d, h->b, h->dn, h->bn, h->kd, h->kb};
+  for (const auto& t : h->rest_tabs)
+    if (std::equal(key, key + 12, t.key)) {
+      *out = (const BatchInst<T>*)t.dev;
+      return 0;
+    }
+  const int n = (int)h->rest_groups.size();
+  iris_handle::RestTab t;
+  std::copy(key, key + 12, t.key);
+  t.host.assign(sizeof(BatchInst<T>) * n, 0);
+  BatchInst<T>* I = (BatchInst<T>*)t.host.data();
+  const StepArgs<T> a0 = step_args_on<T>(h, h->fe, h->fh, h->ge, h->gh, h->d, h->b, h->dn, h->bn);
+  const StepArgs<T> a1 = step_args_on<T>(h, h->ge, h->gh, h->ke, h->kh, h->dn, h->bn, h->kd, h->kb);
+  const int tx = (int)h->grd_fused.x, ty = (int)h->grd_fused.y;
+  for (int g = 0; g < n; ++g) {
+    const RestGroup& G = h->rest_groups[g];
+    I[g].a[0] = a0;
+    I[g].a[1] = a1;
+    I[g].a[0].zchunk = I[g].a[1].zchunk = G.len;
+    I[g].tp = (const TfsfConst*)h->tfc;
+    I[g].chunks[0] = h->chunks_rest + G.off;
+    I[g].chunks[1] = h->chunks_rest + G.off + G.nlean;
+    I[g].tm[0] = TileMap{tx, ty, tx * ty * G.nlean, tx * h->xcd_rows};
+    I[g].tm[1] = TileMap{tx, ty, tx * ty * G.ngen, tx * h->xcd_rows};
+  }
+  int rc = dalloc(h, &t.dev, t.host.size());
+  if (rc) return rc;
+  HIP_TRY(h, hipMemcpyAsync(t.dev, t.host.data(), t.host.size(), hipMemcpyHostToDevice, h->stream));
+  h->rest_tabs.push_back(std::move(t));
+  *out = (const BatchInst<T>*)h->rest_tabs.back().dev;
+  return 0;
+}
+
 // Steps it and it+1 in one pass: the two-step kernel over its chunks (step n
 // from the current buffers, step n+2 into the third), the other chunks by two
 // single-step launches (current -> second buffer -> third), then the DFT
@@ -455,7 +507,24 @@ int launch_double(iris_handle* h, int it, const PassLists& L, bool dft) {
   const RestGroup one{h->zchunk, 0, L.nlean, L.ngen};
   const RestGroup* groups = L.groups ? L.groups : &one;
   const int ngroups = L.groups ? L.ngroups : 1;
+  // the handle's own lists in several lengths: one launch per class over
+  // all groups (k = 0: step n+1 into the second buffers, 1: step n+2)
+  const BatchInst<T>* tab = nullptr;
+  if (L.groups == h->rest_groups.data() && ngroups > 1) {
+    int rc = rest_table<T>(h, &tab);
+    if (rc) return rc;
+  }
   auto rest = [&](StepArgs<T> a, int step) {
+    if (tab) {
+      const int k = step == it ? 0 : 1;
+      if (h->rest_nwg[0])
+        hipLaunchKernelGGL((k_yee_fused_batch<T, false>), dim3(h->rest_nwg[0]), h->blk_fused, 0, h->stream, tab,
+                           (const int*)h->rest_wg[0], ngroups, k, step);
+      if (h->rest_nwg[1])
+        hipLaunchKernelGGL((k_yee_fused_batch<T, true>), dim3(h->rest_nwg[1]), h->blk_fused, 0, h->stream, tab,
+                           (const int*)h->rest_wg[1], ngroups, k, step);
+      return;
+    }
     for (int g = 0; g < ngroups; ++g) {
       const RestGroup& G = groups[g];
       if (G.nlean + G.ngen == 0) continue;
@@ -831,8 +900,15 @@ int build_passes(iris_handle* h, const PassConfig& c) {
   h->planes2 = h->planes2p = h->upml2p = 0;
   h->rest_groups.clear();
   h->nrest_lean = h->nrest_gen = 0;
-  h->zc2 = std::min(c.zc2 > 0 ? c.zc2 : 2 * h->zchunk, kChunkLenMask + 1);
-  const int zc2 = h->zc2;
+  for (int k = 0; k < 2; ++k) {
+    dfree(h, (void**)&h->rest_wg[k]);
+    h->rest_nwg[k] = 0;
+  }
+  for (auto& t : h->rest_tabs) dfree(h, &t.dev);
+  h->rest_tabs.clear();
+  // UPML-variant chunks: up to 2 zc planes; plain two-step chunks: below
+  const int zc2p = std::min(2 * h->zchunk, kChunkLenMask + 1);
+  int zc2 = zc2p;
   {
     // fused z-chunks: a chunk is bulk when every plane it reads or writes,
     // z0-1 .. z1+1, lies strictly between the UPML slabs, outside the
@@ -904,16 +980,12 @@ int build_passes(iris_handle* h, const PassConfig& c) {
         if (kind[z] != 2) continue;
         int e = z;
         while (e + 1 <= p.nz && kind[e + 1] == 2) ++e;
-        nch += (e - z + 1 + zc2 - 1) / zc2;
+        nch += (e - z + 1 + zc2p - 1) / zc2p;
         z = e;
       }
       const int64_t tiles = (int64_t)((p.nx + kT2OX - 1) / kT2OX) *
                             ((p.ny + Yee2Shape<T, true>::OY - 1) / Yee2Shape<T, true>::OY);
-      int dev = 0, ncu = 256;
-      if (hipGetDevice(&dev) != hipSuccess ||
-          hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-        ncu = 256;
-      if (tiles * nch < ncu) upml_two = false;
+      if (tiles * nch < h->ncu) upml_two = false;
     }
     if (!upml_two)
       for (int z = 1; z <= p.nz; ++z)
@@ -926,6 +998,47 @@ int build_passes(iris_handle* h, const PassConfig& c) {
       if (e == z) { kind[z] = 0; --nout; }
       z = e;
     }
+    // Plain chunk length.  One workgroup of the plain kernel fills a CU (16
+    // waves with FP32 fields, 12 with FP64, and most of the LDS), so a launch
+    // takes ceil(workgroups / CUs) rounds of (planes + 3) pipeline steps.
+    // When chunks of 2 zc planes take few rounds, a nearly empty last round
+    // costs as much as a full one (on a 128 x 128 x 768 grid 1830
+    // workgroups, 8 rounds, ran 7% slower than 1770, 7 rounds); then the
+    // length is the one of zc .. 4 zc with the fewest such steps (the
+    // shortest among equals): 22 planes there, 5-7% faster.  With many
+    // rounds the last one matters little and longer chunks measured slower
+    // (82 against 61 planes on 512 x 512 x 1024: 3-6%), so 2 zc stays.
+    auto plain_runs = [&]() {
+      std::vector<int> runs;
+      for (int z = 1; z <= p.nz; ++z) {
+        if (kind[z] != 1) continue;
+        int e = z;
+        while (e + 1 <= p.nz && kind[e + 1] == 1) ++e;
+        runs.push_back(e - z + 1);
+        z = e;
+      }
+      return runs;
+    };
+    const int64_t t2 = (int64_t)((p.nx + kT2OX - 1) / kT2OX) * ((p.ny + Yee2Shape<T>::OY - 1) / Yee2Shape<T>::OY);
+    auto rounds = [&](const std::vector<int>& runs, int L) {
+      int64_t nch = 0;
+      for (int r : runs) nch += (r + L - 1) / L;
+      return (t2 * nch + h->ncu - 1) / h->ncu;
+    };
+    if (c.zc2 > 0) {
+      zc2 = std::min(c.zc2, kChunkLenMask + 1);
+    } else if (nout > 0 && rounds(plain_runs(), zc2p) < 16) {
+      const std::vector<int> runs = plain_runs();
+      int64_t best = -1;
+      for (int L = h->zchunk; L <= std::min(4 * h->zchunk, kChunkLenMask + 1); ++L) {
+        const int64_t cost = rounds(runs, L) * (L + 3);
+        if (best < 0 || cost < best) {
+          best = cost;
+          zc2 = L;
+        }
+      }
+    }
+    h->zc2 = zc2;
     if (nout > 0) {
       std::vector<int> c2, c2p;
       std::map<int, std::pair<std::vector<int>, std::vector<int>>> byl;  // length -> (lean, general)
@@ -953,7 +1066,8 @@ int build_passes(iris_handle* h, const PassConfig& c) {
           while (s1 + 1 <= e && kind[s1 + 1] == kind[s0]) ++s1;
           // near-equal chunks of at most zc2 planes (a short last chunk
           // would repeat the pipeline's 3 planes of fill for few planes)
-          const int nseg = s1 - s0 + 1, nch = (nseg + zc2 - 1) / zc2;
+          const int step = kind[s0] == 1 ? zc2 : zc2p;
+          const int nseg = s1 - s0 + 1, nch = (nseg + step - 1) / step;
           for (int k = 0, c0 = s0; k < nch; ++k) {
             const int len = nseg / nch + (k < nseg % nch ? 1 : 0);
             int v = c0 | ((len - 1) << kChunkLenShift);
@@ -979,6 +1093,21 @@ int build_passes(iris_handle* h, const PassConfig& c) {
         h->rest_groups.push_back(G);
         h->nrest_lean += G.nlean;
         h->nrest_gen += G.ngen;
+      }
+      if (h->rest_groups.size() > 1) {
+        const int t = (int)(h->grd_fused.x * h->grd_fused.y);
+        std::vector<int> w[2] = {{0}, {0}};
+        for (const RestGroup& G : h->rest_groups) {
+          w[0].push_back(w[0].back() + t * G.nlean);
+          w[1].push_back(w[1].back() + t * G.ngen);
+        }
+        for (int k = 0; k < 2; ++k) {
+          if ((rc = dalloc(h, (void**)&h->rest_wg[k], sizeof(int) * w[k].size()))) return rc;
+          HIP_TRY(h, hipMemcpyAsync(h->rest_wg[k], w[k].data(), sizeof(int) * w[k].size(), hipMemcpyHostToDevice,
+                                    h->stream));
+          h->rest_nwg[k] = w[k].back();
+        }
+        HIP_TRY(h, hipStreamSynchronize(h->stream));
       }
       h->two_step = true;
       h->n2 = (int)c2.size();
@@ -2216,6 +2345,8 @@ int iris_create(const iris_params* p, const double* eps_inv, const double* fpzmn
       h->adj_ty = std::atoi(tyenv);
   }
   (void)hipGetDevice(&h->device);
+  if (hipDeviceGetAttribute(&h->ncu, hipDeviceAttributeMultiprocessorCount, h->device) != hipSuccess || h->ncu < 1)
+    h->ncu = 256;
   e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     delete h;
