@@ -875,6 +875,16 @@ void dfree(iris_handle* h, void** p) {
   *p = nullptr;
 }
 
+// A two-step tile of R rows covers footprint cells x0-1 .. x0+62 and
+// y0-1 .. y0+R-2; lanes outside 1..n stand for the periodic image of one
+// cell inside (a single wrap), which holds when the last tile's footprint
+// ends at or before 2n (so nx >= 32 always; ny >= 8 with 16 rows).
+bool two_step_fits(const iris_params& p, int R) {
+  const int OY = R - 3;
+  const int64_t tx = (p.nx + kT2OX - 1) / kT2OX, ty = (p.ny + OY - 1) / OY;
+  return (tx - 1) * kT2OX + kT2X - 1 <= 2LL * p.nx && (ty - 1) * OY + R - 1 <= 2LL * p.ny;
+}
+
 // (Re)builds the chunk lists of the step passes for layout c.  The field
 // state is not touched; the handle's stream is synchronised.
 template <typename T>
@@ -953,7 +963,8 @@ int build_passes(iris_handle* h, const PassConfig& c) {
       if (two > 0.95 * one) want = false;
     }
     h->bloch = !(p.ak[0][0] == 0.0 && p.ak[0][1] == 0.0 && p.ak[1][0] == 0.0 && p.ak[1][1] == 0.0);
-    const bool shape = h->fused && want && p.nx >= kT2X && p.ny >= Yee2Shape<T>::R && p.nz >= 6;
+    const bool shape = h->fused && want && two_step_fits(p, Yee2Shape<T>::R) && p.nz >= 6;
+    const bool shape_pml = two_step_fits(p, Yee2Shape<T, true>::R);
     auto tfsf_plane = [&](int z) { return p.em_source && (z == p.iz_min - 1 || z == p.iz_min); };
     auto upml_plane = [&](int z) { return z <= p.pmlzi || z >= p.pmlzf; };
     // 0: single step; 1: plain two-step output plane; 2: UPML-variant output plane
@@ -965,6 +976,7 @@ int build_passes(iris_handle* h, const PassConfig& c) {
       if (!ok) continue;
       bool plain = true;
       for (int q = z - 1; plain && q <= z + 2; ++q) plain = !upml_plane(q);
+      if (!plain && !shape_pml) continue;
       kind[z] = plain ? 1 : 2;
       ++nout;
     }
@@ -1497,7 +1509,7 @@ int build_lists(iris_handle* h, ListSet& out, int nobox, int face_on) {
     if (p.em_source && (z == p.iz_min - 1 || z == p.iz_min)) return false;
     return true;
   };
-  const bool shape = h->two_step && p.nx >= kT2X && p.ny >= Yee2Shape<T>::R;
+  const bool shape = h->two_step && two_step_fits(p, Yee2Shape<T>::R);
   const int zc = h->zchunk;
   std::vector<char> dbl;
   int ndbl = 0;

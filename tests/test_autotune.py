@@ -25,8 +25,8 @@ def setups(tmp_path):
     return {
         "two_step": two,
         "two_step_bloch": oblique(two, 25.0),
-        "single": make(tmp_path, "one", ixmax=40, iymax=12, izmax=80, pmlzi=9, pmlzf=71, hi=36, dh=8, dx=6,
-                       eps_bbox=3.0, box=(1, 40, 1, 12, 14, 66)),
+        "single": make(tmp_path, "one", ixmax=20, iymax=12, izmax=80, pmlzi=9, pmlzf=71, hi=36, dh=8, dx=6,
+                       eps_bbox=3.0, box=(1, 20, 1, 12, 14, 66)),
     }
 
 

@@ -358,6 +358,12 @@ def test_bloch_phases_match_oracle(tmp_path, fused):
 
 
 TWO_STEP_CASES = {
+    # narrower than a tile: one 64-lane tile over 40 cells in x (lanes 41..63
+    # stand for cells 1..23), and 9 rows in y under a 16- (12-) row tile
+    "narrow_x": dict(ixmax=40, iymax=21, izmax=96, pmlzi=10, pmlzf=86, hi=44, dh=10, dx=8, eps_bbox=2.25,
+                     tfsf_o=20, box=(1, 40, 2, 20, 15, 80), nof=3, nt=61, no_downloads=3),
+    "narrow_y": dict(ixmax=70, iymax=9, izmax=96, pmlzi=10, pmlzf=86, hi=44, dh=10, dx=8, eps_bbox=3.0,
+                     tfsf_o=20, box=(3, 66, 1, 9, 15, 80), nof=3, nt=61, no_downloads=3),
     # partial tiles in x (70 = 61 + 9) and y (21 = 13 + 8 with FP32 fields,
     # 2*8 + 5 with FP64), structure, two boxes whose side faces cross the
     # two-step chunks, odd cadence segments
@@ -441,7 +447,7 @@ def _nrel(a, b):
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
 
 
-@pytest.mark.parametrize("name", ["ipol2_2box", "ipol1_stride", "oblique"])
+@pytest.mark.parametrize("name", ["ipol2_2box", "ipol1_stride", "oblique", "narrow_x"])
 @pytest.mark.parametrize("precision", [8, 4])
 def test_two_step_pass_with_bloch_phase(tmp_path, name, precision):
     """With a Bloch phase in x and y the two-step pass's image lanes carry
