@@ -218,15 +218,18 @@ int iris_pass_info(const iris_handle* h, int64_t info[6]);
 /* Layout tuner (no counterpart in the reference).  Before the first step,
  * times `steps` steps (2..1000) of candidate layouts of the step passes --
  * chunk lengths, the two-step pass and its UPML variant on or off, the
- * dealing of tiles to XCDs -- and keeps the fastest.  Only layouts with the
+ * two-step tile width, the dealing of tiles to XCDs -- and keeps the
+ * fastest.  Only layouts with the
  * same results bit for bit are compared (with a Bloch phase the choice of the
  * two-step pass stays as it is).  The field state is restored and the handle
  * stays at step 0.  Needs device memory for one copy of the state; call it
  * before iris_checkpoint.  info[0] = two-step pass on, info[1] = UPML planes
  * in it, info[2] / info[3] = planes per single-step / two-step chunk,
  * info[4] / info[5] = tile rows per XCD run of the single-step / two-step
- * pass, info[6] = layouts timed, info[7] = ns per step of the one kept. */
-int iris_autotune(iris_handle* h, int32_t steps, int64_t info[8]);
+ * pass, info[6] = layouts timed, info[7] = ns per step of the one kept,
+ * info[8] = footprint cells per row of a two-step tile (64 or 32; 0 when
+ * the two-step pass is off). */
+int iris_autotune(iris_handle* h, int32_t steps, int64_t info[9]);
 
 /* ---- Gradient of a face-flux objective.  The reference has no adjoint;
  * the differentiable form stands beside the plain solver on the same handle.

@@ -97,6 +97,7 @@ struct PassConfig {
   int zc2 = 0;        // planes per two-step chunk (0: 2*zc)
   int xcd_rows = 0;   // tile rows per XCD run of the single-step pass (0: identity order)
   int xcd_rows2 = 4;  // the same for the two-step pass
+  int w2 = 0;         // two-step tile width, 64 or 32 cells (0: 64)
 };
 
 struct iris_handle {
@@ -154,6 +155,7 @@ struct iris_handle {
   int xcd_rows2 = 4;        // tile rows per XCD run of the two-step pass
   int zc2 = 0;              // planes per plain two-step chunk
   int ncu = 256;            // compute units of the device
+  int t2w = 64;             // two-step tile width (footprint cells per row)
   PassConfig pass;          // the layout the lists were built for
   // single steps of the two-step layout's other chunks when they come in
   // several lengths: one batched launch per class (lean, general) over the
@@ -253,10 +255,10 @@ struct iris_batch {
   // the two-step handles (`own`), batched: pairs of steps by one launch per
   // kernel over all of them.  Tables per pair parity (the pair runs from the
   // current buffers to the third ones, the next pair back).
-  std::vector<iris_handle*> own2[2];  // without / with a Bloch phase
-  void* y2[2][2] = {};                // Batch2Inst<T>[2][n]: [bloch][upml variant], pair parity outer
-  int* y2wg[2][2] = {};               // workgroup prefixes
-  int y2nwg[2][2] = {};
+  std::vector<iris_handle*> own2[4];  // by kernel: Bloch phase (bit 0), 32-cell tiles (bit 1)
+  void* y2[4][2] = {};                // Batch2Inst<T>[2][n]: [group][upml variant], pair parity outer
+  int* y2wg[4][2] = {};               // workgroup prefixes
+  int y2nwg[4][2] = {};
   void* rinst = nullptr;              // BatchInst<T>[2][nr]: their single-step chunks, per (handle, length group)
   int nr = 0;
   int* rwg[2] = {nullptr, nullptr};   // per class (lean, general)
@@ -449,6 +451,18 @@ int dft_sample(iris_handle* h, int n);
 template <typename T>
 int dft_flush(iris_handle* h);
 
+// One launch of the two-step kernel (variant PH, PML) at the handle's tile
+// width, tm.nblk workgroups.
+template <typename T, bool PH, bool PML>
+void launch_yee2(const iris_handle* h, const StepArgs<T>& a, cplx<T>* eb, cplx<T>* hb, cplx<T>* db,
+                 const int* chunks, const TileMap& tm, const Yee2Samples<T>& smp) {
+  const dim3 grid((unsigned)tm.nblk), blk(kT2X, Yee2Shape<T, PML>::R);
+  if (h->t2w == 32)
+    hipLaunchKernelGGL((k_yee2<T, PH, PML, 32>), grid, blk, 0, h->stream, a, eb, hb, db, chunks, tm, smp);
+  else
+    hipLaunchKernelGGL((k_yee2<T, PH, PML, 64>), grid, blk, 0, h->stream, a, eb, hb, db, chunks, tm, smp);
+}
+
 // Argument table of the handle's single-step length groups for its current
 // buffers (entry g: a[0] = step n+1 from the current into the second
 // buffers, a[1] = step n+2 from the second into the third), made on first
@@ -541,11 +555,9 @@ int launch_double(iris_handle* h, int it, const PassLists& L, bool dft) {
     const StepArgs<T> a2 = step_args_on<T>(h, h->fe, h->fh, h->ke, h->kh, h->d, h->b, h->dn, h->bn);
     const TileMap tm{h->tx2, h->ty2, h->tx2 * h->ty2 * L.n2, h->tx2 * h->xcd_rows2};
     if (h->bloch)
-      hipLaunchKernelGGL((k_yee2<T, true>), dim3((unsigned)tm.nblk), dim3(kT2X, Yee2Shape<T>::R), 0, h->stream, a2,
-                         (C*)h->ge, (C*)h->gh, nullptr, L.chunks2, tm, smp);
+      launch_yee2<T, true, false>(h, a2, (C*)h->ge, (C*)h->gh, nullptr, L.chunks2, tm, smp);
     else
-      hipLaunchKernelGGL((k_yee2<T, false>), dim3((unsigned)tm.nblk), dim3(kT2X, Yee2Shape<T>::R), 0, h->stream, a2,
-                         (C*)h->ge, (C*)h->gh, nullptr, L.chunks2, tm, smp);
+      launch_yee2<T, false, false>(h, a2, (C*)h->ge, (C*)h->gh, nullptr, L.chunks2, tm, smp);
   }
   if (L.n2p) {
     // the UPML variant: D and B of step n+2 into the third aux buffers
@@ -553,11 +565,9 @@ int launch_double(iris_handle* h, int it, const PassLists& L, bool dft) {
     const StepArgs<T> a2 = step_args_on<T>(h, h->fe, h->fh, h->ke, h->kh, h->d, h->b, h->kd, h->kb);
     const TileMap tm{h->tx2, h->ty2p, h->tx2 * h->ty2p * L.n2p, h->tx2 * h->xcd_rows2};
     if (h->bloch)
-      hipLaunchKernelGGL((k_yee2<T, true, true>), dim3((unsigned)tm.nblk), dim3(kT2X, Yee2Shape<T, true>::R), 0,
-                         h->stream, a2, (C*)h->ge, (C*)h->gh, (C*)h->dn, L.chunks2p, tm, smp);
+      launch_yee2<T, true, true>(h, a2, (C*)h->ge, (C*)h->gh, (C*)h->dn, L.chunks2p, tm, smp);
     else
-      hipLaunchKernelGGL((k_yee2<T, false, true>), dim3((unsigned)tm.nblk), dim3(kT2X, Yee2Shape<T, true>::R), 0,
-                         h->stream, a2, (C*)h->ge, (C*)h->gh, (C*)h->dn, L.chunks2p, tm, smp);
+      launch_yee2<T, false, true>(h, a2, (C*)h->ge, (C*)h->gh, (C*)h->dn, L.chunks2p, tm, smp);
   }
   if (L.nlean + L.ngen) {
     ProfScope ps(h, IRIS_K_YEE_REST);
@@ -860,6 +870,7 @@ PassConfig pass_config_env(const iris_params& p) {
   num("IRIS_ZCHUNK2", &c.zc2, 0);
   num("IRIS_XCD_ROWS", &c.xcd_rows, 0);
   num("IRIS_XCD_ROWS2", &c.xcd_rows2, 0);
+  num("IRIS_TILE_W2", &c.w2, 0);
   const char* t = std::getenv("IRIS_TWO_STEP");
   if (t && (t[0] == '0' || t[0] == '1')) c.two = t[0] - '0';
   const char* u = std::getenv("IRIS_TWO_STEP_UPML");
@@ -875,14 +886,29 @@ void dfree(iris_handle* h, void** p) {
   *p = nullptr;
 }
 
-// A two-step tile of R rows covers footprint cells x0-1 .. x0+62 and
-// y0-1 .. y0+R-2; lanes outside 1..n stand for the periodic image of one
-// cell inside (a single wrap), which holds when the last tile's footprint
-// ends at or before 2n (so nx >= 32 always; ny >= 8 with 16 rows).
-bool two_step_fits(const iris_params& p, int R) {
-  const int OY = R - 3;
-  const int64_t tx = (p.nx + kT2OX - 1) / kT2OX, ty = (p.ny + OY - 1) / OY;
-  return (tx - 1) * kT2OX + kT2X - 1 <= 2LL * p.nx && (ty - 1) * OY + R - 1 <= 2LL * p.ny;
+// A two-step tile of R waves covers footprint cells x0-1 .. x0+W-2 and
+// y0-1 .. y0+RY-2 (W = kT2W cells per row, RY = 64 R / W rows); lanes outside
+// 1..n stand for the periodic image of one cell inside (a single wrap),
+// which holds when the last tile's footprint ends at or before 2n (with
+// 64 x 16 tiles: nx >= 32, ny >= 8).
+struct T2Geom {
+  int W, R, RY, OX, OY;
+  int tx(int nx) const { return (nx + OX - 1) / OX; }
+  int ty(int ny) const { return (ny + OY - 1) / OY; }
+};
+template <typename T>
+T2Geom t2geom(int W, bool pml) {
+  T2Geom g;
+  g.W = W;
+  g.R = pml ? Yee2Shape<T, true>::R : Yee2Shape<T>::R;
+  g.RY = g.R * kT2X / W;
+  g.OX = W - 3;
+  g.OY = g.RY - 3;
+  return g;
+}
+bool two_step_fits(const iris_params& p, const T2Geom& g) {
+  return (int64_t)(g.tx(p.nx) - 1) * g.OX + g.W - 1 <= 2LL * p.nx &&
+         (int64_t)(g.ty(p.ny) - 1) * g.OY + g.RY - 1 <= 2LL * p.ny;
 }
 
 // (Re)builds the chunk lists of the step passes for layout c.  The field
@@ -955,16 +981,23 @@ int build_passes(iris_handle* h, const PassConfig& c) {
     // step).  Measured on MI355X with 768-plane grids, the two-step pass is
     // slower only where this ratio is above 1 (one 64-wide row of tiles
     // against two).
+    // tile width: 64 cells a row by default, or 32 (two rows per wave,
+    // twice the rows; iris_autotune tries it); the other where only that one
+    // fits.  32-cell tiles measured 6% slower at 512 x 512 x 1024 and within
+    // run-to-run noise (+-4%) at 128 x 128 x 768 and on the 64-grating sweep.
+    int W = (c.w2 == 32 || c.w2 == 64) ? c.w2 : 64;
+    if (!two_step_fits(p, t2geom<T>(W, false)) && two_step_fits(p, t2geom<T>(96 - W, false))) W = 96 - W;
+    h->t2w = W;
+    const T2Geom g2 = t2geom<T>(W, false), g2p = t2geom<T>(W, true);
     bool want = c.two != 0;
     if (c.two < 0) {
-      constexpr int R = Yee2Shape<T>::R, OY = Yee2Shape<T>::OY;
-      const double two = (double)((p.nx + kT2OX - 1) / kT2OX) * ((p.ny + OY - 1) / OY) * R / 2.0;
+      const double two = (double)g2.tx(p.nx) * g2.ty(p.ny) * g2.R / 2.0;
       const double one = (double)((p.nx + kFTX - 1) / kFTX) * ((p.ny + kFTY - 1) / kFTY) * (kFTY + 1);
       if (two > 0.95 * one) want = false;
     }
     h->bloch = !(p.ak[0][0] == 0.0 && p.ak[0][1] == 0.0 && p.ak[1][0] == 0.0 && p.ak[1][1] == 0.0);
-    const bool shape = h->fused && want && two_step_fits(p, Yee2Shape<T>::R) && p.nz >= 6;
-    const bool shape_pml = two_step_fits(p, Yee2Shape<T, true>::R);
+    const bool shape = h->fused && want && two_step_fits(p, g2) && p.nz >= 6;
+    const bool shape_pml = two_step_fits(p, g2p);
     auto tfsf_plane = [&](int z) { return p.em_source && (z == p.iz_min - 1 || z == p.iz_min); };
     auto upml_plane = [&](int z) { return z <= p.pmlzi || z >= p.pmlzf; };
     // 0: single step; 1: plain two-step output plane; 2: UPML-variant output plane
@@ -995,8 +1028,7 @@ int build_passes(iris_handle* h, const PassConfig& c) {
         nch += (e - z + 1 + zc2p - 1) / zc2p;
         z = e;
       }
-      const int64_t tiles = (int64_t)((p.nx + kT2OX - 1) / kT2OX) *
-                            ((p.ny + Yee2Shape<T, true>::OY - 1) / Yee2Shape<T, true>::OY);
+      const int64_t tiles = (int64_t)g2p.tx(p.nx) * g2p.ty(p.ny);
       if (tiles * nch < h->ncu) upml_two = false;
     }
     if (!upml_two)
@@ -1031,7 +1063,7 @@ int build_passes(iris_handle* h, const PassConfig& c) {
       }
       return runs;
     };
-    const int64_t t2 = (int64_t)((p.nx + kT2OX - 1) / kT2OX) * ((p.ny + Yee2Shape<T>::OY - 1) / Yee2Shape<T>::OY);
+    const int64_t t2 = (int64_t)g2.tx(p.nx) * g2.ty(p.ny);
     auto rounds = [&](const std::vector<int>& runs, int L) {
       int64_t nch = 0;
       for (int r : runs) nch += (r + L - 1) / L;
@@ -1124,9 +1156,9 @@ int build_passes(iris_handle* h, const PassConfig& c) {
       h->two_step = true;
       h->n2 = (int)c2.size();
       h->n2p = (int)c2p.size();
-      h->tx2 = (p.nx + kT2OX - 1) / kT2OX;
-      h->ty2 = (p.ny + Yee2Shape<T>::OY - 1) / Yee2Shape<T>::OY;
-      h->ty2p = (p.ny + Yee2Shape<T, true>::OY - 1) / Yee2Shape<T, true>::OY;
+      h->tx2 = g2.tx(p.nx);
+      h->ty2 = g2.ty(p.ny);
+      h->ty2p = g2p.ty(p.ny);
       dfree(h, (void**)&h->chunks2);
       dfree(h, (void**)&h->chunks2p);
       dfree(h, (void**)&h->chunks_rest);
@@ -1462,11 +1494,9 @@ int adjoint_double(iris_handle* h, int n, int nobox, int face_on, const void* re
     const TileMap tm{h->tx2, h->ty2, h->tx2 * h->ty2 * L.n2, h->tx2 * h->xcd_rows2};
     Yee2Samples<T> none{};
     if (h->bloch)
-      hipLaunchKernelGGL((k_yee2<T, true>), dim3((unsigned)tm.nblk), dim3(kT2X, Yee2Shape<T>::R), 0, h->stream, a2,
-                         (C*)h->ae2, (C*)h->ah2, nullptr, (const int*)L.chunks2, tm, none);
+      launch_yee2<T, true, false>(h, a2, (C*)h->ae2, (C*)h->ah2, nullptr, (const int*)L.chunks2, tm, none);
     else
-      hipLaunchKernelGGL((k_yee2<T, false>), dim3((unsigned)tm.nblk), dim3(kT2X, Yee2Shape<T>::R), 0, h->stream, a2,
-                         (C*)h->ae2, (C*)h->ah2, nullptr, (const int*)L.chunks2, tm, none);
+      launch_yee2<T, false, false>(h, a2, (C*)h->ae2, (C*)h->ah2, nullptr, (const int*)L.chunks2, tm, none);
   }
   const StepArgs<T> a3 = adj_args<T>(h, h->ae2, h->ah2, h->ae3, h->ah3, h->ad2, h->ab2, h->ad, h->ab);
   adj_sources<T>(h, a3, n - 1, nobox, face_on);
@@ -1509,7 +1539,7 @@ int build_lists(iris_handle* h, ListSet& out, int nobox, int face_on) {
     if (p.em_source && (z == p.iz_min - 1 || z == p.iz_min)) return false;
     return true;
   };
-  const bool shape = h->two_step && two_step_fits(p, Yee2Shape<T>::R);
+  const bool shape = h->two_step && two_step_fits(p, t2geom<T>(h->t2w, false));
   const int zc = h->zchunk;
   std::vector<char> dbl;
   int ndbl = 0;
@@ -1776,7 +1806,7 @@ int trial_steps(iris_handle* h, int steps, double* us) {
 // single steps (and the UPML variant) stays as it is.  The field state is
 // saved before and restored after; the handle stays at step 0.
 template <typename T>
-int autotune_impl(iris_handle* h, int steps, int64_t info[8]) {
+int autotune_impl(iris_handle* h, int steps, int64_t info[9]) {
   const size_t bytes = state_bytes(h);
   void* snap = nullptr;
   if (hipMalloc(&snap, bytes) != hipSuccess) {
@@ -1823,6 +1853,15 @@ int autotune_impl(iris_handle* h, int steps, int64_t info[8]) {
   }
   if (!rc) rc = build_passes<T>(h, best);
   const bool two = h->two_step;
+  if (!rc && two) {  // tile width of the two-step pass
+    const int w0 = h->t2w;
+    for (int v : {64, 32}) {
+      PassConfig c = best;
+      c.w2 = v;
+      if (v != w0) consider(c);
+    }
+    if (!rc) rc = build_passes<T>(h, best);
+  }
   for (int f : {1, 2}) {  // single-step chunks: shorter, longer
     PassConfig c = best;
     c.zc = f == 1 ? zc0 / 2 : std::min(2 * zc0, p.nz);
@@ -1872,6 +1911,7 @@ int autotune_impl(iris_handle* h, int steps, int64_t info[8]) {
   info[5] = h->xcd_rows2;
   info[6] = tried;
   info[7] = (int64_t)(tbest * 1000.0 + 0.5);
+  info[8] = h->two_step ? h->t2w : 0;
   return 0;
 }
 
@@ -1949,7 +1989,7 @@ template <typename T>
 int batch_tables2(iris_batch* b) {
   using C = cplx<T>;
   if (b->own.empty()) return 0;
-  for (int bl = 0; bl < 2; ++bl) {
+  for (int bl = 0; bl < 4; ++bl) {
     const std::vector<iris_handle*>& hs = b->own2[bl];
     const int n = (int)hs.size();
     if (!n) continue;
@@ -2028,8 +2068,8 @@ int batch_tables2(iris_batch* b) {
 template <typename T>
 int batch_create2(iris_batch* b) {
   int rc;
-  for (iris_handle* h : b->own) b->own2[h->bloch ? 1 : 0].push_back(h);
-  for (int bl = 0; bl < 2; ++bl) {
+  for (iris_handle* h : b->own) b->own2[(h->bloch ? 1 : 0) + (h->t2w == 32 ? 2 : 0)].push_back(h);
+  for (int bl = 0; bl < 4; ++bl) {
     const std::vector<iris_handle*>& hs = b->own2[bl];
     if (hs.empty()) continue;
     for (int pm = 0; pm < 2; ++pm) {
@@ -2095,6 +2135,16 @@ int batch_sample2(iris_batch* b, int half, int par, int n) {
   return 0;
 }
 
+template <typename T, bool PH, bool PML>
+void launch_yee2_batch(const iris_batch* b, int W, unsigned nblk, const Batch2Inst<T>* I, const int* wg, int n,
+                       int slot) {
+  const dim3 blk(kT2X, Yee2Shape<T, PML>::R);
+  if (W == 32)
+    hipLaunchKernelGGL((k_yee2_batch<T, PH, PML, 32>), dim3(nblk), blk, 0, b->stream, I, wg, n, slot);
+  else
+    hipLaunchKernelGGL((k_yee2_batch<T, PH, PML, 64>), dim3(nblk), blk, 0, b->stream, I, wg, n, slot);
+}
+
 // steps it_first..it_last of the two-step handles: pairs of steps while at
 // least three remain (as run_steps takes them alone), then single steps
 template <typename T>
@@ -2119,25 +2169,21 @@ int batch_run2(iris_batch* b, int it_first, int it_last) {
     const bool s2 = it + 1 >= p.to_fourier && ((it + 1) % p.sample_fourier) == 0;
     const int slot = s1 ? b->own[0]->nsamp : -1;
     rest(0, it);
-    for (int bl = 0; bl < 2; ++bl) {
+    for (int bl = 0; bl < 4; ++bl) {
       const int n = (int)b->own2[bl].size();
       for (int pm = 0; pm < 2; ++pm) {
         if (!b->y2nwg[bl][pm]) continue;
         const Batch2Inst<T>* I = (const Batch2Inst<T>*)b->y2[bl][pm] + (size_t)pp * n;
-        const dim3 g(b->y2nwg[bl][pm]);
         const int* w = b->y2wg[bl][pm];
-        if (bl == 0 && pm == 0)
-          hipLaunchKernelGGL((k_yee2_batch<T, false, false>), g, dim3(kT2X, Yee2Shape<T>::R), 0, b->stream, I, w, n,
-                             slot);
-        else if (bl == 0)
-          hipLaunchKernelGGL((k_yee2_batch<T, false, true>), g, dim3(kT2X, Yee2Shape<T, true>::R), 0, b->stream, I,
-                             w, n, slot);
-        else if (pm == 0)
-          hipLaunchKernelGGL((k_yee2_batch<T, true, false>), g, dim3(kT2X, Yee2Shape<T>::R), 0, b->stream, I, w, n,
-                             slot);
-        else
-          hipLaunchKernelGGL((k_yee2_batch<T, true, true>), g, dim3(kT2X, Yee2Shape<T, true>::R), 0, b->stream, I,
-                             w, n, slot);
+        const int W = (bl & 2) ? 32 : 64;
+        const unsigned g = (unsigned)b->y2nwg[bl][pm];
+        if (bl & 1) {
+          if (pm) launch_yee2_batch<T, true, true>(b, W, g, I, w, n, slot);
+          else launch_yee2_batch<T, true, false>(b, W, g, I, w, n, slot);
+        } else {
+          if (pm) launch_yee2_batch<T, false, true>(b, W, g, I, w, n, slot);
+          else launch_yee2_batch<T, false, false>(b, W, g, I, w, n, slot);
+        }
       }
     }
     rest(1, it + 1);
@@ -2682,7 +2728,7 @@ int iris_pass_info(const iris_handle* h, int64_t info[6]) {
   return 0;
 }
 
-int iris_autotune(iris_handle* h, int32_t steps, int64_t info[8]) {
+int iris_autotune(iris_handle* h, int32_t steps, int64_t info[9]) {
   if (!h || !info) return fail(h, IRIS_EINVAL, "handle or info is NULL");
   if (h->it != 0) return fail(h, IRIS_ESTATE, "iris_autotune: call it before the first step (the handle is at step %d)", h->it);
   if (h->ck_seg > 0) return fail(h, IRIS_ESTATE, "iris_autotune: call it before iris_checkpoint");

@@ -33,16 +33,19 @@
 
 namespace iris {
 
-constexpr int kT2X = 64;
-constexpr int kT2OX = kT2X - 3;  // output cells per row of a tile
+constexpr int kT2X = 64;  // lanes per wave (the workgroup's x dimension)
 // Waves per workgroup (footprint rows) and the waves per SIMD the register
 // allocation must allow: FP32 fields 16 rows, one workgroup per CU (98 KB of
 // LDS); FP64 fields 12 rows, one workgroup per CU (147 KB of LDS).
 // The UPML variant carries D and B through the pipeline as well (twelve
 // more complex registers per lane) and uses 12 / 8 rows.
-template <typename T, bool PML = false> struct Yee2Shape {
-  static constexpr int R = sizeof(T) == 4 ? (PML ? 12 : 16) : (PML ? 8 : 12);
-  static constexpr int OY = R - 3;  // output rows per tile
+// W = footprint cells per tile row: 64 (one row per wave) or 32 (two rows
+// per wave, so twice the rows: fewer halo cells per output on small grids).
+template <typename T, bool PML = false, int W = 64> struct Yee2Shape {
+  static constexpr int R = sizeof(T) == 4 ? (PML ? 12 : 16) : (PML ? 8 : 12);  // waves
+  static constexpr int RY = R * kT2X / W;  // footprint rows
+  static constexpr int OX = W - 3;         // output cells per tile row
+  static constexpr int OY = RY - 3;        // output rows per tile
   static constexpr int WAVES_PER_EU = (R + 3) / 4;
 };
 
@@ -152,16 +155,18 @@ template <typename T> struct Yee2Samples {
 // the export below, D of step n+1 on plane z0 also goes to db: the
 // single-step pass under the chunk recomputes E of step n+2 there.
 // The work of one workgroup: tile (bx, by) of chunk bz.
-template <typename T, bool PH, bool PML>
+template <typename T, bool PH, bool PML, int kT2W>
 __device__ __forceinline__ void yee2_tile(const StepArgs<T>& a, cplx<T>* __restrict__ eb, cplx<T>* __restrict__ hb,
                                           cplx<T>* __restrict__ db, const int* __restrict__ chunks, int bx, int by,
                                           int bz, const Yee2Samples<T>& smp, int slot) {
   using C = cplx<T>;
-  constexpr int kT2R = Yee2Shape<T, PML>::R, kT2OY = Yee2Shape<T, PML>::OY;
-  __shared__ C s1[2][3][kT2R][kT2X];  // E1 of planes z (z & 1) and z-1
-  __shared__ C sh[3][kT2R][kT2X];  // H1 of plane z-1
-  __shared__ C s2[3][kT2R][kT2X];  // E2 of plane z-1
-  const int lane = threadIdx.x, wy = threadIdx.y;
+  constexpr int kT2R = Yee2Shape<T, PML, kT2W>::RY, kT2OX = Yee2Shape<T, PML, kT2W>::OX,
+                kT2OY = Yee2Shape<T, PML, kT2W>::OY;
+  __shared__ C s1[2][3][kT2R][kT2W];  // E1 of planes z (z & 1) and z-1
+  __shared__ C sh[3][kT2R][kT2W];  // H1 of plane z-1
+  __shared__ C s2[3][kT2R][kT2W];  // E2 of plane z-1
+  // footprint cell (cx, cy) of this lane: one row per wave, or two
+  const int cx = threadIdx.x % kT2W, cy = threadIdx.y * (kT2X / kT2W) + threadIdx.x / kT2W;
   const int nx = a.nx, ny = a.ny;
   const int64_t cs = a.cs, sxy = a.sxy;
   const int craw = chunks[bz];
@@ -170,16 +175,16 @@ __device__ __forceinline__ void yee2_tile(const StepArgs<T>& a, cplx<T>* __restr
   const bool exp_below = (craw & kChunkExportBelow) != 0, exp_above = (craw & kChunkExportAbove) != 0;
 
   // footprint cell of this lane and its periodic source
-  const int xg = bx * kT2OX + lane, yg = by * kT2OY + wy;  // x0-1+lane, y0-1+wy
+  const int xg = bx * kT2OX + cx, yg = by * kT2OY + cy;  // x0-1+cx, y0-1+cy
   const int xs = xg < 1 ? xg + nx : (xg > nx ? xg - nx : xg);
   const int ys = yg < 1 ? yg + ny : (yg > ny ? yg - ny : yg);
-  const bool own = lane >= 1 && lane <= kT2OX && wy >= 1 && wy <= kT2OY && xg <= nx && yg <= ny;
+  const bool own = cx >= 1 && cx <= kT2OX && cy >= 1 && cy <= kT2OY && xg <= nx && yg <= ny;
   const int64_t col = (int64_t)(xs + a.xoff) + a.sx * ys;
   const int64_t colxm = col - 1;     // x-1 (the ghost x = 0 when xs = 1)
   const int64_t colym = col - a.sx;  // y-1 (the ghost y = 0 when ys = 1)
   // LDS neighbours, clamped into the tile (edge lanes compute values nobody uses)
-  const int lp = min(lane + 1, kT2X - 1), lm = max(lane - 1, 0);
-  const int wp = min(wy + 1, kT2R - 1), wm = max(wy - 1, 0);
+  const int lp = min(cx + 1, kT2W - 1), lm = max(cx - 1, 0);
+  const int wp = min(cy + 1, kT2R - 1), wm = max(cy - 1, 0);
   const C* __restrict__ e0 = a.e;
   const C* __restrict__ h0 = a.h;
   // Bloch factor of the cell this lane stands for (1 inside); every lane
@@ -238,8 +243,8 @@ __device__ __forceinline__ void yee2_tile(const StepArgs<T>& a, cplx<T>* __restr
     C h1n[3] = {};
     const int pz = z & 1, pm = pz ^ 1;  // LDS slot of E1(z) and of E1(z-1)
     if (z >= z0) {
-      const C e3yp = s1[pm][2][wp][lane], e1yp = s1[pm][0][wp][lane];
-      const C e3xp = s1[pm][2][wy][lp], e2xp = s1[pm][1][wy][lp];
+      const C e3yp = s1[pm][2][wp][cx], e1yp = s1[pm][0][wp][cx];
+      const C e3xp = s1[pm][2][cy][lp], e2xp = s1[pm][1][cy][lp];
       const C c0 = cadd(csub(csub(e3yp, e1p[2]), e1[1]), e1p[1]);
       const C c1 = cadd(csub(csub(e1[0], e1p[0]), e3xp), e1p[2]);
       const C c2 = cadd(csub(csub(e2xp, e1p[1]), e1yp), e1p[0]);
@@ -255,18 +260,18 @@ __device__ __forceinline__ void yee2_tile(const StepArgs<T>& a, cplx<T>* __restr
     }
     // slot pz last held E1(z-2), read before the previous plane's second
     // barrier; sh was last read before the previous plane's second barrier
-    s1[pz][0][wy][lane] = e1[0];
-    s1[pz][1][wy][lane] = e1[1];
-    s1[pz][2][wy][lane] = e1[2];
-    sh[0][wy][lane] = h1n[0];
-    sh[1][wy][lane] = h1n[1];
-    sh[2][wy][lane] = h1n[2];
+    s1[pz][0][cy][cx] = e1[0];
+    s1[pz][1][cy][cx] = e1[1];
+    s1[pz][2][cy][cx] = e1[2];
+    sh[0][cy][cx] = h1n[0];
+    sh[1][cy][cx] = h1n[1];
+    sh[2][cy][cx] = h1n[2];
     __syncthreads();
     // ---- E2(z-1) from E1(z-1), H1(z-1) (own and LDS -x/-y), H1(z-2)
     C e2n[3] = {}, d2n[3] = {};
     if (z >= z0 + 1) {
-      const C h3ym = sh[2][wm][lane], h1ym = sh[0][wm][lane];
-      const C h3xm = sh[2][wy][lm], h2xm = sh[1][wy][lm];
+      const C h3ym = sh[2][wm][cx], h1ym = sh[0][wm][cx];
+      const C h3xm = sh[2][cy][lm], h2xm = sh[1][cy][lm];
       const C c[3] = {cadd(csub(csub(h1n[2], h3ym), h1n[1]), h1pp[1]),
                       cadd(csub(csub(h1n[0], h1pp[0]), h1n[2]), h3xm),
                       cadd(csub(csub(h1n[1], h2xm), h1n[0]), h1ym)};
@@ -279,8 +284,8 @@ __device__ __forceinline__ void yee2_tile(const StepArgs<T>& a, cplx<T>* __restr
     // ---- H2(z-2) from H1(z-2), E2(z-2) (own and LDS +x/+y), E2(z-1)
     C h2n[3] = {}, b2n[3] = {};
     if (z >= z0 + 2) {
-      const C e3yp = s2[2][wp][lane], e1yp = s2[0][wp][lane];
-      const C e3xp = s2[2][wy][lp], e2xp = s2[1][wy][lp];
+      const C e3yp = s2[2][wp][cx], e1yp = s2[0][wp][cx];
+      const C e3xp = s2[2][cy][lp], e2xp = s2[1][cy][lp];
       const C c0 = cadd(csub(csub(e3yp, e2p[2]), e2n[1]), e2p[1]);
       const C c1 = cadd(csub(csub(e2n[0], e2p[0]), e3xp), e2p[2]);
       const C c2 = cadd(csub(csub(e2xp, e2p[1]), e1yp), e2p[0]);
@@ -294,9 +299,9 @@ __device__ __forceinline__ void yee2_tile(const StepArgs<T>& a, cplx<T>* __restr
       }
     }
     __syncthreads();  // the E2(z-2) neighbours and H1(z-1) neighbours have been read
-    s2[0][wy][lane] = e2n[0];
-    s2[1][wy][lane] = e2n[1];
-    s2[2][wy][lane] = e2n[2];
+    s2[0][cy][cx] = e2n[0];
+    s2[1][cy][cx] = e2n[1];
+    s2[2][cy][cx] = e2n[2];
 
     if (own) {
       // step n+2 of the chunk's planes, with H's min-side ghost planes
@@ -357,13 +362,13 @@ __device__ __forceinline__ void yee2_tile(const StepArgs<T>& a, cplx<T>* __restr
   }
 }
 
-template <typename T, bool PH, bool PML = false>
+template <typename T, bool PH, bool PML = false, int W = 64>
 __global__ __launch_bounds__((kT2X * Yee2Shape<T, PML>::R), (Yee2Shape<T, PML>::WAVES_PER_EU)) void k_yee2(
     StepArgs<T> a, cplx<T>* __restrict__ eb, cplx<T>* __restrict__ hb, cplx<T>* __restrict__ db,
     const int* __restrict__ chunks, TileMap tm, Yee2Samples<T> smp) {
   int bx, by, bz;
   tile_of(tm, blockIdx.x, &bx, &by, &bz);
-  yee2_tile<T, PH, PML>(a, eb, hb, db, chunks, bx, by, bz, smp, 0);
+  yee2_tile<T, PH, PML, W>(a, eb, hb, db, chunks, bx, by, bz, smp, 0);
 }
 
 // Batched two-step pass: the chunks of many grids in one launch.  Entry i of
@@ -383,7 +388,7 @@ template <typename T> struct Batch2Inst {
   Yee2Samples<T> smp;  // ring bases (slot 0)
 };
 
-template <typename T, bool PH, bool PML>
+template <typename T, bool PH, bool PML, int W>
 __global__ __launch_bounds__((kT2X * Yee2Shape<T, PML>::R), (Yee2Shape<T, PML>::WAVES_PER_EU)) void k_yee2_batch(
     const Batch2Inst<T>* __restrict__ inst, const int* __restrict__ wg_start, int ninst, int slot) {
   const int blk = blockIdx.x;
@@ -393,7 +398,7 @@ __global__ __launch_bounds__((kT2X * Yee2Shape<T, PML>::R), (Yee2Shape<T, PML>::
   if (L >= I.tm.nblk) return;  // padding (no barrier has been reached)
   int bx, by, bz;
   tile_of(I.tm, L, &bx, &by, &bz);
-  yee2_tile<T, PH, PML>(I.a, I.eb, I.hb, I.db, I.chunks, bx, by, bz, I.smp, slot);
+  yee2_tile<T, PH, PML, W>(I.a, I.eb, I.hb, I.db, I.chunks, bx, by, bz, I.smp, slot);
 }
 
 // DFT samples of one step from a buffer that holds that step only on the

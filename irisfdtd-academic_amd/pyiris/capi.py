@@ -240,11 +240,12 @@ class Engine:
         """Time candidate layouts of the step passes for `steps` steps each and
         keep the fastest (before the first step; results unchanged bit for
         bit).  Returns {two_step, upml_planes, zchunk, zchunk2, xcd_rows,
-        xcd_rows2, layouts, ns_per_step}."""
-        v = np.zeros(8, dtype=np.int64)
+        xcd_rows2, layouts, ns_per_step, tile_width}."""
+        v = np.zeros(9, dtype=np.int64)
         self._check(self.lib.iris_autotune(self._h, int(steps), _p(v)), "iris_autotune")
         return dict(two_step=bool(v[0]), upml_planes=int(v[1]), zchunk=int(v[2]), zchunk2=int(v[3]),
-                    xcd_rows=int(v[4]), xcd_rows2=int(v[5]), layouts=int(v[6]), ns_per_step=int(v[7]))
+                    xcd_rows=int(v[4]), xcd_rows2=int(v[5]), layouts=int(v[6]), ns_per_step=int(v[7]),
+                    tile_width=int(v[8]))
 
     def checkpoint(self, segment, record_bytes=0):
         """Prepare the coming forward run for adjoint(): H of its last steps

@@ -233,15 +233,18 @@ def test_adjoint_with_two_step_forward(tmp_path, monkeypatch):
     s = make_setup(tmp_path, "two_step")
     w = weights_for(s.params)
     for prec in (4, 8):
-        with Engine(s, precision=prec) as eng:
-            assert eng.pass_info()["two_step"]
-        g2, _ = engine_gradient(s, w, prec, 17)
         monkeypatch.setenv("IRIS_TWO_STEP", "0")
         with Engine(s, precision=prec) as eng:
             assert not eng.pass_info()["two_step"]
         g1, _ = engine_gradient(s, w, prec, 17)
         monkeypatch.delenv("IRIS_TWO_STEP")
-        assert np.array_equal(g1, g2)
+        for width in ("64", "32"):  # tiles of 64 or 32 cells a row
+            monkeypatch.setenv("IRIS_TILE_W2", width)
+            with Engine(s, precision=prec) as eng:
+                assert eng.pass_info()["two_step"]
+            g2, _ = engine_gradient(s, w, prec, 17)
+            assert np.array_equal(g1, g2), width
+        monkeypatch.delenv("IRIS_TILE_W2")
         assert np.abs(g1).max() > 0
 
 

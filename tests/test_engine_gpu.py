@@ -384,8 +384,9 @@ TWO_STEP_CASES = {
 }
 
 
-def _run_two_step(setup, precision, two_step, zchunk):
-    keys = {"IRIS_TWO_STEP": "1" if two_step else "0", "IRIS_TWO_STEP_UPML": "1", "IRIS_ZCHUNK": str(zchunk)}
+def _run_two_step(setup, precision, two_step, zchunk, width=0):
+    keys = {"IRIS_TWO_STEP": "1" if two_step else "0", "IRIS_TWO_STEP_UPML": "1", "IRIS_ZCHUNK": str(zchunk),
+            "IRIS_TILE_W2": str(width)}
     old = {k: os.environ.get(k) for k in keys}
     os.environ.update(keys)
     try:
@@ -413,13 +414,15 @@ def _run_two_step(setup, precision, two_step, zchunk):
 @pytest.mark.parametrize("name", list(TWO_STEP_CASES))
 @pytest.mark.parametrize("precision", [8, 4])
 @pytest.mark.parametrize("zchunk", [1, 5, 16])
-def test_two_step_pass_is_bitwise_two_single_steps(tmp_path, name, precision, zchunk):
-    """Two steps per pass over the plain z-chunks give bit for bit the fields,
-    DFT accumulators, fluxes and previous-step H of single steps."""
-    _announce(f"two-step {name} prec={precision} zchunk={zchunk}")
+@pytest.mark.parametrize("width", [64, 32])
+def test_two_step_pass_is_bitwise_two_single_steps(tmp_path, name, precision, zchunk, width):
+    """Two steps per pass over the plain z-chunks (tiles of 64 or 32 cells a
+    row) give bit for bit the fields, DFT accumulators, fluxes and
+    previous-step H of single steps."""
+    _announce(f"two-step {name} prec={precision} zchunk={zchunk} width={width}")
     s = load_input(write_input(tmp_path, inputs.render(**TWO_STEP_CASES[name])))
     one = _run_two_step(s, precision, False, zchunk)
-    two = _run_two_step(s, precision, True, zchunk)
+    two = _run_two_step(s, precision, True, zchunk, width)
     assert one["used"] == 0 and two["used"] > 0
     assert (two["used_pml"] > 0) == (two["upml_planes"] > 0)
     for b in one["res"]:
@@ -449,19 +452,20 @@ def _nrel(a, b):
 
 @pytest.mark.parametrize("name", ["ipol2_2box", "ipol1_stride", "oblique", "narrow_x"])
 @pytest.mark.parametrize("precision", [8, 4])
-def test_two_step_pass_with_bloch_phase(tmp_path, name, precision):
+@pytest.mark.parametrize("width", [64, 32])
+def test_two_step_pass_with_bloch_phase(tmp_path, name, precision, width):
     """With a Bloch phase in x and y the two-step pass's image lanes carry
     the phase: fields, DFT accumulators and fluxes equal those of single
     steps up to rounding (1e-12 FP64, 1e-5 FP32, normwise), and at FP64 the
     fluxes match the oracle to 1e-10."""
-    _announce(f"two-step bloch {name} prec={precision}")
+    _announce(f"two-step bloch {name} prec={precision} width={width}")
     if name == "oblique":
         from pyiris.sweep import oblique
         s = oblique(load_input(write_input(tmp_path, inputs.render(**TWO_STEP_CASES["lossy_structure"]))), 35.0)
     else:
         s = _bloch(load_input(write_input(tmp_path, inputs.render(**TWO_STEP_CASES[name]))))
     one = _run_two_step(s, precision, False, 5)
-    two = _run_two_step(s, precision, True, 5)
+    two = _run_two_step(s, precision, True, 5, width)
     assert one["used"] == 0 and two["used"] > 0
     assert (two["used_pml"] > 0) == (two["upml_planes"] > 0)
     tol = 1e-12 if precision == 8 else 1e-5
