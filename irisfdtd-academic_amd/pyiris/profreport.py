@@ -35,7 +35,7 @@ from pmcprobe import COPY_BYTES  # noqa: E402
 
 CLASSES = [
     ("yee2_batch", r"k_yee2_batch"),
-    ("yee2_pml", r"k_yee2<\w+, \w+, true>"),
+    ("yee2_pml", r"k_yee2<\w+, \w+, true\b"),
     ("yee2", r"k_yee2<"),
     ("yee_batch", r"k_yee_fused_batch"),
     ("yee", r"k_yee_fused<"),
