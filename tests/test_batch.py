@@ -152,3 +152,25 @@ def test_batch_rejects(tmp_path):
     finally:
         e8.close()
         e4.close()
+
+
+@pytest.mark.gpu
+def test_tuned_handles_in_a_batch(tmp_path):
+    """Handles whose layouts iris_autotune chose (each its own) batch
+    together and still give the fluxes of the default runs alone."""
+    print("[gpu-test] tuned handles in a batch", flush=True)
+    setups = mixed_setups(tmp_path)
+    nt = setups[0].params.nt
+    ref = [alone(s, 4, [(1, nt)]) for s in setups]
+    engines = [Engine(s, precision=4) for s in setups]
+    try:
+        for e in engines:
+            e.autotune(4)
+        with Batch(engines) as b:
+            b.step(1, nt)
+        for e, r in zip(engines, ref):
+            assert np.array_equal(e.flux(1), r[0])
+            assert np.array_equal(e.field(0), r[2]) and np.array_equal(e.field(1), r[3])
+    finally:
+        for e in engines:
+            e.close()
