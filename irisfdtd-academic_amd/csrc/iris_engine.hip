@@ -2095,7 +2095,7 @@ int batch_create2(iris_batch* b) {
       b->nr++;
     }
     for (int bx = 0; bx < h->p.nbox; ++bx)
-      if (h->box[bx].g.ntot) start.push_back(start.back() + h->box[bx].g.ntot);
+      if (h->box[bx].g.ntot) start.push_back(start.back() + (h->box[bx].g.ntot + 255) / 256 * 256);
   }
   for (int k = 0; k < 2; ++k) {
     if ((rc = balloc(b, (void**)&b->rwg[k], sizeof(int) * wg[k].size()))) return rc;
@@ -2250,7 +2250,7 @@ int batch_create_impl(iris_batch* b) {
     wg[0].push_back(wg[0].back() + t * (int)lean.size());
     wg[1].push_back(wg[1].back() + t * (int)gen.size());
     for (int bx = 0; bx < h->p.nbox; ++bx)
-      if (h->box[bx].g.ntot) start.push_back(start.back() + h->box[bx].g.ntot);
+      if (h->box[bx].g.ntot) start.push_back(start.back() + (h->box[bx].g.ntot + 255) / 256 * 256);
   }
   for (int k = 0; k < 2; ++k) {
     if ((rc = balloc(b, (void**)&b->wg[k], sizeof(int) * wg[k].size()))) return rc;

@@ -58,8 +58,10 @@ __global__ __launch_bounds__(256) void k_dft_sample(StepArgs<T> a, BoxGeom g, cp
 }
 
 // Batched sampling: entry e (one box of one instance of a batch) owns the
-// concatenated point indices start[e] .. start[e+1]-1 and reads the fields of
-// its instance's state a[par].
+// concatenated point indices start[e] .. start[e] + g.ntot - 1 and reads the
+// fields of its instance's state a[par].  Every start[e] is a multiple of the
+// 256-thread block, so a block works for one entry: the lookup and the
+// entry's arguments are uniform (scalar loads).
 template <typename T> struct BatchSamp {
   StepArgs<T> a[2];
   BoxGeom g;
@@ -70,16 +72,18 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_dft_sample_batch(const BatchSamp<T>* __restrict__ ent,
                                                           const int64_t* __restrict__ start, int nent, int par,
                                                           int slot) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= start[nent]) return;
+  const int64_t b0 = (int64_t)blockIdx.x * blockDim.x;
+  if (b0 >= start[nent]) return;
   int lo = 0, hi = nent;
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
-    if (start[mid] <= i) lo = mid; else hi = mid;
+    if (start[mid] <= b0) lo = mid; else hi = mid;
   }
+  lo = __builtin_amdgcn_readfirstlane(lo);
   const BatchSamp<T>& E = ent[lo];
   const StepArgs<T>& a = E.a[par];
-  const int64_t r = i - start[lo];
+  const int64_t r = b0 + threadIdx.x - start[lo];
+  if (r >= E.g.ntot) return;
   int c, x, y, z;
   box_point(E.g, r, &c, &x, &y, &z);
   if (E.zmask && E.zmask[z]) return;
