@@ -83,6 +83,7 @@ template <typename T> struct StepArgs {
   C fxp, fyp, fzp;  // exp(+i ak) Bloch factors
   C fxm, fym, fzm;  // exp(-i ak)
   int zchunk;       // planes per thread in the z-marching passes
+  int nt_out = 0;   // two-step pass: stream its E and H of step n+2 past the L2 (1) or not (0)
   // offset of component c of UPML slab k at in-slab index pidx
   __device__ int64_t aux(int k, int c, int64_t pidx) const {
     return (k ? pb1 + c * ps1 : c * ps0) + pidx;

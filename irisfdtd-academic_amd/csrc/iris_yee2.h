@@ -131,6 +131,21 @@ __device__ inline void h_upml3(const PmlPlane<T>& P, const cplx<T> h[3], const c
   }
 }
 
+// Streaming store of a complex value (the L2 does not keep the line).
+template <typename C> __device__ inline void st_nt(C* p, C v) {
+  using E = decltype(v.x);
+  typedef E V2 __attribute__((ext_vector_type(2)));
+  V2 w;
+  w.x = v.x;
+  w.y = v.y;
+  __builtin_nontemporal_store(w, reinterpret_cast<V2*>(p));
+}
+template <typename C> __device__ inline void store3_nt(C* f, int64_t i, int64_t cs, const C v[3]) {
+  st_nt(f + i, v[0]);
+  st_nt(f + i + cs, v[1]);
+  st_nt(f + i + 2 * cs, v[2]);
+}
+
 // Sampling of step n+1 by k_yee2: up to two boxes; samp[b] already points at
 // the ring slot of that step, or is null when the step is not sampled.
 template <typename T> struct Yee2Samples {
@@ -306,7 +321,8 @@ __device__ __forceinline__ void yee2_tile(const StepArgs<T>& a, cplx<T>* __restr
     if (own) {
       // step n+2 of the chunk's planes, with H's min-side ghost planes
       if (z >= z0 + 1 && z - 1 <= z1) {
-        store3(a.en, col + sxy * (z - 1), cs, e2n);
+        if (a.nt_out) store3_nt(a.en, col + sxy * (z - 1), cs, e2n);
+        else store3(a.en, col + sxy * (z - 1), cs, e2n);
         if (upml(z - 1)) {
           a.dn[auxi(z - 1, 0)] = d2n[0];
           a.dn[auxi(z - 1, 1)] = d2n[1];
@@ -315,7 +331,8 @@ __device__ __forceinline__ void yee2_tile(const StepArgs<T>& a, cplx<T>* __restr
       }
       if (z >= z0 + 2) {
         const int64_t ih = col + sxy * (z - 2);
-        store3(a.hn, ih, cs, h2n);
+        if (a.nt_out) store3_nt(a.hn, ih, cs, h2n);
+        else store3(a.hn, ih, cs, h2n);
         if (upml(z - 2)) {
           a.bn[auxi(z - 2, 0)] = b2n[0];
           a.bn[auxi(z - 2, 1)] = b2n[1];
