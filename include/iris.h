@@ -244,8 +244,10 @@ int iris_autotune(iris_handle* h, int32_t steps, int64_t info[9]);
  * fit in record_bytes (0: none; < 0: as many as fit in 90% of the free
  * device memory beside the snapshots and the adjoint's buffers), at most nt.
  * Call before the first step.  Allocates everything up front (IRIS_ENOMEM
- * when it does not fit).  segment <= 0 frees it all.  No counterpart in the
- * reference. */
+ * when it does not fit).  segment < 0 chooses it: the larger of
+ * ceil(sqrt(nt)) and sqrt(nt * snapshot bytes / record bytes), where the
+ * snapshots and one segment's records take the least memory together.
+ * segment = 0 frees it all.  No counterpart in the reference. */
 int iris_checkpoint(iris_handle* h, int32_t segment, int64_t record_bytes);
 
 /* Layout chosen by iris_checkpoint: info[0] = segment, info[1] = snapshots,

@@ -2463,11 +2463,22 @@ int iris_checkpoint(iris_handle* h, int32_t segment, int64_t record_bytes) {
   h->ck_seg = 0;
   h->tail_t0 = 0;
   h->tail_n = 0;
-  if (segment <= 0) return 0;
+  if (segment == 0) return 0;
   const iris_params& p = h->p;
   const int nt = p.nt;
   h->rec_plane = h->sxy * (p.hf - p.hi + 2);
   const size_t rstep = rec_step_bytes(h), sbytes = state_bytes(h);
+  if (segment < 0) {
+    // automatic: the snapshots ((nt / segment) states) and one segment's
+    // records (segment records of the structure planes) take the least
+    // memory together at segment = sqrt(nt * state / record); never below
+    // ceil(sqrt(nt)).  On a 256 x 256 x 1024 grid with N = 16000 that is
+    // ~1000 steps: 16 snapshots instead of 126 (which would not fit).
+    const double s_mem = std::sqrt((double)nt * (double)sbytes / (double)rstep);
+    const double s_min = std::ceil(std::sqrt((double)nt));
+    segment = (int32_t)std::min<double>(nt, std::max(s_min, std::floor(s_mem)));
+    segment = std::max<int32_t>(segment, 1);
+  }
   int64_t nrec = 0;
   if (record_bytes > 0) {
     nrec = std::min<int64_t>(nt, record_bytes / (int64_t)rstep);

@@ -9,8 +9,9 @@ dObjective/d eps_inv for every structure cell.  Any objective built in torch
 from Jabs -- e.g. the transmission T = J6/(J5+J6) of the reference's
 post-processing -- is differentiated this way.
 
-The engine holds the forward snapshots (iris_checkpoint, default every
-ceil(sqrt(nt)) steps) and the H records of the run's last steps (by default
+The engine holds the forward snapshots (iris_checkpoint; by default every
+ceil(sqrt(nt)) steps or more, where the snapshots and a segment's records
+take the least memory) and the H records of the run's last steps (by default
 as many as fit in device memory) until the backward has run.
 """
 import math
@@ -55,7 +56,7 @@ class FluxFunction(torch.autograd.Function):
         s = with_structure(setup, w.detach().cpu().numpy())
         eng = Engine(s, precision=precision)
         try:
-            eng.checkpoint(segment or default_segment(s.params.nt), record_bytes)
+            eng.checkpoint(segment or -1, record_bytes)
             eng.run_to(s.params.nt)
             jabs = eng.flux(nobox)
         except Exception:
@@ -83,7 +84,9 @@ class FluxFunction(torch.autograd.Function):
 def flux(w, setup, precision=4, nobox=1, segment=None, record_bytes=-1):
     """Differentiable Jabs (nof, 6): w is the structure's eps_inv as a tensor
     (hf-hi+1, ny, nx), real or complex; precision 4 (FP32 fields, FP64 DFT)
-    or 8; segment = steps between forward snapshots (default ceil(sqrt(nt)));
+    or 8; segment = steps between forward snapshots (default: chosen by the
+    engine, at least ceil(sqrt(nt)), larger where snapshots would crowd the
+    device memory);
     record_bytes = device memory for H records of the run's last steps
     (-1: as much as fits, 0: none)."""
     return FluxFunction.apply(w, setup, precision, nobox, segment, record_bytes)

@@ -250,8 +250,9 @@ class Engine:
     def checkpoint(self, segment, record_bytes=0):
         """Prepare the coming forward run for adjoint(): H of its last steps
         recorded in up to record_bytes of device memory (-1: as much as fits),
-        a snapshot of the state every `segment` steps before them.  Call
-        before the first step; segment <= 0 turns it off."""
+        a snapshot of the state every `segment` steps before them (< 0: the
+        engine chooses, see iris_checkpoint).  Call before the first step;
+        segment = 0 turns it off."""
         self._check(self.lib.iris_checkpoint(self._h, int(segment), int(record_bytes)), "iris_checkpoint")
 
     def checkpoint_info(self):

@@ -345,6 +345,29 @@ def test_recorded_tail(tmp_path, precision):
 
 
 @pytest.mark.gpu
+def test_automatic_snapshot_interval(tmp_path):
+    """checkpoint(-1) picks an interval of at least ceil(sqrt(nt)) and gives
+    the gradient of an explicit interval bit for bit."""
+    import math
+    print("[gpu-test] automatic snapshot interval", flush=True)
+    s = make_setup(tmp_path, "two_step")
+    p = s.params
+    w = weights_for(p)
+    g17, _ = engine_gradient(s, w, 4, 17)
+    with Engine(s, precision=4) as eng:
+        eng.checkpoint(-1, 0)
+        info = eng.checkpoint_info()
+        seg = info["segment"]
+        assert math.ceil(math.sqrt(p.nt)) <= seg <= p.nt
+        want = math.sqrt(p.nt * info["snapshot_bytes"] / info["record_step_bytes"])
+        assert seg == max(math.ceil(math.sqrt(p.nt)), min(p.nt, math.floor(want)))
+        assert info["snapshots"] == -(-p.nt // seg)
+        eng.run_to(p.nt)
+        g = eng.adjoint(1, w)
+    assert np.array_equal(g, g17)
+
+
+@pytest.mark.gpu
 def test_adjoint_call_order(tmp_path):
     print("[gpu-test] adjoint call order", flush=True)
     s = make_setup(tmp_path, "slab_ipol2")
