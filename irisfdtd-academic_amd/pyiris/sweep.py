@@ -14,6 +14,12 @@ sweep: per instance a grid of nx, ny cells drawn from {32..128}, a design
 layer whose eps_r per (x, y) column is drawn uniform in [1, 4] and smoothed by
 a 3-cell periodic box filter in x and y, and an incidence angle drawn uniform
 in [0, 60] degrees.
+
+metasurface() and angle_averaged_gradient() are the pieces of an
+angle-averaged gradient: a structure whose eps_r is drawn per voxel block,
+and the mean over incidence angles of the summed transmission with its
+gradient with respect to the structure's eps_inv (one forward run and one
+adjoint per angle).
 """
 import math
 import os
@@ -78,3 +84,30 @@ def grating_sweep(n=64, seed=3, nz=768, nt=10000, nof=50, layer=30, precision=4,
             out.append(oblique(s, theta))
             drawn.append((nx, ny, theta))
     return out, drawn
+
+
+def metasurface(setup, block=32, eps_range=(1.0, 6.0), seed=5):
+    """Copy of `setup` whose structure eps_r (x 1..nx, y 1..ny, z hi..hf) is
+    drawn uniform in eps_range per block x block x block voxel (seed); the
+    voxels tile the period from its corner."""
+    from .adjoint import structure_of, with_structure
+    nz_s, ny, nx = structure_of(setup).shape
+    rng = np.random.default_rng(seed)
+    nb = [-(-n // block) for n in (nz_s, ny, nx)]
+    eps = rng.uniform(eps_range[0], eps_range[1], size=nb)
+    eps = np.repeat(np.repeat(np.repeat(eps, block, 0), block, 1), block, 2)[:nz_s, :ny, :nx]
+    return with_structure(setup, 1.0 / eps)
+
+
+def angle_averaged_gradient(setup, thetas, precision=4, nobox=1, segment=None, record_bytes=-1):
+    """Mean over the incidence angles thetas (degrees, xz plane, matching
+    Bloch phase) of sum_w T(w), T = J6/(J5+J6) of box nobox, and its gradient
+    with respect to the real eps_inv of the structure cells, (hf-hi+1, ny, nx).
+    Each angle is one forward run and one adjoint (pyiris.adjoint.flux)."""
+    import torch
+    from .adjoint import flux, structure_of, transmission
+    w = torch.tensor(structure_of(setup).real, requires_grad=True)
+    obj = sum(transmission(flux(w, oblique(setup, th), precision, nobox, segment, record_bytes)).sum()
+              for th in thetas) / len(thetas)
+    obj.backward()
+    return float(obj.detach()), w.grad.numpy().copy()

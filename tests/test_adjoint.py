@@ -344,6 +344,64 @@ def test_recorded_tail(tmp_path, precision):
         assert np.array_equal(g, g0), steps
 
 
+def test_metasurface_voxels(tmp_path):
+    """metasurface() draws one eps_r per block^3 voxel, uniform in the range."""
+    from pyiris.sweep import metasurface
+    kw = dict(ixmax=10, iymax=7, izmax=64, pmlzi=8, pmlzf=57, hi=26, dh=6, dx=0, eps_bbox=2.0, tfsf_o=14,
+              box=(1, 10, 1, 7, 12, 48), nof=2, nt=20, no_downloads=20)
+    d = tmp_path / "m"
+    d.mkdir()
+    s = metasurface(load_input(write_input(d, inputs.render(**kw))), block=4, eps_range=(1.0, 6.0), seed=5)
+    eps = 1.0 / structure_of(s).real
+    assert eps.shape == (6, 7, 10)
+    assert eps.min() >= 1.0 and eps.max() <= 6.0
+    for z0 in (0, 4):
+        for y0 in (0, 4):
+            for x0 in (0, 4, 8):
+                blk = eps[z0:z0 + 4, y0:y0 + 4, x0:x0 + 4]
+                assert np.all(blk == blk.flat[0])
+    assert len(np.unique(eps)) == 2 * 2 * 3
+
+
+@pytest.mark.gpu
+def test_angle_averaged_gradient_finite_differences(tmp_path):
+    """Angle-averaged gradient (three incidence angles, each with its Bloch
+    phase) of the summed transmission of a voxel metasurface, against central
+    differences of the averaged objective at two cells (FP64 fields)."""
+    from pyiris.adjoint import structure_of, with_structure
+    from pyiris.sweep import angle_averaged_gradient, metasurface, oblique
+    print("[gpu-test] angle-averaged gradient", flush=True)
+    kw = dict(ixmax=16, iymax=16, izmax=96, pmlzi=10, pmlzf=86, hi=40, dh=8, dx=0, eps_bbox=2.0, tfsf_o=14,
+              box=(1, 16, 1, 16, 14, 80), nof=3, nt=260, no_downloads=260, lno=500.0, dlambda=150.0)
+    d = tmp_path / "meta"
+    d.mkdir()
+    s = metasurface(load_input(write_input(d, inputs.render(**kw))), block=4, seed=5)
+    eps = 1.0 / structure_of(s).real
+    assert eps.min() >= 1.0 and eps.max() <= 6.0 and np.array_equal(eps[0, :4, :4], np.full((4, 4), eps[0, 0, 0]))
+    thetas = (0.0, 20.0, 40.0)
+    obj, g = angle_averaged_gradient(s, thetas, precision=8, record_bytes=0)
+
+    def objective(w):
+        tot = 0.0
+        for th in thetas:
+            with Engine(oblique(with_structure(s, w), th), precision=8) as eng:
+                eng.run_to(s.params.nt)
+                j = eng.flux(1)
+            tot += (j[:, 5] / (j[:, 4] + j[:, 5])).sum()
+        return tot / len(thetas)
+
+    w0 = structure_of(s).real
+    assert abs(objective(w0) - obj) < 1e-12 * abs(obj)
+    h = 1e-4
+    for cell in [(2, 5, 7), (6, 11, 3)]:
+        wp, wm = w0.copy(), w0.copy()
+        wp[cell] += h
+        wm[cell] -= h
+        fd = (objective(wp) - objective(wm)) / (2 * h)
+        assert abs(fd - g[cell]) < 1e-5 * np.abs(g).max() + 1e-4 * abs(fd), (cell, fd, g[cell])
+    assert np.abs(g).max() > 0
+
+
 @pytest.mark.gpu
 def test_automatic_snapshot_interval(tmp_path):
     """checkpoint(-1) picks an interval of at least ceil(sqrt(nt)) and gives
