@@ -34,7 +34,7 @@ namespace {
 
 // samples buffered before folding into emg: 16 with FP64 fields (exact
 // reference order), 32 with FP32 fields (half the accumulator traffic)
-constexpr int kDftSlots64 = 16, kDftSlots32 = 32, kDftSlotsMax = 32;
+constexpr int kDftSlots64 = 16, kDftSlots32 = 64, kDftSlotsMax = 64;
 template <typename T> constexpr int dft_slots() { return sizeof(T) == 8 ? kDftSlots64 : kDftSlots32; }
 
 thread_local std::string g_create_error = "no error";
@@ -661,10 +661,18 @@ int dft_flush(iris_handle* h) {
   for (int b = 0; b < h->p.nbox; ++b) {
     DeviceBox& B = h->box[b];
     if (B.g.ntot == 0) continue;
-    dim3 grid((unsigned)((B.g.ntot + 255) / 256), 1);
     ProfScope ps(h, IRIS_K_DFT_ACCUM);
-    hipLaunchKernelGGL((k_dft_fold<cplx<T>, dft_slots<T>()>), grid, dim3(256), 0, h->stream, B.g,
-                       (const cplx<T>*)B.samp, nk, (const double2*)h->ph, nof, h->p.dt, B.emg);
+    if constexpr (exact) {
+      const dim3 grid((unsigned)((B.g.ntot + 255) / 256), 1);
+      hipLaunchKernelGGL((k_dft_fold<cplx<T>, dft_slots<T>()>), grid, dim3(256), 0, h->stream, B.g,
+                         (const cplx<T>*)B.samp, nk, (const double2*)h->ph, nof, h->p.dt, B.emg);
+    } else {
+      // four waves a workgroup, 16 point-components a wave
+      int64_t tiles = 0;
+      for (int q = 0; q < 3; ++q) tiles += (B.g.npts[q] + 15) / 16;
+      hipLaunchKernelGGL((k_dft_fold_mfma<dft_slots<T>()>), dim3((unsigned)((tiles + 3) / 4)), dim3(256), 0,
+                         h->stream, B.g, (const float2*)B.samp, nk, (const double2*)h->ph, nof, B.emg);
+    }
   }
   HIP_TRY(h, hipGetLastError());
   h->nsamp = 0;
@@ -1224,13 +1232,13 @@ int create_impl(iris_handle* h, const double* eps_inv, const double* fpmn, const
   // DFT
   if ((rc = dalloc(h, (void**)&h->fw, sizeof(double) * p.nof))) return rc;
   HIP_TRY(h, hipMemcpyAsync(h->fw, fw, sizeof(double) * p.nof, hipMemcpyHostToDevice, h->stream));
-  if ((rc = dalloc(h, (void**)&h->ph, sizeof(double2) * (size_t)p.nof * kDftSlotsMax))) return rc;
+  if ((rc = dalloc(h, (void**)&h->ph, sizeof(double2) * (size_t)p.nof * dft_slots<T>()))) return rc;
   if ((rc = dalloc(h, (void**)&h->flux, sizeof(double) * 6 * (size_t)p.nof))) return rc;
   for (int b = 0; b < p.nbox; ++b) {
     DeviceBox& B = h->box[b];
     box_geometry(p, b, B);
     if ((rc = dalloc(h, (void**)&B.emg, sizeof(double2) * (size_t)std::max<int64_t>(B.g.ntot, 1) * p.nof))) return rc;
-    if ((rc = dalloc(h, &B.samp, h->csize * (size_t)std::max<int64_t>(B.g.ntot, 1) * kDftSlotsMax))) return rc;
+    if ((rc = dalloc(h, &B.samp, h->csize * (size_t)std::max<int64_t>(B.g.ntot, 1) * dft_slots<T>()))) return rc;
   }
   // launch shapes: x-rows of up to 64 (one wavefront), 256 threads per block
   const int bx = std::min(64, pow2ceil(p.nx));

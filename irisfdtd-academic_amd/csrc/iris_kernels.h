@@ -110,28 +110,24 @@ __global__ void k_dft_phase(const double* fw, int nof, int nk, int nrows, int n0
   ph[t] = (scale == 1.0) ? make_double2(c, s) : make_double2(scale * c, scale * s);
 }
 
-// emg(i,iw) += dt*F_k(i)*exp(i w n_k) for the buffered samples k < nk.
-// Thread = one point-component; its samples stay in registers for the sweep
-// over all wavelengths, so each sample is read once per fold and each
-// accumulator is read and written once per fold.  The phase table is staged
-// through LDS in chunks of kDftChunk wavelengths: every lane reads the same
-// phase (an LDS broadcast), and the reads do not wait on the L2 latency of
-// scalar loads.  Rows k >= nk of the phase table are zero, so a partial fold
-// runs the same branch-free sweep and the extra terms add exact zeros.
-//
-// FP64 fields (EXACT): f = dt*F in double, then per sample the reference's
-// update acc = acc + f*exp(i w n) with separate multiplies and adds, the
-// running sum in a register -- the reference's rounding sequence.
-// FP32 fields: the float samples themselves stay in registers (two per
-// sample; widening to double is exact), dt is folded into the phase table,
-// and the sum uses FP64 fused multiply-adds.
+// emg(i,iw) += dt*F_k(i)*exp(i w n_k) for the buffered samples k < nk, with
+// FP64 fields.  Thread = one point-component; its samples stay in registers
+// for the sweep over all wavelengths, so each sample is read once per fold
+// and each accumulator is read and written once per fold.  The phase table
+// is staged through LDS in chunks of kDftChunk wavelengths: every lane reads
+// the same phase (an LDS broadcast), and the reads do not wait on the L2
+// latency of scalar loads.  Rows k >= nk of the phase table are zero, so a
+// partial fold runs the same branch-free sweep and the extra terms add exact
+// zeros.  Per sample the reference's update acc = acc + f*exp(i w n) with
+// f = dt*F and separate multiplies and adds, the running sum in a register --
+// the reference's rounding sequence.  (FP32 fields: k_dft_fold_mfma below.)
 constexpr int kDftChunk = 16;
 
 template <typename S, int KMAX>
 __global__ __launch_bounds__(256) void k_dft_fold(BoxGeom g, const S* __restrict__ samp, int nk,
                                                   const double2* __restrict__ ph, int nof, double dt,
                                                   double2* __restrict__ emg) {
-  constexpr bool EXACT = sizeof(S) == 16;
+  static_assert(sizeof(S) == 16, "FP64 samples; FP32 fields fold with k_dft_fold_mfma");
   __shared__ double2 sph[KMAX * kDftChunk];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < g.ntot;
@@ -147,29 +143,8 @@ __global__ __launch_bounds__(256) void k_dft_fold(BoxGeom g, const S* __restrict
 #pragma unroll
   for (int k = 0; k < KMAX; ++k) f[k] = samp[(int64_t)(k < nk ? k : 0) * g.ntot + ic];
 #pragma unroll
-  for (int k = 0; k < KMAX; ++k) {
-    if constexpr (EXACT) {
-      f[k] = (k < nk) ? make_double2(dt * f[k].x, dt * f[k].y) : make_double2(0.0, 0.0);
-    } else {
-      if (k >= nk) f[k] = make_float2(0.f, 0.f);
-    }
-  }
-  auto mac = [&](double2 acc, int k, const double2 p) -> double2 {
-    if constexpr (EXACT) {
-      return cadd(acc, cmul(f[k], p));
-    } else {
-      // widen at the use: the empty asm keeps the compiler from hoisting
-      // 2*KMAX doubles into registers for the whole sweep
-      float2 fk = f[k];
-      asm volatile("" : "+v"(fk.x), "+v"(fk.y));
-      const double fx = (double)fk.x, fy = (double)fk.y;
-      acc.x = fma(fx, p.x, acc.x);
-      acc.x = fma(-fy, p.y, acc.x);
-      acc.y = fma(fx, p.y, acc.y);
-      acc.y = fma(fy, p.x, acc.y);
-      return acc;
-    }
-  };
+  for (int k = 0; k < KMAX; ++k) f[k] = (k < nk) ? make_double2(dt * f[k].x, dt * f[k].y) : make_double2(0.0, 0.0);
+  auto mac = [&](double2 acc, int k, const double2 p) -> double2 { return cadd(acc, cmul(f[k], p)); };
   for (int c0 = 0; c0 < nof; c0 += kDftChunk) {
     const int cn = min(kDftChunk, nof - c0);
     __syncthreads();  // the previous chunk's phases are no longer read
@@ -197,6 +172,91 @@ __global__ __launch_bounds__(256) void k_dft_fold(BoxGeom g, const S* __restrict
 #pragma unroll
       for (int k = 0; k < KMAX; ++k) acc = mac(acc, k, sph[k * kDftChunk + u0]);
       acc_base[(int64_t)(c0 + u0) * np] = acc;
+    }
+  }
+}
+
+// The fold with FP32 samples as an FP64 matrix product on the matrix
+// cores: per pair block, C[w][p] += sum_k P[k][w] * S[p][k] over the
+// buffered samples, complex, written as two real products over the inner
+// index k' = 2k + c (c = 0 real part of S_k, 1 imaginary part):
+//   Re C += B_r A,  Im C += B_i A,  A[k'][p] = S_c[p][k],
+//   B_r[w][k'] = (Re P, -Im P)[c],  B_i[w][k'] = (Im P, Re P)[c].
+// A wave takes 16 point-components p (one pair block; columns past its end
+// are zero) and all wavelengths in groups of 16.  v_mfma_f64_16x16x4f64
+// lanes hold its first operand at [l % 16][l / 16], its second at
+// [l / 16][l % 16] and rows 4r + l / 16 of column l % 16 of the result in
+// register r: the phases are the first operand (staged in LDS per group),
+// the samples the second (in registers for the whole sweep), and a result
+// register of the 16 lanes of one row group is 16 consecutive points of one
+// wavelength, so the accumulators move in contiguous 256-byte pieces.  P
+// carries dt (k_dft_phase with scale dt); slots k >= nk add zero.
+typedef double iris_d4 __attribute__((ext_vector_type(4)));
+
+template <int KMAX>
+__global__ __launch_bounds__(256) void k_dft_fold_mfma(BoxGeom g, const float2* __restrict__ samp, int nk,
+                                                       const double2* __restrict__ ph, int nof,
+                                                       double2* __restrict__ emg) {
+  __shared__ double2 sph[KMAX][16];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int col = lane & 15, grp = lane >> 4;  // point (second operand column), inner index within a step
+  // pair block j and the tile of 16 point-components in it
+  int64_t tl = (int64_t)blockIdx.x * 4 + wave;
+  int j = -1;
+  for (int q = 0; q < 3; ++q) {
+    const int64_t tq = (g.npts[q] + 15) / 16;
+    if (tl < tq) {
+      j = q;
+      break;
+    }
+    tl -= tq;
+  }
+  const bool active = j >= 0;
+  const int64_t np = active ? g.npts[j] : 0;
+  const int64_t p0 = tl * 16;
+  const bool colok = active && p0 + col < np;
+  const int64_t ic = colok ? g.start[j] + p0 + col : 0;
+  double sv[KMAX / 2];
+#pragma unroll
+  for (int q = 0; q < KMAX / 2; ++q) {
+    const int k = 2 * q + (grp >> 1);
+    float v = 0.f;
+    if (colok && k < nk) {
+      const float2 f = samp[(int64_t)k * g.ntot + ic];
+      v = (grp & 1) ? f.y : f.x;
+    }
+    sv[q] = (double)v;
+  }
+  double2* const cbase = active ? emg + g.start[j] * (int64_t)nof + p0 + col : emg;
+  for (int c0 = 0; c0 < nof; c0 += 16) {
+    __syncthreads();  // the previous group's phases are no longer read
+    for (int t = threadIdx.x; t < KMAX * 16; t += blockDim.x) {
+      const int k = t >> 4, u = t & 15;
+      sph[k][u] = (c0 + u < nof) ? ph[k * nof + c0 + u] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    if (!active) continue;
+    iris_d4 cr, ci;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int w = c0 + 4 * r + grp;  // this register's wavelength
+      double2 v = make_double2(0.0, 0.0);
+      if (colok && w < nof) v = cbase[(int64_t)w * np];
+      cr[r] = v.x;
+      ci[r] = v.y;
+    }
+#pragma unroll
+    for (int q = 0; q < KMAX / 2; ++q) {
+      const double2 P = sph[2 * q + (grp >> 1)][col];  // first operand row = wavelength c0 + col
+      const double br = (grp & 1) ? -P.y : P.x;
+      const double bi = (grp & 1) ? P.x : P.y;
+      cr = __builtin_amdgcn_mfma_f64_16x16x4f64(br, sv[q], cr, 0, 0, 0);
+      ci = __builtin_amdgcn_mfma_f64_16x16x4f64(bi, sv[q], ci, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int w = c0 + 4 * r + grp;
+      if (colok && w < nof) cbase[(int64_t)w * np] = make_double2(cr[r], ci[r]);
     }
   }
 }
