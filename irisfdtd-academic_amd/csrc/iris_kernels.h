@@ -193,15 +193,15 @@ __global__ __launch_bounds__(256) void k_dft_fold(BoxGeom g, const S* __restrict
 // carries dt (k_dft_phase with scale dt); slots k >= nk add zero.
 typedef double iris_d4 __attribute__((ext_vector_type(4)));
 
-template <int KMAX>
-__global__ __launch_bounds__(256) void k_dft_fold_mfma(BoxGeom g, const float2* __restrict__ samp, int nk,
+template <int KMAX, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_dft_fold_mfma(BoxGeom g, const float2* __restrict__ samp, int nk,
                                                        const double2* __restrict__ ph, int nof,
                                                        double2* __restrict__ emg) {
   __shared__ double2 sph[KMAX][16];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 15, grp = lane >> 4;  // point (second operand column), inner index within a step
   // pair block j and the tile of 16 point-components in it
-  int64_t tl = (int64_t)blockIdx.x * 4 + wave;
+  int64_t tl = (int64_t)blockIdx.x * WAVES + wave;
   int j = -1;
   for (int q = 0; q < 3; ++q) {
     const int64_t tq = (g.npts[q] + 15) / 16;
