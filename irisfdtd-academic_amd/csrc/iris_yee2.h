@@ -177,9 +177,16 @@ __device__ __forceinline__ void yee2_tile(const StepArgs<T>& a, cplx<T>* __restr
   using C = cplx<T>;
   constexpr int kT2R = Yee2Shape<T, PML, kT2W>::RY, kT2OX = Yee2Shape<T, PML, kT2W>::OX,
                 kT2OY = Yee2Shape<T, PML, kT2W>::OY;
-  __shared__ C s1[2][3][kT2R][kT2W];  // E1 of planes z (z & 1) and z-1
-  __shared__ C sh[3][kT2R][kT2W];  // H1 of plane z-1
-  __shared__ C s2[3][kT2R][kT2W];  // E2 of plane z-1
+  // LDS planes by the parity of z.  With FP32 fields H1 and E2 are double
+  // buffered as E1 is, and one barrier a plane orders every reuse: a slot
+  // written at plane z is read after that plane's barrier and written again
+  // at plane z+2, after plane z+1's barrier, which follows all of plane z's
+  // reads.  FP64 fields (twice the bytes) keep single H1 and E2 slots and a
+  // second barrier before E2 is rewritten.
+  constexpr int NB = sizeof(T) == 4 ? 2 : 1;
+  __shared__ C s1[2][3][kT2R][kT2W];   // E1 of planes z (z & 1) and z-1
+  __shared__ C sh[NB][3][kT2R][kT2W];  // H1 of plane z-1
+  __shared__ C s2[NB][3][kT2R][kT2W];  // E2 of plane z-1
   // footprint cell (cx, cy) of this lane: one row per wave, or two
   const int cx = threadIdx.x % kT2W, cy = threadIdx.y * (kT2X / kT2W) + threadIdx.x / kT2W;
   const int nx = a.nx, ny = a.ny;
@@ -278,15 +285,16 @@ __device__ __forceinline__ void yee2_tile(const StepArgs<T>& a, cplx<T>* __restr
     s1[pz][0][cy][cx] = e1[0];
     s1[pz][1][cy][cx] = e1[1];
     s1[pz][2][cy][cx] = e1[2];
-    sh[0][cy][cx] = h1n[0];
-    sh[1][cy][cx] = h1n[1];
-    sh[2][cy][cx] = h1n[2];
+    const int bz = NB == 2 ? pz : 0, bm = NB == 2 ? pm : 0;  // H1 / E2 slots of this plane and the last
+    sh[bz][0][cy][cx] = h1n[0];
+    sh[bz][1][cy][cx] = h1n[1];
+    sh[bz][2][cy][cx] = h1n[2];
     __syncthreads();
     // ---- E2(z-1) from E1(z-1), H1(z-1) (own and LDS -x/-y), H1(z-2)
     C e2n[3] = {}, d2n[3] = {};
     if (z >= z0 + 1) {
-      const C h3ym = sh[2][wm][cx], h1ym = sh[0][wm][cx];
-      const C h3xm = sh[2][cy][lm], h2xm = sh[1][cy][lm];
+      const C h3ym = sh[bz][2][wm][cx], h1ym = sh[bz][0][wm][cx];
+      const C h3xm = sh[bz][2][cy][lm], h2xm = sh[bz][1][cy][lm];
       const C c[3] = {cadd(csub(csub(h1n[2], h3ym), h1n[1]), h1pp[1]),
                       cadd(csub(csub(h1n[0], h1pp[0]), h1n[2]), h3xm),
                       cadd(csub(csub(h1n[1], h2xm), h1n[0]), h1ym)};
@@ -299,8 +307,8 @@ __device__ __forceinline__ void yee2_tile(const StepArgs<T>& a, cplx<T>* __restr
     // ---- H2(z-2) from H1(z-2), E2(z-2) (own and LDS +x/+y), E2(z-1)
     C h2n[3] = {}, b2n[3] = {};
     if (z >= z0 + 2) {
-      const C e3yp = s2[2][wp][cx], e1yp = s2[0][wp][cx];
-      const C e3xp = s2[2][cy][lp], e2xp = s2[1][cy][lp];
+      const C e3yp = s2[bm][2][wp][cx], e1yp = s2[bm][0][wp][cx];
+      const C e3xp = s2[bm][2][cy][lp], e2xp = s2[bm][1][cy][lp];
       const C c0 = cadd(csub(csub(e3yp, e2p[2]), e2n[1]), e2p[1]);
       const C c1 = cadd(csub(csub(e2n[0], e2p[0]), e3xp), e2p[2]);
       const C c2 = cadd(csub(csub(e2xp, e2p[1]), e1yp), e2p[0]);
@@ -313,10 +321,10 @@ __device__ __forceinline__ void yee2_tile(const StepArgs<T>& a, cplx<T>* __restr
         h2n[2] = csub(h1pp[2], cscale(a.dtcq2, c2));
       }
     }
-    __syncthreads();  // the E2(z-2) neighbours and H1(z-1) neighbours have been read
-    s2[0][cy][cx] = e2n[0];
-    s2[1][cy][cx] = e2n[1];
-    s2[2][cy][cx] = e2n[2];
+    if constexpr (NB == 1) __syncthreads();  // the E2(z-2) and H1(z-1) neighbours have been read
+    s2[bz][0][cy][cx] = e2n[0];
+    s2[bz][1][cy][cx] = e2n[1];
+    s2[bz][2][cy][cx] = e2n[2];
 
     if (own) {
       // step n+2 of the chunk's planes, with H's min-side ghost planes
