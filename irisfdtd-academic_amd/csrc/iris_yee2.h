@@ -18,8 +18,9 @@
 // copies the reference writes).  The workgroup marches z with a pipeline:
 // at plane z it forms E1(z), H1(z-1), E2(z-1), H2(z-2) (1: step n+1,
 // 2: step n+2); the z-1 and z-2 values of each lane stay in registers, and the
-// lateral neighbours come through LDS planes (E1 double-buffered, H1, E2)
-// with two barriers per plane.
+// lateral neighbours come through LDS planes: E1, H1 and E2 double-buffered
+// by plane parity with one barrier per plane (FP32 fields), or E1 double and
+// H1, E2 single with two barriers per plane (FP64 fields).
 //
 // Besides E2/H2 (and the H min-side ghost planes) of its chunk it writes:
 //   - E1 and H1 (with H's min-side ghosts) of its first plane when the chunk
@@ -35,8 +36,8 @@ namespace iris {
 
 constexpr int kT2X = 64;  // lanes per wave (the workgroup's x dimension)
 // Waves per workgroup (footprint rows) and the waves per SIMD the register
-// allocation must allow: FP32 fields 16 rows, one workgroup per CU (98 KB of
-// LDS); FP64 fields 12 rows, one workgroup per CU (147 KB of LDS).
+// allocation must allow: FP32 fields 16 rows, FP64 fields 12 rows, one
+// workgroup per CU (144 KB of LDS either way).
 // The UPML variant carries D and B through the pipeline as well (twelve
 // more complex registers per lane) and uses 12 / 8 rows.
 // W = footprint cells per tile row: 64 (one row per wave) or 32 (two rows
