@@ -35,9 +35,9 @@ namespace {
 // samples buffered before folding into emg: 16 with FP64 fields (exact
 // reference order), 32 with FP32 fields (half the accumulator traffic)
 constexpr int kDftSlots64 = 16, kDftSlots32 = 64, kDftSlotsMax = 64;
-// waves per workgroup of the matrix-core fold (4, 8 and 16 measured within
-// 2%: 16 share each staged group of phases among the most tiles)
-constexpr int kFoldWaves = 16;
+// waves per workgroup of the matrix-core fold and the wavelength groups (16
+// each) whose phases it stages at once (KMAX x 16 x 16 bytes a group)
+constexpr int kFoldWaves = 16, kFoldGroups = 8;
 template <typename T> constexpr int dft_slots() { return sizeof(T) == 8 ? kDftSlots64 : kDftSlots32; }
 
 thread_local std::string g_create_error = "no error";
@@ -674,7 +674,7 @@ int dft_flush(iris_handle* h) {
       // each staged group of phases)
       int64_t tiles = 0;
       for (int q = 0; q < 3; ++q) tiles += (B.g.npts[q] + 15) / 16;
-      hipLaunchKernelGGL((k_dft_fold_mfma<dft_slots<T>(), kFoldWaves>),
+      hipLaunchKernelGGL((k_dft_fold_mfma<dft_slots<T>(), kFoldWaves, kFoldGroups>),
                          dim3((unsigned)((tiles + kFoldWaves - 1) / kFoldWaves)), dim3(64 * kFoldWaves), 0, h->stream,
                          B.g, (const float2*)B.samp, nk, (const double2*)h->ph, nof, B.emg);
     }

@@ -191,13 +191,21 @@ __global__ __launch_bounds__(256) void k_dft_fold(BoxGeom g, const S* __restrict
 // register of the 16 lanes of one row group is 16 consecutive points of one
 // wavelength, so the accumulators move in contiguous 256-byte pieces.  P
 // carries dt (k_dft_phase with scale dt); slots k >= nk add zero.
+//
+// The phases of up to GMAX wavelength groups are staged at once, so the
+// workgroup passes a barrier once per GMAX groups, not twice per group, and
+// its waves drift apart: each wave sweeps the staged groups starting at its
+// own (wave mod groups), and loads the next group's accumulators before the
+// matrix products of the current one.  While some waves wait on accumulator
+// loads, others keep the matrix cores busy.
 typedef double iris_d4 __attribute__((ext_vector_type(4)));
 
-template <int KMAX, int WAVES>
+template <int KMAX, int WAVES, int GMAX>
 __global__ __launch_bounds__(64 * WAVES) void k_dft_fold_mfma(BoxGeom g, const float2* __restrict__ samp, int nk,
                                                        const double2* __restrict__ ph, int nof,
                                                        double2* __restrict__ emg) {
-  __shared__ double2 sph[KMAX][16];
+  static_assert((KMAX & (KMAX - 1)) == 0, "KMAX is a power of two");
+  __shared__ double2 sph[GMAX][KMAX][16];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 15, grp = lane >> 4;  // point (second operand column), inner index within a step
   // pair block j and the tile of 16 point-components in it
@@ -228,35 +236,52 @@ __global__ __launch_bounds__(64 * WAVES) void k_dft_fold_mfma(BoxGeom g, const f
     sv[q] = (double)v;
   }
   double2* const cbase = active ? emg + g.start[j] * (int64_t)nof + p0 + col : emg;
-  for (int c0 = 0; c0 < nof; c0 += 16) {
-    __syncthreads();  // the previous group's phases are no longer read
-    for (int t = threadIdx.x; t < KMAX * 16; t += blockDim.x) {
-      const int k = t >> 4, u = t & 15;
-      sph[k][u] = (c0 + u < nof) ? ph[k * nof + c0 + u] : make_double2(0.0, 0.0);
-    }
-    __syncthreads();
-    if (!active) continue;
-    iris_d4 cr, ci;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int w = c0 + 4 * r + grp;  // this register's wavelength
-      double2 v = make_double2(0.0, 0.0);
-      if (colok && w < nof) v = cbase[(int64_t)w * np];
-      cr[r] = v.x;
-      ci[r] = v.y;
-    }
-#pragma unroll
-    for (int q = 0; q < KMAX / 2; ++q) {
-      const double2 P = sph[2 * q + (grp >> 1)][col];  // first operand row = wavelength c0 + col
-      const double br = (grp & 1) ? -P.y : P.x;
-      const double bi = (grp & 1) ? P.x : P.y;
-      cr = __builtin_amdgcn_mfma_f64_16x16x4f64(br, sv[q], cr, 0, 0, 0);
-      ci = __builtin_amdgcn_mfma_f64_16x16x4f64(bi, sv[q], ci, 0, 0, 0);
-    }
+  // accumulators of wavelength group c0 (register r: wavelength c0 + 4r + grp)
+  auto load = [&](int c0, double2 v[4]) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int w = c0 + 4 * r + grp;
-      if (colok && w < nof) cbase[(int64_t)w * np] = make_double2(cr[r], ci[r]);
+      v[r] = (colok && w < nof) ? cbase[(int64_t)w * np] : make_double2(0.0, 0.0);
+    }
+  };
+  const int ng = (nof + 15) / 16;
+  for (int g0 = 0; g0 < ng; g0 += GMAX) {
+    const int gn = min(GMAX, ng - g0);
+    if (g0) __syncthreads();  // the previous block's phases are no longer read
+    for (int t = threadIdx.x; t < gn * KMAX * 16; t += blockDim.x) {
+      const int gi = t / (KMAX * 16), k = (t >> 4) & (KMAX - 1), u = t & 15;
+      const int c = (g0 + gi) * 16 + u;
+      sph[gi][k][u] = (c < nof) ? ph[k * nof + c] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    if (!active) continue;
+    int gi = wave % gn;
+    double2 nx[4];
+    load((g0 + gi) * 16, nx);
+    for (int s = 0; s < gn; ++s) {
+      const int c0 = (g0 + gi) * 16;
+      iris_d4 cr, ci;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        cr[r] = nx[r].x;
+        ci[r] = nx[r].y;
+      }
+      const int gnext = gi + 1 == gn ? 0 : gi + 1;
+      if (s + 1 < gn) load((g0 + gnext) * 16, nx);
+#pragma unroll
+      for (int q = 0; q < KMAX / 2; ++q) {
+        const double2 P = sph[gi][2 * q + (grp >> 1)][col];  // first operand row = wavelength c0 + col
+        const double br = (grp & 1) ? -P.y : P.x;
+        const double bi = (grp & 1) ? P.x : P.y;
+        cr = __builtin_amdgcn_mfma_f64_16x16x4f64(br, sv[q], cr, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f64_16x16x4f64(bi, sv[q], ci, 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int w = c0 + 4 * r + grp;
+        if (colok && w < nof) cbase[(int64_t)w * np] = make_double2(cr[r], ci[r]);
+      }
+      gi = gnext;
     }
   }
 }
