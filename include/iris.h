@@ -231,6 +231,19 @@ int iris_pass_info(const iris_handle* h, int64_t info[6]);
  * the two-step pass is off). */
 int iris_autotune(iris_handle* h, int32_t steps, int64_t info[9]);
 
+/* Steps as a captured HIP graph.  iris_step runs whole periods of steps --
+ * P = ring slots x sample_fourier (64 x with FP32 fields, 16 x with FP64):
+ * the DFT sample ring fills and folds once and the field buffers return to
+ * their assignment -- as replays of one captured graph of the period's
+ * launches, where the period starts with an empty ring at or after
+ * to_fourier and a whole period fits in the call.  Results are bitwise those
+ * of the launches one by one.  Not while profiling or recording snapshots;
+ * IRIS_GRAPH=0 at iris_create turns it off.  info[0] = 1 when a graph is
+ * captured, info[1] = replays so far, info[2] = P.  No counterpart in the
+ * reference (its loop, code/driver.f90:338-500, calls the step routines
+ * one by one). */
+int iris_graph_info(const iris_handle* h, int64_t info[3]);
+
 /* ---- Gradient of a face-flux objective.  The reference has no adjoint;
  * the differentiable form stands beside the plain solver on the same handle.
  * Its forward run is the plain one (iris_step), so the flux it returns is

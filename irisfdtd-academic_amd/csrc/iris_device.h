@@ -96,6 +96,9 @@ struct TfsfConst {
   double q, ux, uz, zopump, ko, dk, sq, sign_kz, c0dt, amp, hamp, win;
   double ein, dtcq2, eps_ms;
   int ixo, iz_min, i_pol, cosine, mode2d, on;
+  // steps added to every step index the kernels are given: 0, except while a
+  // captured graph of steps replays at a later step than it was captured at
+  int shift;
 };
 
 }  // namespace iris

@@ -183,6 +183,17 @@ struct iris_handle {
   DeviceBox box[2];
   int nsamp = 0;            // samples waiting in the ring
   int samp_n0 = 0;          // DFT phase index of the first waiting sample
+  // steps as a captured HIP graph: one period of steps (the sample ring
+  // fills once, the buffers come back to their assignment) captured at step
+  // g_it0 and replayed at later steps with the TF/SF step shift set
+  bool graph_on = true;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int g_it0 = 0;
+  void* g_key[12] = {};     // buffer assignment the graph was captured with
+  void* g_warm[12] = {};    // assignment whose period last ran eagerly (tables made)
+  bool g_warmed = false;
+  int64_t g_replays = 0;
   int it = 0;
   int zchunk = 1;
   dim3 blk_yee, grd_yee, blk_tf, grd_tf;
@@ -660,7 +671,7 @@ int dft_flush(iris_handle* h) {
   constexpr bool exact = sizeof(T) == 8;
   const int nrows = dft_slots<T>();
   hipLaunchKernelGGL(k_dft_phase, dim3((nof * nrows + 255) / 256), dim3(256), 0, h->stream, h->fw, nof, nk, nrows,
-                     h->samp_n0, h->p.sample_fourier, exact ? 1.0 : h->p.dt, h->ph);
+                     h->samp_n0, h->p.sample_fourier, exact ? 1.0 : h->p.dt, h->ph, (const TfsfConst*)h->tfc);
   for (int b = 0; b < h->p.nbox; ++b) {
     DeviceBox& B = h->box[b];
     if (B.g.ntot == 0) continue;
@@ -701,9 +712,130 @@ int dft_sample(iris_handle* h, int n) {
   return 0;
 }
 
+// ---- steps as a captured HIP graph -----------------------------------------
+// The launches of one period of P steps, P = ring slots x sample cadence (the
+// sample ring fills and folds exactly once, and the rotating field buffers
+// come back to the assignment they started from), are captured once and
+// replayed.  Step indices reach the kernels as launch arguments; a replay
+// s steps after the capture runs with the handle's TF/SF step shift set to
+// s, which the TF/SF source and the DFT phases add to their step index.  The
+// first period at a buffer assignment runs eagerly: it makes the per-
+// assignment argument tables (an upload a capture cannot hold).
+
+void graph_release(iris_handle* h) {
+  if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
+  h->gexec = nullptr;
+  h->graph = nullptr;
+  h->g_warmed = false;
+}
+
+void buffer_key(const iris_handle* h, void* key[12]) {
+  void* k[12] = {h->fe, h->fh, h->ge, h->gh, h->ke, h->kh, h->d, h->b, h->dn, h->bn, h->kd, h->kb};
+  std::copy(k, k + 12, key);
+}
+
+template <typename T> int graph_period(const iris_handle* h) { return dft_slots<T>() * h->p.sample_fourier; }
+
+// Whether the P steps from `it` may go through the graph: a whole period
+// fits before the call's last step (and, with two-step pairs, a single step
+// still follows), the ring is empty, every step of it samples on cadence,
+// and nothing records per-step timing or snapshots.
+template <typename T>
+bool graph_fits(const iris_handle* h, int it, int it_last) {
+  const int P = graph_period<T>(h);
+  return h->graph_on && !h->prof && h->ck_seg == 0 && h->nsamp == 0 && it >= h->p.to_fourier &&
+         it_last - it + 1 >= P + (h->two_step ? 1 : 0);
+}
+
+// P steps from `it` by the eager launches (under capture: their recording)
+template <typename T>
+int period_steps(iris_handle* h, int it, const PassLists& L) {
+  const int P = graph_period<T>(h);
+  int rc;
+  if (h->two_step) {
+    for (int k = 0; k < P; k += 2)
+      if ((rc = launch_double<T>(h, it + k, L, true))) return rc;
+    return 0;
+  }
+  for (int k = 0; k < P; ++k) {
+    const int s = it + k;
+    if ((rc = launch_step<T>(h, s))) return rc;
+    if (s >= h->p.to_fourier && (s % h->p.sample_fourier) == 0 &&
+        (rc = dft_sample<T>(h, s + 1 - h->p.to_fourier)))
+      return rc;
+    h->it = s;
+  }
+  return 0;
+}
+
+// Steps it .. it+P-1: a replay of the captured period, or its capture and
+// first launch, or an eager period that prepares the capture.  *shifted is
+// set when the step shift was left non-zero.
+template <typename T>
+int graph_period_run(iris_handle* h, int it, const PassLists& L, bool* shifted) {
+  const int P = graph_period<T>(h);
+  void* key[12];
+  buffer_key(h, key);
+  if (h->gexec && std::equal(key, key + 12, h->g_key) && (it - h->g_it0) % h->p.sample_fourier == 0) {
+    hipLaunchKernelGGL(k_set_shift, dim3(1), dim3(64), 0, h->stream, h->tfc, it - h->g_it0);
+    HIP_TRY(h, hipGetLastError());
+    HIP_TRY(h, hipGraphLaunch(h->gexec, h->stream));
+    *shifted = true;
+    h->it = it + P - 1;
+    h->g_replays++;
+    return 0;
+  }
+  if (!(h->g_warmed && std::equal(key, key + 12, h->g_warm))) {
+    std::copy(key, key + 12, h->g_warm);
+    h->g_warmed = true;
+    return period_steps<T>(h, it, L);
+  }
+  // capture; the host state advances as the steps are recorded, so it is
+  // restored if the capture fails (nothing has run)
+  graph_release(h);
+  if (*shifted) {
+    hipLaunchKernelGGL(k_set_shift, dim3(1), dim3(64), 0, h->stream, h->tfc, 0);
+    *shifted = false;
+  }
+  const int nsamp = h->nsamp, n0 = h->samp_n0, it_prev = h->it;
+  HIP_TRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  int rc = period_steps<T>(h, it, L);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(h->stream, &g);
+  hipGraphExec_t x = nullptr;
+  hipError_t ei = hipSuccess;
+  if (!rc && e == hipSuccess) ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  if (rc || e != hipSuccess || ei != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    h->fe = key[0]; h->fh = key[1]; h->ge = key[2]; h->gh = key[3]; h->ke = key[4]; h->kh = key[5];
+    h->d = key[6]; h->b = key[7]; h->dn = key[8]; h->bn = key[9]; h->kd = key[10]; h->kb = key[11];
+    h->nsamp = nsamp;
+    h->samp_n0 = n0;
+    h->it = it_prev;
+    if (rc) return rc;
+    return fail(h, IRIS_EDEVICE, "graph capture of steps %d..%d: %s", it, it + P - 1,
+                hipGetErrorString(e != hipSuccess ? e : ei));
+  }
+  h->graph = g;
+  h->gexec = x;
+  h->g_it0 = it;
+  std::copy(key, key + 12, h->g_key);
+  HIP_TRY(h, hipGraphLaunch(h->gexec, h->stream));
+  return 0;
+}
+
 template <typename T>
 int run_steps(iris_handle* h, int it_first, int it_last) {
   int it = it_first;
+  bool shifted = false;  // the TF/SF step shift is non-zero (reset before eager steps)
+  auto unshift = [&]() -> int {
+    if (!shifted) return 0;
+    hipLaunchKernelGGL(k_set_shift, dim3(1), dim3(64), 0, h->stream, h->tfc, 0);
+    shifted = false;
+    HIP_TRY(h, hipGetLastError());
+    return 0;
+  };
   // two steps per pass while at least three steps remain: every call ends
   // with a single-step pass, so the previous step's E and H stay whole
   PassLists L;
@@ -717,12 +849,24 @@ int run_steps(iris_handle* h, int it_first, int it_last) {
   L.n2p = h->n2p;
   L.groups = h->rest_groups.data();
   L.ngroups = (int)h->rest_groups.size();
+  while (graph_fits<T>(h, it, it_last)) {
+    int rc = graph_period_run<T>(h, it, L, &shifted);
+    if (rc) return rc;
+    it += graph_period<T>(h);
+  }
+  if (int rc = unshift()) return rc;
   while (h->two_step && it_last - it + 1 >= 3) {
+    if (graph_fits<T>(h, it, it_last)) {
+      // the ring has just emptied: the rest of the call in periods
+      int rc = run_steps<T>(h, it, it_last);
+      return rc;
+    }
     int rc = launch_double<T>(h, it, L, true);
     if (rc) return rc;
     it += 2;
   }
   for (; it <= it_last; ++it) {
+    if (graph_fits<T>(h, it, it_last)) return run_steps<T>(h, it, it_last);
     int rc = launch_step<T>(h, it);
     if (rc) return rc;
     if (it >= h->p.to_fourier && (it % h->p.sample_fourier) == 0) {
@@ -938,6 +1082,7 @@ int build_passes(iris_handle* h, const PassConfig& c) {
   const iris_params& p = h->p;
   int rc;
   HIP_TRY(h, hipStreamSynchronize(h->stream));
+  graph_release(h);  // captured with the old lists
   h->pass = c;
   h->xcd_rows = c.xcd_rows;
   h->xcd_rows2 = c.xcd_rows2;
@@ -1372,7 +1517,7 @@ int adjoint_step(iris_handle* h, int n, int nobox, int face_on, const void* rec)
   // 1. sources: the DFT sample taken after step n
   if (n >= p.to_fourier && (n % p.sample_fourier) == 0 && face_on) {
     hipLaunchKernelGGL(k_dft_phase, dim3((p.nof + 255) / 256), dim3(256), 0, h->stream, h->fw, p.nof, 1, 1,
-                       n + 1 - p.to_fourier, p.sample_fourier, 1.0, h->ph);
+                       n + 1 - p.to_fourier, p.sample_fourier, 1.0, h->ph, nullptr);
     const DeviceBox& B = h->box[nobox - 1];
     for (int j = 0; j < 3; ++j)
       for (int s = 0; s < 2; ++s) {
@@ -1432,7 +1577,7 @@ void adj_sources(iris_handle* h, const StepArgs<T>& a, int n, int nobox, int fac
   const iris_params& p = h->p;
   if (!(n >= p.to_fourier && (n % p.sample_fourier) == 0 && face_on)) return;
   hipLaunchKernelGGL(k_dft_phase, dim3((p.nof + 255) / 256), dim3(256), 0, h->stream, h->fw, p.nof, 1, 1,
-                     n + 1 - p.to_fourier, p.sample_fourier, 1.0, h->ph);
+                     n + 1 - p.to_fourier, p.sample_fourier, 1.0, h->ph, nullptr);
   const DeviceBox& B = h->box[nobox - 1];
   const int lo[3] = {B.g.bxi, B.g.byi, B.g.bzi}, hi[3] = {B.g.bxf, B.g.byf, B.g.bzf};
   for (int j = 0; j < 3; ++j) {
@@ -2379,6 +2524,7 @@ void batch_release(iris_batch* b) {
 void release(iris_handle* h) {
   if (!h) return;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  graph_release(h);
   prof_drain(h);
   for (hipEvent_t e : h->pool) (void)hipEventDestroy(e);
   h->pool.clear();
@@ -2431,6 +2577,8 @@ int iris_create(const iris_params* p, const double* eps_inv, const double* fpzmn
     const char* tyenv = std::getenv("IRIS_ADJ_TY");
     if (tyenv && (std::atoi(tyenv) == 7 || std::atoi(tyenv) == 8 || std::atoi(tyenv) == 15))
       h->adj_ty = std::atoi(tyenv);
+    const char* genv = std::getenv("IRIS_GRAPH");
+    h->graph_on = !(genv && genv[0] == '0');
   }
   (void)hipGetDevice(&h->device);
   if (hipDeviceGetAttribute(&h->ncu, hipDeviceAttributeMultiprocessorCount, h->device) != hipSuccess || h->ncu < 1)
@@ -2766,6 +2914,14 @@ int iris_pass_info(const iris_handle* h, int64_t info[6]) {
   info[3] = h->two_step ? 1 : 0;
   info[4] = h->two_step ? h->planes2p : 0;
   info[5] = h->two_step ? h->upml2p : 0;
+  return 0;
+}
+
+int iris_graph_info(const iris_handle* h, int64_t info[3]) {
+  if (!h || !info) return fail(nullptr, IRIS_EINVAL, "handle or info is NULL");
+  info[0] = h->gexec ? 1 : 0;
+  info[1] = h->g_replays;
+  info[2] = (int64_t)(h->prec == 8 ? kDftSlots64 : kDftSlots32) * h->p.sample_fourier;
   return 0;
 }
 

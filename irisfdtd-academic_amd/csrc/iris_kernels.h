@@ -96,9 +96,10 @@ __global__ __launch_bounds__(256) void k_dft_sample_batch(const BatchSamp<T>* __
 // scale is 1 for the exact fold and dt for the fused-multiply-add fold.
 // Rows nk..nrows-1 are zero.
 __global__ void k_dft_phase(const double* fw, int nof, int nk, int nrows, int n0, int stride, double scale,
-                            double2* ph) {
+                            double2* ph, const TfsfConst* __restrict__ tp) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nof * nrows) return;
+  if (tp) n0 += tp->shift;  // sample indices count steps
   const int k = t / nof, iw = t % nof;
   if (k >= nk) {
     ph[t] = make_double2(0.0, 0.0);
@@ -108,6 +109,11 @@ __global__ void k_dft_phase(const double* fw, int nof, int nk, int nrows, int n0
   double s, c;
   sincos(arg, &s, &c);
   ph[t] = (scale == 1.0) ? make_double2(c, s) : make_double2(scale * c, scale * s);
+}
+
+// Sets the step shift of a handle's TF/SF constants (one lane).
+__global__ void k_set_shift(TfsfConst* tp, int shift) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) tp->shift = shift;
 }
 
 // emg(i,iw) += dt*F_k(i)*exp(i w n_k) for the buffered samples k < nk, with

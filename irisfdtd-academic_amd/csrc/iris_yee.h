@@ -204,7 +204,7 @@ __global__ __launch_bounds__(256) void k_hpass(StepArgs<T> a) {
 
 // incident values of comps {1,3,5} (i_pol=2) or {2,4,6} (i_pol=1) at (ix, iz)
 __device__ inline void tfsf_incident(const TfsfConst& t, int it, int ix, int iz, double2* inc) {
-  const double c0dt_it = t.c0dt * (double)it;
+  const double c0dt_it = t.c0dt * (double)(it + t.shift);
   const double A = (t.q * (double)(ix - t.ixo)) * t.ux + (t.q * ((double)iz - t.zopump)) * t.uz;
   const double kzz = t.ko * ((t.sq * t.sign_kz) * A - c0dt_it);
   const double tt = (A - c0dt_it / t.sq) / t.dk;

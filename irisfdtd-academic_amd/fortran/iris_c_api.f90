@@ -100,6 +100,12 @@ module iris_c_api
       type(c_ptr), value :: info
     end function
 
+    integer(c_int) function iris_graph_info(h, info) bind(C, name="iris_graph_info")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      type(c_ptr), value :: info
+    end function
+
     integer(c_int) function iris_checkpoint_info(h, info) bind(C, name="iris_checkpoint_info")
       import :: c_int, c_ptr
       type(c_ptr), value :: h

@@ -97,6 +97,8 @@ def load_library(path=None):
     lib.iris_pass_info.restype = ct.c_int
     lib.iris_autotune.argtypes = [H, ct.c_int32, ct.c_void_p]
     lib.iris_autotune.restype = ct.c_int
+    lib.iris_graph_info.argtypes = [H, ct.c_void_p]
+    lib.iris_graph_info.restype = ct.c_int
     lib.iris_checkpoint.argtypes = [H, ct.c_int32, ct.c_int64]
     lib.iris_checkpoint.restype = ct.c_int
     lib.iris_checkpoint_info.argtypes = [H, ct.c_void_p]
@@ -132,7 +134,8 @@ def load_library(path=None):
 EXPORTED = ["iris_create", "iris_step", "iris_sync", "iris_flux", "iris_dft_points", "iris_get_dft",
             "iris_get_field", "iris_set_field", "iris_get_plane", "iris_diagnostics", "iris_current_step",
             "iris_destroy", "iris_last_error", "iris_version", "iris_profile", "iris_profile_read",
-            "iris_pass_info", "iris_autotune", "iris_checkpoint", "iris_checkpoint_info", "iris_adjoint",
+            "iris_pass_info", "iris_autotune", "iris_graph_info", "iris_checkpoint", "iris_checkpoint_info",
+            "iris_adjoint",
             "iris_batch_create", "iris_batch_step", "iris_batch_info", "iris_batch_destroy", "iris_batch_last_error"]
 
 KERNELS = ["epass", "hpass", "tfsf", "dft_sample", "dft_accum", "flux", "yee", "yee2", "yee_rest", "adjoint",
@@ -235,6 +238,13 @@ class Engine:
         self._check(self.lib.iris_pass_info(self._h, _p(v)), "iris_pass_info")
         return dict(zchunk=int(v[0]), two_step_planes=int(v[1]), two_step_chunks=int(v[2]), two_step=bool(v[3]),
                     upml_two_step_planes=int(v[4]), upml_planes_in_two_step=int(v[5]))
+
+    def graph_info(self):
+        """{captured, replays, period}: steps run as replays of a captured
+        HIP graph of one period of `period` steps."""
+        v = np.zeros(3, dtype=np.int64)
+        self._check(self.lib.iris_graph_info(self._h, _p(v)), "iris_graph_info")
+        return dict(captured=bool(v[0]), replays=int(v[1]), period=int(v[2]))
 
     def autotune(self, steps=8):
         """Time candidate layouts of the step passes for `steps` steps each and

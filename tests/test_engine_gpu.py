@@ -487,15 +487,17 @@ def test_two_step_pass_with_bloch_phase(tmp_path, name, precision, width):
                 assert _nrel(j2, jo) < 1e-10
 
 
-def test_fp32_dft_many_wavelengths_against_fp64(tmp_path):
+@pytest.mark.parametrize("nof", [37, 150])
+def test_fp32_dft_many_wavelengths_against_fp64(tmp_path, nof):
     """The FP32 fields' fold (matrix cores, 64 samples a fold, wavelengths
-    in groups of 16): 37 wavelengths (a partial last group) and 150 sampled
-    steps (a partial last fold) give the FP64 run's DFT accumulators and
-    fluxes to FP32 field accuracy."""
-    _announce("fp32 dft many wavelengths")
+    in groups of 16, the phases of up to 8 groups staged at once): 37
+    wavelengths (a partial last group) or 150 (two staging blocks, the second
+    partial) and 150 sampled steps (a partial last fold) give the FP64 run's
+    DFT accumulators and fluxes to FP32 field accuracy."""
+    _announce(f"fp32 dft many wavelengths nof={nof}")
     s = load_input(write_input(tmp_path, inputs.render(
         ixmax=70, iymax=21, izmax=96, pmlzi=10, pmlzf=86, hi=44, dh=10, dx=8, eps_bbox=2.25, tfsf_o=20,
-        box=(3, 66, 2, 20, 15, 80), nof=37, dlambda=10.0, lno=450.0, nt=150, no_downloads=150)))
+        box=(3, 66, 2, 20, 15, 80), nof=nof, dlambda=1500.0 / nof, lno=450.0, nt=150, no_downloads=150)))
     out = {}
     for prec in (4, 8):
         with Engine(s, precision=prec) as eng:
@@ -503,7 +505,7 @@ def test_fp32_dft_many_wavelengths_against_fp64(tmp_path):
             out[prec] = (eng.flux(1), {j: eng.dft(1, j) for j in (1, 2, 3)})
     for j in (1, 2, 3):
         a, b = out[4][1][j], out[8][1][j]
-        assert a.shape == b.shape and a.shape[0] == 37
+        assert a.shape == b.shape and a.shape[0] == nof
         assert _nrel(a, b) < 2e-5, (j, _nrel(a, b))
     assert _nrel(out[4][0], out[8][0]) < 2e-5
     assert np.abs(out[8][0]).max() > 0

@@ -1,0 +1,81 @@
+"""Steps as a captured HIP graph (iris_graph_info): whole periods of steps
+(the DFT sample ring fills and folds once, the field buffers return to their
+assignment) run as replays of one captured graph, with the TF/SF step shift
+set per replay.  A run with graphs must equal the run with the launches one
+by one (IRIS_GRAPH=0) bit for bit, whatever the calls' step ranges, with
+eager work and synchronisations between the replays."""
+import numpy as np
+import pytest
+
+from conftest import write_input
+from pyiris import Engine, inputs, load_input
+from pyiris.sweep import oblique
+
+pytestmark = pytest.mark.gpu
+
+
+def make(tmp_path, name, **kw):
+    base = dict(nt=700, no_downloads=700, tfsf_o=20, lno=450.0, dlambda=100.0, nof=3)
+    base.update(kw)
+    d = tmp_path / name
+    d.mkdir()
+    return load_input(write_input(d, inputs.render(**base)))
+
+
+def setups(tmp_path):
+    two = make(tmp_path, "two", ixmax=70, iymax=21, izmax=96, pmlzi=10, pmlzf=86, hi=44, dh=10, dx=8,
+               eps_bbox=2.25, box=(1, 70, 1, 21, 15, 80))
+    return {
+        "two_step": two,
+        "two_step_bloch": oblique(two, 30.0),
+        "two_step_cadence": make(tmp_path, "cad", ixmax=70, iymax=21, izmax=96, pmlzi=10, pmlzf=86, hi=44, dh=10,
+                                 dx=8, eps_bbox=2.25, box=(1, 70, 1, 21, 15, 80), to_fourier=37, sample_fourier=3,
+                                 nt=1400, no_downloads=1400),
+        "single": make(tmp_path, "one", ixmax=20, iymax=12, izmax=80, pmlzi=9, pmlzf=71, hi=36, dh=8, dx=6,
+                       eps_bbox=3.0, box=(1, 20, 1, 12, 14, 66), mbox=(2, 19, 2, 11, 20, 60)),
+    }
+
+
+def run(s, precision, cuts):
+    out = []
+    with Engine(s, precision=precision) as eng:
+        a = 1
+        for c in cuts + [s.params.nt]:
+            eng.step(a, c)
+            out.append(eng.flux(1))          # synchronises between the calls
+            a = c + 1
+        out += [eng.field(0), eng.field(1)] + [eng.dft(1, j) for j in (1, 2, 3)]
+        if s.params.nbox == 2:
+            out.append(eng.flux(2))
+        info = eng.graph_info()
+    return out, info
+
+
+@pytest.mark.parametrize("case", ["two_step", "two_step_bloch", "two_step_cadence", "single"])
+@pytest.mark.parametrize("precision", [8, 4])
+def test_graph_replays_are_bitwise_the_launches(tmp_path, monkeypatch, case, precision):
+    print(f"[gpu-test] graph {case} precision={precision}", flush=True)
+    s = setups(tmp_path)[case]
+    cuts = [250, 251, 333]
+    monkeypatch.setenv("IRIS_GRAPH", "0")
+    ref, info0 = run(s, precision, cuts)
+    assert not info0["captured"] and info0["replays"] == 0
+    monkeypatch.setenv("IRIS_GRAPH", "1")
+    got, info = run(s, precision, cuts)
+    assert info["period"] == (64 if precision == 4 else 16) * s.params.sample_fourier
+    assert info["captured"] and info["replays"] >= 2, info
+    for k, (a, b) in enumerate(zip(got, ref)):
+        assert np.array_equal(a, b), k
+    assert np.abs(ref[0]).max() > 0
+
+
+def test_no_graph_while_profiling(tmp_path):
+    print("[gpu-test] graph off while profiling", flush=True)
+    s = setups(tmp_path)["two_step"]
+    with Engine(s, precision=4) as eng:
+        eng.profile(True)
+        eng.run_to(s.params.nt)
+        assert eng.graph_info()["replays"] == 0
+    with Engine(s, precision=4) as eng:
+        eng.run_to(s.params.nt)
+        assert eng.graph_info()["replays"] >= 1
