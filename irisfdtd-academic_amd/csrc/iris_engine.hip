@@ -33,8 +33,9 @@ using namespace iris;
 namespace {
 
 // samples buffered before folding into emg: 16 with FP64 fields (exact
-// reference order), 32 with FP32 fields (half the accumulator traffic)
-constexpr int kDftSlots64 = 16, kDftSlots32 = 64, kDftSlotsMax = 64;
+// reference order), 64 with FP32 fields (the matrix-core fold: a quarter of
+// the accumulator traffic)
+constexpr int kDftSlots64 = 16, kDftSlots32 = 64;
 // waves per workgroup of the matrix-core fold and the wavelength groups (16
 // each) whose phases it stages at once (KMAX x 16 x 16 bytes a group)
 constexpr int kFoldWaves = 16, kFoldGroups = 8;

@@ -79,3 +79,42 @@ def test_no_graph_while_profiling(tmp_path):
     with Engine(s, precision=4) as eng:
         eng.run_to(s.params.nt)
         assert eng.graph_info()["replays"] >= 1
+
+
+def _interplay(s, precision, tune, batch_with, poke):
+    """Steps 1..nt with a tuner call first, an excursion through a batch, and
+    a field write between calls, as chosen; returns the end state."""
+    from pyiris import Batch
+    with Engine(s, precision=precision) as eng:
+        if tune:
+            eng.autotune(4)
+        eng.step(1, 300)
+        if poke:
+            e = eng.field(0)
+            e[s.params.nz // 2, 3, 4, 0] += 1e-3           # any change the later steps carry
+            eng.set_field(0, e)
+        if batch_with is not None:
+            with Engine(batch_with, precision=precision) as other:
+                other.run_to(300)
+                with Batch([eng, other]) as b:
+                    b.step(301, 340)
+        eng.step(eng.it + 1, s.params.nt)
+        return [eng.flux(1), eng.field(0), eng.field(1)], eng.graph_info()
+
+
+@pytest.mark.parametrize("what", ["autotune", "batch", "set_field"])
+def test_graph_with_tuner_batch_and_field_writes(tmp_path, monkeypatch, what):
+    """A graph captured before iris_autotune's new layout, before a batch
+    ran the handle, or before a field write is replaced or replayed correctly:
+    bitwise the launches one by one."""
+    print(f"[gpu-test] graph interplay {what}", flush=True)
+    ss = setups(tmp_path)
+    s, other = ss["two_step"], ss["two_step_bloch"]
+    kw = dict(tune=what == "autotune", batch_with=other if what == "batch" else None, poke=what == "set_field")
+    monkeypatch.setenv("IRIS_GRAPH", "0")
+    ref, _ = _interplay(s, 4, **kw)
+    monkeypatch.setenv("IRIS_GRAPH", "1")
+    got, info = _interplay(s, 4, **kw)
+    assert info["replays"] >= 1, info
+    for k, (a, b) in enumerate(zip(got, ref)):
+        assert np.array_equal(a, b), k
