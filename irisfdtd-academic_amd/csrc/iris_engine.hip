@@ -474,11 +474,13 @@ int dft_flush(iris_handle* h);
 template <typename T, bool PH, bool PML>
 void launch_yee2(const iris_handle* h, const StepArgs<T>& a, cplx<T>* eb, cplx<T>* hb, cplx<T>* db,
                  const int* chunks, const TileMap& tm, const Yee2Samples<T>& smp) {
-  const dim3 grid((unsigned)tm.nblk), blk(kT2X, Yee2Shape<T, PML>::R);
+  const dim3 grid((unsigned)tm.nblk);
   if (h->t2w == 32)
-    hipLaunchKernelGGL((k_yee2<T, PH, PML, 32>), grid, blk, 0, h->stream, a, eb, hb, db, chunks, tm, smp);
+    hipLaunchKernelGGL((k_yee2<T, PH, PML, 32>), grid, dim3(kT2X, Yee2Shape<T, PML, 32>::R), 0, h->stream, a, eb,
+                       hb, db, chunks, tm, smp);
   else
-    hipLaunchKernelGGL((k_yee2<T, PH, PML, 64>), grid, blk, 0, h->stream, a, eb, hb, db, chunks, tm, smp);
+    hipLaunchKernelGGL((k_yee2<T, PH, PML, 64>), grid, dim3(kT2X, Yee2Shape<T, PML, 64>::R), 0, h->stream, a, eb,
+                       hb, db, chunks, tm, smp);
 }
 
 // Argument table of the handle's single-step length groups for its current
@@ -1058,6 +1060,7 @@ void dfree(iris_handle* h, void** p) {
 // 64 x 16 tiles: nx >= 32, ny >= 8).
 struct T2Geom {
   int W, R, RY, OX, OY;
+  int per_cu;  // workgroups of the kernel a CU holds at once
   int tx(int nx) const { return (nx + OX - 1) / OX; }
   int ty(int ny) const { return (ny + OY - 1) / OY; }
 };
@@ -1065,7 +1068,13 @@ template <typename T>
 T2Geom t2geom(int W, bool pml) {
   T2Geom g;
   g.W = W;
-  g.R = pml ? Yee2Shape<T, true>::R : Yee2Shape<T>::R;
+  if (W == 32) {
+    g.R = pml ? Yee2Shape<T, true, 32>::R : Yee2Shape<T, false, 32>::R;
+    g.per_cu = pml ? Yee2Shape<T, true, 32>::WG_PER_CU : Yee2Shape<T, false, 32>::WG_PER_CU;
+  } else {
+    g.R = pml ? Yee2Shape<T, true, 64>::R : Yee2Shape<T, false, 64>::R;
+    g.per_cu = pml ? Yee2Shape<T, true, 64>::WG_PER_CU : Yee2Shape<T, false, 64>::WG_PER_CU;
+  }
   g.RY = g.R * kT2X / W;
   g.OX = W - 3;
   g.OY = g.RY - 3;
@@ -1150,10 +1159,11 @@ int build_passes(iris_handle* h, const PassConfig& c) {
     // step).  Measured on MI355X with 768-plane grids, the two-step pass is
     // slower only where this ratio is above 1 (one 64-wide row of tiles
     // against two).
-    // tile width: 64 cells a row by default, or 32 (two rows per wave,
-    // twice the rows; iris_autotune tries it); the other where only that one
-    // fits.  32-cell tiles measured 6% slower at 512 x 512 x 1024 and within
-    // run-to-run noise (+-4%) at 128 x 128 x 768 and on the 64-grating sweep.
+    // tile width: 64 cells a row by default, or 32 (two rows per wave;
+    // iris_autotune tries it); the other where only that one fits.  With
+    // FP32 fields the 32-wide tiles (8 waves, two workgroups a CU) measured
+    // within noise of the 64-wide ones at 512 x 512 x 1024 and 12% faster at
+    // 128 x 128 x 768; on the 64-grating sweep, batched, 3% slower.
     int W = (c.w2 == 32 || c.w2 == 64) ? c.w2 : 64;
     if (!two_step_fits(p, t2geom<T>(W, false)) && two_step_fits(p, t2geom<T>(96 - W, false))) W = 96 - W;
     h->t2w = W;
@@ -1236,7 +1246,8 @@ int build_passes(iris_handle* h, const PassConfig& c) {
     auto rounds = [&](const std::vector<int>& runs, int L) {
       int64_t nch = 0;
       for (int r : runs) nch += (r + L - 1) / L;
-      return (t2 * nch + h->ncu - 1) / h->ncu;
+      const int64_t slots = (int64_t)h->ncu * g2.per_cu;
+      return (t2 * nch + slots - 1) / slots;
     };
     if (c.zc2 > 0) {
       zc2 = std::min(c.zc2, kChunkLenMask + 1);
@@ -2314,11 +2325,12 @@ int batch_sample2(iris_batch* b, int half, int par, int n) {
 template <typename T, bool PH, bool PML>
 void launch_yee2_batch(const iris_batch* b, int W, unsigned nblk, const Batch2Inst<T>* I, const int* wg, int n,
                        int slot) {
-  const dim3 blk(kT2X, Yee2Shape<T, PML>::R);
   if (W == 32)
-    hipLaunchKernelGGL((k_yee2_batch<T, PH, PML, 32>), dim3(nblk), blk, 0, b->stream, I, wg, n, slot);
+    hipLaunchKernelGGL((k_yee2_batch<T, PH, PML, 32>), dim3(nblk), dim3(kT2X, Yee2Shape<T, PML, 32>::R), 0,
+                       b->stream, I, wg, n, slot);
   else
-    hipLaunchKernelGGL((k_yee2_batch<T, PH, PML, 64>), dim3(nblk), blk, 0, b->stream, I, wg, n, slot);
+    hipLaunchKernelGGL((k_yee2_batch<T, PH, PML, 64>), dim3(nblk), dim3(kT2X, Yee2Shape<T, PML, 64>::R), 0,
+                       b->stream, I, wg, n, slot);
 }
 
 // steps it_first..it_last of the two-step handles: pairs of steps while at

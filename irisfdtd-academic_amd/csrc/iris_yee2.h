@@ -41,13 +41,18 @@ constexpr int kT2X = 64;  // lanes per wave (the workgroup's x dimension)
 // The UPML variant carries D and B through the pipeline as well (twelve
 // more complex registers per lane) and uses 12 / 8 rows.
 // W = footprint cells per tile row: 64 (one row per wave) or 32 (two rows
-// per wave, so twice the rows: fewer halo cells per output on small grids).
+// per wave).  The 32-wide plain tile with FP32 fields has 8 waves (16 rows,
+// 72 KB of LDS), so two workgroups share a CU and one computes while the
+// other waits at its barrier; the other 32-wide tiles keep the wave count of
+// the 64-wide ones (twice the rows: fewer halo cells per output).
 template <typename T, bool PML = false, int W = 64> struct Yee2Shape {
-  static constexpr int R = sizeof(T) == 4 ? (PML ? 12 : 16) : (PML ? 8 : 12);  // waves
+  static constexpr bool HALF = sizeof(T) == 4 && !PML && W == 32;
+  static constexpr int R = sizeof(T) == 4 ? (PML ? 12 : (HALF ? 8 : 16)) : (PML ? 8 : 12);  // waves
   static constexpr int RY = R * kT2X / W;  // footprint rows
   static constexpr int OX = W - 3;         // output cells per tile row
   static constexpr int OY = RY - 3;        // output rows per tile
-  static constexpr int WAVES_PER_EU = (R + 3) / 4;
+  static constexpr int WG_PER_CU = HALF ? 2 : 1;
+  static constexpr int WAVES_PER_EU = (R * WG_PER_CU + 3) / 4;
 };
 
 // Index in the concatenated face-point list of box g (inverse of box_point,
@@ -389,7 +394,7 @@ __device__ __forceinline__ void yee2_tile(const StepArgs<T>& a, cplx<T>* __restr
 }
 
 template <typename T, bool PH, bool PML = false, int W = 64>
-__global__ __launch_bounds__((kT2X * Yee2Shape<T, PML>::R), (Yee2Shape<T, PML>::WAVES_PER_EU)) void k_yee2(
+__global__ __launch_bounds__((kT2X * Yee2Shape<T, PML, W>::R), (Yee2Shape<T, PML, W>::WAVES_PER_EU)) void k_yee2(
     StepArgs<T> a, cplx<T>* __restrict__ eb, cplx<T>* __restrict__ hb, cplx<T>* __restrict__ db,
     const int* __restrict__ chunks, TileMap tm, Yee2Samples<T> smp) {
   int bx, by, bz;
@@ -415,7 +420,7 @@ template <typename T> struct Batch2Inst {
 };
 
 template <typename T, bool PH, bool PML, int W>
-__global__ __launch_bounds__((kT2X * Yee2Shape<T, PML>::R), (Yee2Shape<T, PML>::WAVES_PER_EU)) void k_yee2_batch(
+__global__ __launch_bounds__((kT2X * Yee2Shape<T, PML, W>::R), (Yee2Shape<T, PML, W>::WAVES_PER_EU)) void k_yee2_batch(
     const Batch2Inst<T>* __restrict__ inst, const int* __restrict__ wg_start, int ninst, int slot) {
   const int blk = blockIdx.x;
   const int i = batch_instance(wg_start, ninst, blk);
