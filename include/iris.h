@@ -238,8 +238,10 @@ int iris_autotune(iris_handle* h, int32_t steps, int64_t info[9]);
  * launches, where the period starts with an empty ring at or after
  * to_fourier and a whole period fits in the call.  Results are bitwise those
  * of the launches one by one.  Not while profiling or recording snapshots;
- * IRIS_GRAPH=0 at iris_create turns it off.  info[0] = 1 when a graph is
- * captured, info[1] = replays so far, info[2] = P.  No counterpart in the
+ * IRIS_GRAPH=0 at iris_create turns it off.  iris_batch_step does the same
+ * for the pairs of steps of its two-step handles, with one graph per batch.
+ * info[0] = 1 when the handle has a graph captured, info[1] = replays so far
+ * that ran the handle's steps (its own graph's and a batch's), info[2] = P.  No counterpart in the
  * reference (its loop, code/driver.f90:338-500, calls the step routines
  * one by one). */
 int iris_graph_info(const iris_handle* h, int64_t info[3]);

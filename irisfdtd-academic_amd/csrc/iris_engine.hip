@@ -284,6 +284,15 @@ struct iris_batch {
   int64_t* ent2_start = nullptr;
   int nent2 = 0;
   int64_t npts2 = 0;
+  // the pairs of one period of steps as a captured HIP graph (as a handle
+  // alone does): captured at step g_it0 at pair parity 0 and replayed there
+  // with every two-step handle's TF/SF step shift set.  The launches read
+  // the batch's tables, which every iris_batch_step rewrites in place for
+  // the handles' buffers at that call, when the pair parity starts at 0.
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int g_it0 = 0;
+  int64_t g_replays = 0;
 };
 
 namespace {
@@ -2352,10 +2361,12 @@ int batch_run2(iris_batch* b, int it_first, int it_last) {
                            (const int*)b->rwg[1], b->nr, k, it_step);
     }
   };
-  while (it_last - it + 1 >= 3) {
+  // one pair of steps from `it` (updates the pair parity)
+  auto pair = [&](int it) -> int {
     const bool s1 = it >= p.to_fourier && (it % p.sample_fourier) == 0;
     const bool s2 = it + 1 >= p.to_fourier && ((it + 1) % p.sample_fourier) == 0;
     const int slot = s1 ? b->own[0]->nsamp : -1;
+    int rc;
     rest(0, it);
     for (int bl = 0; bl < 4; ++bl) {
       const int n = (int)b->own2[bl].size();
@@ -2379,8 +2390,70 @@ int batch_run2(iris_batch* b, int it_first, int it_last) {
     if (s1 && (rc = batch_sample2<T>(b, 0, 0, it + 1 - p.to_fourier))) return rc;
     if (s2 && (rc = batch_sample2<T>(b, 1, pp, it + 2 - p.to_fourier))) return rc;
     pp ^= 1;
+    return 0;
+  };
+  // periods of pairs as graph replays where they fit (see iris_batch)
+  const int P = dft_slots<T>() * p.sample_fourier;
+  bool graph_ok = b->own[0]->graph_on;
+  for (iris_handle* h : b->own) graph_ok = graph_ok && !h->prof;
+  bool shifted = false;
+  auto set_shift = [&](int s) -> int {
+    for (iris_handle* h : b->own) hipLaunchKernelGGL(k_set_shift, dim3(1), dim3(64), 0, b->stream, h->tfc, s);
+    if (hipGetLastError() != hipSuccess) return bfail(b, IRIS_EDEVICE, "step shift launch failed");
+    shifted = s != 0;
+    return 0;
+  };
+  while (it_last - it + 1 >= 3) {
+    if (graph_ok && pp == 0 && b->own[0]->nsamp == 0 && it >= p.to_fourier && it_last - it + 1 >= P + 1) {
+      if (b->gexec && (it - b->g_it0) % p.sample_fourier == 0) {
+        if ((rc = set_shift(it - b->g_it0))) return rc;
+        if (hipGraphLaunch(b->gexec, b->stream) != hipSuccess) return bfail(b, IRIS_EDEVICE, "hipGraphLaunch failed");
+        b->g_replays++;
+        for (iris_handle* h : b->own) h->g_replays++;
+        it += P;
+        continue;
+      }
+      // capture (the host state advances as the pairs are recorded; it is
+      // restored if the capture fails, since nothing has run)
+      if (shifted && (rc = set_shift(0))) return rc;
+      if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+      if (b->graph) (void)hipGraphDestroy(b->graph);
+      b->gexec = nullptr;
+      b->graph = nullptr;
+      std::vector<std::pair<int, int>> saved;
+      for (iris_handle* h : b->own) saved.push_back({h->nsamp, h->samp_n0});
+      if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+        return bfail(b, IRIS_EDEVICE, "hipStreamBeginCapture failed");
+      int crc = 0;
+      for (int k = 0; k < P && !crc; k += 2) crc = pair(it + k);
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(b->stream, &g);
+      hipGraphExec_t x = nullptr;
+      hipError_t ei = hipSuccess;
+      if (!crc && e == hipSuccess) ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+      if (crc || e != hipSuccess || ei != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        pp = 0;
+        for (size_t i = 0; i < saved.size(); ++i) {
+          b->own[i]->nsamp = saved[i].first;
+          b->own[i]->samp_n0 = saved[i].second;
+        }
+        if (crc) return crc;
+        return bfail(b, IRIS_EDEVICE, "graph capture of batched steps %d..%d: %s", it, it + P - 1,
+                     hipGetErrorString(e != hipSuccess ? e : ei));
+      }
+      b->graph = g;
+      b->gexec = x;
+      b->g_it0 = it;
+      if (hipGraphLaunch(b->gexec, b->stream) != hipSuccess) return bfail(b, IRIS_EDEVICE, "hipGraphLaunch failed");
+      it += P;
+      continue;
+    }
+    if (shifted && (rc = set_shift(0))) return rc;
+    if ((rc = pair(it))) return rc;
     it += 2;
   }
+  if (shifted && (rc = set_shift(0))) return rc;
   for (iris_handle* h : b->own) {
     if (pp) {
       std::swap(h->fe, h->ke);
@@ -2526,6 +2599,10 @@ int batch_step_impl(iris_batch* b, int it_first, int it_last) {
 void batch_release(iris_batch* b) {
   if (!b) return;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->gexec = nullptr;
+  b->graph = nullptr;
   for (void* q : b->allocs) (void)hipFree(q);
   b->allocs.clear();
   if (b->done) (void)hipEventDestroy(b->done);

@@ -118,3 +118,42 @@ def test_graph_with_tuner_batch_and_field_writes(tmp_path, monkeypatch, what):
     assert info["replays"] >= 1, info
     for k, (a, b) in enumerate(zip(got, ref)):
         assert np.array_equal(a, b), k
+
+
+def _batch_run(group, precision, cuts):
+    from pyiris import Batch
+    engines = [Engine(s, precision=precision) for s in group]
+    try:
+        out = []
+        with Batch(engines) as b:
+            a = 1
+            for c in cuts + [group[0].params.nt]:
+                b.step(a, c)
+                out.append([e.flux(1) for e in engines])     # synchronises between the calls
+                a = c + 1
+        for e in engines:
+            out.append([e.field(0), e.field(1)])
+        return out, [e.graph_info() for e in engines], [e.pass_info()["two_step"] for e in engines]
+    finally:
+        for e in engines:
+            e.close()
+
+
+@pytest.mark.parametrize("precision", [8, 4])
+def test_batch_graph_replays_are_bitwise_the_launches(tmp_path, monkeypatch, precision):
+    """A batch's pairs of steps as replays of one captured graph per batch
+    equal the launches one by one, across calls cut at arbitrary steps."""
+    print(f"[gpu-test] batch graph precision={precision}", flush=True)
+    ss = setups(tmp_path)
+    group = [ss["two_step"], ss["two_step_bloch"], ss["single"]]
+    cuts = [250, 251, 333]
+    monkeypatch.setenv("IRIS_GRAPH", "0")
+    ref, info0, _ = _batch_run(group, precision, cuts)
+    assert all(i["replays"] == 0 for i in info0)
+    monkeypatch.setenv("IRIS_GRAPH", "1")
+    got, info, two = _batch_run(group, precision, cuts)
+    assert two[0] and two[1]
+    assert all(i["replays"] >= 1 for i, t in zip(info, two) if t), info
+    for k, (ra, ga) in enumerate(zip(ref, got)):
+        for a, b in zip(ra, ga):
+            assert np.array_equal(a, b), k
