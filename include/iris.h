@@ -218,7 +218,7 @@ int iris_pass_info(const iris_handle* h, int64_t info[6]);
 /* Layout tuner (no counterpart in the reference).  Before the first step,
  * times `steps` steps (2..1000) of candidate layouts of the step passes --
  * chunk lengths, the two-step pass and its UPML variant on or off, the
- * two-step tile width, the dealing of tiles to XCDs -- and keeps the
+ * tile widths of both, the dealing of tiles to XCDs -- and keeps the
  * fastest.  Only layouts with the
  * same results bit for bit are compared (with a Bloch phase the choice of the
  * two-step pass stays as it is).  The field state is restored and the handle

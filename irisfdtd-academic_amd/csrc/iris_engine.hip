@@ -102,6 +102,7 @@ struct PassConfig {
   int xcd_rows = 0;   // tile rows per XCD run of the single-step pass (0: identity order)
   int xcd_rows2 = 4;  // the same for the two-step pass
   int w2 = 0;         // two-step tile width, 64 or 32 cells (0: 64)
+  int w2p = 0;        // the same for its UPML variant (0: 32)
   int nt2 = -1;       // two-step pass streams its outputs past the L2: 0 no, 1 yes, -1 up to 160 x 160 cells a plane
 };
 
@@ -161,6 +162,8 @@ struct iris_handle {
   int zc2 = 0;              // planes per plain two-step chunk
   int ncu = 256;            // compute units of the device
   int t2w = 64;             // two-step tile width (footprint cells per row)
+  int t2wp = 64;            // the same for its UPML variant
+  int tx2p = 0;             // tile columns of the UPML variant
   int nt2 = 0;              // two-step pass: streaming stores of its outputs
   PassConfig pass;          // the layout the lists were built for
   // single steps of the two-step layout's other chunks when they come in
@@ -272,10 +275,11 @@ struct iris_batch {
   // the two-step handles (`own`), batched: pairs of steps by one launch per
   // kernel over all of them.  Tables per pair parity (the pair runs from the
   // current buffers to the third ones, the next pair back).
-  std::vector<iris_handle*> own2[4];  // by kernel: Bloch phase (bit 0), 32-cell tiles (bit 1)
-  void* y2[4][2] = {};                // Batch2Inst<T>[2][n]: [group][upml variant], pair parity outer
-  int* y2wg[4][2] = {};               // workgroup prefixes
-  int y2nwg[4][2] = {};
+  // by kernels: Bloch phase (bit 0), 32-cell tiles of the plain pass (bit 1) and of its UPML variant (bit 2)
+  std::vector<iris_handle*> own2[8];
+  void* y2[8][2] = {};                // Batch2Inst<T>[2][n]: [group][upml variant], pair parity outer
+  int* y2wg[8][2] = {};               // workgroup prefixes
+  int y2nwg[8][2] = {};
   void* rinst = nullptr;              // BatchInst<T>[2][nr]: their single-step chunks, per (handle, length group)
   int nr = 0;
   int* rwg[2] = {nullptr, nullptr};   // per class (lean, general)
@@ -484,7 +488,7 @@ template <typename T, bool PH, bool PML>
 void launch_yee2(const iris_handle* h, const StepArgs<T>& a, cplx<T>* eb, cplx<T>* hb, cplx<T>* db,
                  const int* chunks, const TileMap& tm, const Yee2Samples<T>& smp) {
   const dim3 grid((unsigned)tm.nblk);
-  if (h->t2w == 32)
+  if ((PML ? h->t2wp : h->t2w) == 32)
     hipLaunchKernelGGL((k_yee2<T, PH, PML, 32>), grid, dim3(kT2X, Yee2Shape<T, PML, 32>::R), 0, h->stream, a, eb,
                        hb, db, chunks, tm, smp);
   else
@@ -592,7 +596,7 @@ int launch_double(iris_handle* h, int it, const PassLists& L, bool dft) {
     // the UPML variant: D and B of step n+2 into the third aux buffers
     ProfScope ps(h, IRIS_K_YEE2_PML);
     const StepArgs<T> a2 = step_args_on<T>(h, h->fe, h->fh, h->ke, h->kh, h->d, h->b, h->kd, h->kb);
-    const TileMap tm{h->tx2, h->ty2p, h->tx2 * h->ty2p * L.n2p, h->tx2 * h->xcd_rows2};
+    const TileMap tm{h->tx2p, h->ty2p, h->tx2p * h->ty2p * L.n2p, h->tx2p * h->xcd_rows2};
     if (h->bloch)
       launch_yee2<T, true, true>(h, a2, (C*)h->ge, (C*)h->gh, (C*)h->dn, L.chunks2p, tm, smp);
     else
@@ -1043,6 +1047,7 @@ PassConfig pass_config_env(const iris_params& p) {
   num("IRIS_XCD_ROWS", &c.xcd_rows, 0);
   num("IRIS_XCD_ROWS2", &c.xcd_rows2, 0);
   num("IRIS_TILE_W2", &c.w2, 0);
+  num("IRIS_TILE_W2P", &c.w2p, 0);
   {
     const char* e = std::getenv("IRIS_NT_STORE2");
     if (e && (e[0] == '0' || e[0] == '1')) c.nt2 = e[0] - '0';
@@ -1176,7 +1181,15 @@ int build_passes(iris_handle* h, const PassConfig& c) {
     int W = (c.w2 == 32 || c.w2 == 64) ? c.w2 : 64;
     if (!two_step_fits(p, t2geom<T>(W, false)) && two_step_fits(p, t2geom<T>(96 - W, false))) W = 96 - W;
     h->t2w = W;
-    const T2Geom g2 = t2geom<T>(W, false), g2p = t2geom<T>(W, true);
+    // the UPML variant's own width, 32 by default: its slabs' launch is a
+    // few rounds of workgroups, and with FP32 fields 32-wide tiles of 12
+    // waves cover a 512 x 512 plane in 450 workgroups against 513 (two slabs:
+    // 4 rounds on 256 CUs instead of 5).  Measured at 512 x 512 x 1024: 13-16%
+    // faster with FP32 fields, 14% with FP64.
+    int Wp = (c.w2p == 32 || c.w2p == 64) ? c.w2p : 32;
+    if (!two_step_fits(p, t2geom<T>(Wp, true)) && two_step_fits(p, t2geom<T>(96 - Wp, true))) Wp = 96 - Wp;
+    h->t2wp = Wp;
+    const T2Geom g2 = t2geom<T>(W, false), g2p = t2geom<T>(Wp, true);
     bool want = c.two != 0;
     if (c.two < 0) {
       const double two = (double)g2.tx(p.nx) * g2.ty(p.ny) * g2.R / 2.0;
@@ -1347,6 +1360,7 @@ int build_passes(iris_handle* h, const PassConfig& c) {
       h->n2p = (int)c2p.size();
       h->tx2 = g2.tx(p.nx);
       h->ty2 = g2.ty(p.ny);
+      h->tx2p = g2p.tx(p.nx);
       h->ty2p = g2p.ty(p.ny);
       dfree(h, (void**)&h->chunks2);
       dfree(h, (void**)&h->chunks2p);
@@ -2050,6 +2064,14 @@ int autotune_impl(iris_handle* h, int steps, int64_t info[9]) {
       if (v != w0) consider(c);
     }
     if (!rc) rc = build_passes<T>(h, best);
+    const int wp0 = h->t2wp;
+    if (!rc && h->upml2p > 0)
+      for (int v : {64, 32}) {
+        PassConfig c = best;
+        c.w2p = v;
+        if (v != wp0) consider(c);
+      }
+    if (!rc) rc = build_passes<T>(h, best);
     const int n0 = h->nt2;
     if (!rc) {
       PassConfig c = best;
@@ -2185,7 +2207,7 @@ template <typename T>
 int batch_tables2(iris_batch* b) {
   using C = cplx<T>;
   if (b->own.empty()) return 0;
-  for (int bl = 0; bl < 4; ++bl) {
+  for (int bl = 0; bl < 8; ++bl) {
     const std::vector<iris_handle*>& hs = b->own2[bl];
     const int n = (int)hs.size();
     if (!n) continue;
@@ -2203,8 +2225,8 @@ int batch_tables2(iris_batch* b) {
           B.hb = (C*)h->gh;
           B.db = pm ? (C*)h->dn : nullptr;
           B.chunks = pm ? h->chunks2p : h->chunks2;
-          const int ty = pm ? h->ty2p : h->ty2;
-          B.tm = TileMap{h->tx2, ty, h->tx2 * ty * (pm ? h->n2p : h->n2), h->tx2 * h->xcd_rows2};
+          const int tx = pm ? h->tx2p : h->tx2, ty = pm ? h->ty2p : h->ty2;
+          B.tm = TileMap{tx, ty, tx * ty * (pm ? h->n2p : h->n2), tx * h->xcd_rows2};
           B.smp = Yee2Samples<T>{};
           B.smp.nbox = h->p.nbox;
           for (int bx = 0; bx < h->p.nbox; ++bx) {
@@ -2264,14 +2286,15 @@ int batch_tables2(iris_batch* b) {
 template <typename T>
 int batch_create2(iris_batch* b) {
   int rc;
-  for (iris_handle* h : b->own) b->own2[(h->bloch ? 1 : 0) + (h->t2w == 32 ? 2 : 0)].push_back(h);
-  for (int bl = 0; bl < 4; ++bl) {
+  for (iris_handle* h : b->own)
+    b->own2[(h->bloch ? 1 : 0) + (h->t2w == 32 ? 2 : 0) + (h->t2wp == 32 ? 4 : 0)].push_back(h);
+  for (int bl = 0; bl < 8; ++bl) {
     const std::vector<iris_handle*>& hs = b->own2[bl];
     if (hs.empty()) continue;
     for (int pm = 0; pm < 2; ++pm) {
       std::vector<int> wg = {0};
       for (iris_handle* h : hs) {
-        const int64_t nb = (int64_t)h->tx2 * (pm ? h->ty2p : h->ty2) * (pm ? h->n2p : h->n2);
+        const int64_t nb = (int64_t)(pm ? h->tx2p : h->tx2) * (pm ? h->ty2p : h->ty2) * (pm ? h->n2p : h->n2);
         wg.push_back(wg.back() + (int)((nb + kNumXcd - 1) / kNumXcd * kNumXcd));
       }
       if ((rc = balloc(b, (void**)&b->y2wg[bl][pm], sizeof(int) * wg.size())) ||
@@ -2368,13 +2391,13 @@ int batch_run2(iris_batch* b, int it_first, int it_last) {
     const int slot = s1 ? b->own[0]->nsamp : -1;
     int rc;
     rest(0, it);
-    for (int bl = 0; bl < 4; ++bl) {
+    for (int bl = 0; bl < 8; ++bl) {
       const int n = (int)b->own2[bl].size();
       for (int pm = 0; pm < 2; ++pm) {
         if (!b->y2nwg[bl][pm]) continue;
         const Batch2Inst<T>* I = (const Batch2Inst<T>*)b->y2[bl][pm] + (size_t)pp * n;
         const int* w = b->y2wg[bl][pm];
-        const int W = (bl & 2) ? 32 : 64;
+        const int W = (bl & (pm ? 4 : 2)) ? 32 : 64;
         const unsigned g = (unsigned)b->y2nwg[bl][pm];
         if (bl & 1) {
           if (pm) launch_yee2_batch<T, true, true>(b, W, g, I, w, n, slot);

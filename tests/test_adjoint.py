@@ -238,13 +238,15 @@ def test_adjoint_with_two_step_forward(tmp_path, monkeypatch):
             assert not eng.pass_info()["two_step"]
         g1, _ = engine_gradient(s, w, prec, 17)
         monkeypatch.delenv("IRIS_TWO_STEP")
-        for width in ("64", "32"):  # tiles of 64 or 32 cells a row
+        for width in ("64", "32"):  # tiles of 64 or 32 cells a row (plain and UPML variant)
             monkeypatch.setenv("IRIS_TILE_W2", width)
+            monkeypatch.setenv("IRIS_TILE_W2P", width)
             with Engine(s, precision=prec) as eng:
                 assert eng.pass_info()["two_step"]
             g2, _ = engine_gradient(s, w, prec, 17)
             assert np.array_equal(g1, g2), width
         monkeypatch.delenv("IRIS_TILE_W2")
+        monkeypatch.delenv("IRIS_TILE_W2P")
         assert np.abs(g1).max() > 0
 
 

@@ -72,19 +72,21 @@ def test_grating_sweep_draws(tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("width", ["auto", "32"])
+@pytest.mark.parametrize("width", ["auto", "32", "32/64"])
 @pytest.mark.parametrize("upml", ["auto", "1"])
 @pytest.mark.parametrize("precision", [8, 4])
 def test_batch_is_bitwise_each_run_alone(tmp_path, monkeypatch, precision, upml, width):
     """Single-step grids, and two-step grids with and without a Bloch phase
     (with the UPML slabs in the two-step pass where the engine would put them
     there, or wherever it applies; two-step tiles of 64 or 32 cells a
-    row)."""
+    row, for the plain pass / its UPML variant)."""
     print(f"[gpu-test] batch vs alone precision={precision} upml={upml} width={width}", flush=True)
     if upml != "auto":
         monkeypatch.setenv("IRIS_TWO_STEP_UPML", upml)
     if width != "auto":
-        monkeypatch.setenv("IRIS_TILE_W2", width)
+        w, wp = (width.split("/") + [width])[:2]
+        monkeypatch.setenv("IRIS_TILE_W2", w)
+        monkeypatch.setenv("IRIS_TILE_W2P", wp)
     setups = mixed_setups(tmp_path)
     nt = setups[0].params.nt
     steps = [(1, 37), (38, 38), (39, nt)]

@@ -386,7 +386,7 @@ TWO_STEP_CASES = {
 
 def _run_two_step(setup, precision, two_step, zchunk, width=0):
     keys = {"IRIS_TWO_STEP": "1" if two_step else "0", "IRIS_TWO_STEP_UPML": "1", "IRIS_ZCHUNK": str(zchunk),
-            "IRIS_TILE_W2": str(width)}
+            "IRIS_TILE_W2": str(width), "IRIS_TILE_W2P": str(width)}
     old = {k: os.environ.get(k) for k in keys}
     os.environ.update(keys)
     try:
