@@ -7,7 +7,7 @@ thick cut by a 100 nm (20-cell) slit along y, normal-incidence TF/SF
 Gaussian pulse covering 400-1000 nm, on-the-fly DFT at 100 wavelengths on a
 full-lateral box (z = 25..1000), output cadence every nt/10 steps (nt =
 20000).  A step is one pass of the time loop (E, H, TF/SF, DFT sample, DFT
-fold every 64 samples with FP32 fields / 16 with FP64, face fluxes at the
+fold every 64 samples with FP32 fields / 32 with FP64, face fluxes at the
 cadence) over the whole grid; the engine advances the plain z-chunks two
 steps per pass.
 

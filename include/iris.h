@@ -232,7 +232,7 @@ int iris_pass_info(const iris_handle* h, int64_t info[6]);
 int iris_autotune(iris_handle* h, int32_t steps, int64_t info[9]);
 
 /* Steps as a captured HIP graph.  iris_step runs whole periods of steps --
- * P = ring slots x sample_fourier (64 x with FP32 fields, 16 x with FP64):
+ * P = ring slots x sample_fourier (64 x with FP32 fields, 32 x with FP64):
  * the DFT sample ring fills and folds once and the field buffers return to
  * their assignment -- as replays of one captured graph of the period's
  * launches, where the period starts with an empty ring at or after

@@ -32,10 +32,11 @@ using namespace iris;
 
 namespace {
 
-// samples buffered before folding into emg: 16 with FP64 fields (exact
-// reference order), 64 with FP32 fields (the matrix-core fold: a quarter of
-// the accumulator traffic)
-constexpr int kDftSlots64 = 16, kDftSlots32 = 64;
+// samples buffered before folding into emg: 32 with FP64 fields (each
+// accumulator takes them one by one in the reference's order, so the ring
+// length changes only how often it is stored and reloaded), 64 with FP32
+// fields (the matrix-core fold)
+constexpr int kDftSlots64 = 32, kDftSlots32 = 64;
 // waves per workgroup of the matrix-core fold and the wavelength groups (16
 // each) whose phases it stages at once (KMAX x 16 x 16 bytes a group)
 constexpr int kFoldWaves = 16, kFoldGroups = 8;

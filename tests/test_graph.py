@@ -62,7 +62,7 @@ def test_graph_replays_are_bitwise_the_launches(tmp_path, monkeypatch, case, pre
     assert not info0["captured"] and info0["replays"] == 0
     monkeypatch.setenv("IRIS_GRAPH", "1")
     got, info = run(s, precision, cuts)
-    assert info["period"] == (64 if precision == 4 else 16) * s.params.sample_fourier
+    assert info["period"] == (64 if precision == 4 else 32) * s.params.sample_fourier
     assert info["captured"] and info["replays"] >= 2, info
     for k, (a, b) in enumerate(zip(got, ref)):
         assert np.array_equal(a, b), k
