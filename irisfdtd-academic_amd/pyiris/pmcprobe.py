@@ -3,8 +3,8 @@
 
 1. Calibration: a plain device-to-device copy of a known number of bytes
    (hipMemcpy, which runs the runtime's copy kernel), three times.
-2. 20 steps of the 512x512x1024 slit-array workload with FP32 fields and, if
-   asked, with FP64 fields (one DFT fold happens at step 16).
+2. 40 steps of the 512x512x1024 slit-array workload with FP32 fields and, if
+   asked, with FP64 fields (one FP64 DFT fold happens at step 32).
 
 Comparing each kernel's counter with the copy's turns the counters into
 bytes: HBM bytes of a kernel = counter(kernel) * copy_bytes / counter(copy).
@@ -36,7 +36,7 @@ def copy_calibration():
     print(f"[pmcprobe] copied {COPY_BYTES} bytes x3", flush=True)
 
 
-def steps(precision, n=20):
+def steps(precision, n=40):
     import bench
     from pyiris import Engine
     s = bench.make_setup(precision)
