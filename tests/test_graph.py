@@ -157,3 +157,19 @@ def test_batch_graph_replays_are_bitwise_the_launches(tmp_path, monkeypatch, pre
     for k, (ra, ga) in enumerate(zip(ref, got)):
         for a, b in zip(ra, ga):
             assert np.array_equal(a, b), k
+
+
+@pytest.mark.parametrize("precision", [8, 4])
+def test_graph_with_the_in_place_passes(tmp_path, monkeypatch, precision):
+    """The in-place two-pass engine (IRIS_FUSED=0: E pass, H pass, TF/SF
+    pass per step, no buffer rotation) replays its periods bitwise too."""
+    print(f"[gpu-test] graph in-place passes precision={precision}", flush=True)
+    s = setups(tmp_path)["single"]
+    monkeypatch.setenv("IRIS_FUSED", "0")
+    monkeypatch.setenv("IRIS_GRAPH", "0")
+    ref, _ = run(s, precision, [250, 251, 333])
+    monkeypatch.setenv("IRIS_GRAPH", "1")
+    got, info = run(s, precision, [250, 251, 333])
+    assert info["replays"] >= 2, info
+    for k, (a, b) in enumerate(zip(got, ref)):
+        assert np.array_equal(a, b), k
