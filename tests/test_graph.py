@@ -173,3 +173,32 @@ def test_graph_with_the_in_place_passes(tmp_path, monkeypatch, precision):
     assert info["replays"] >= 2, info
     for k, (a, b) in enumerate(zip(got, ref)):
         assert np.array_equal(a, b), k
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_graph_random_cadences_and_cuts(tmp_path, monkeypatch, seed):
+    """Seeded draws of the DFT start step, sampling cadence, precision, grid
+    and call cuts: graph replays equal the launches one by one."""
+    rng = np.random.default_rng(100 + seed)
+    to_fourier = int(rng.integers(0, 90))
+    sample = int(rng.integers(1, 4))
+    precision = int(rng.choice([4, 8]))
+    two = bool(rng.integers(0, 2))
+    nt = 400 + 64 * sample * int(rng.integers(2, 5))
+    cuts = sorted(int(c) for c in rng.choice(np.arange(5, nt - 5), size=3, replace=False))
+    print(f"[gpu-test] graph random seed={seed} to_fourier={to_fourier} sample={sample} prec={precision} "
+          f"two={two} nt={nt} cuts={cuts}", flush=True)
+    if two:
+        s = make(tmp_path, "r", ixmax=70, iymax=21, izmax=96, pmlzi=10, pmlzf=86, hi=44, dh=10, dx=8, eps_bbox=2.25,
+                 box=(1, 70, 1, 21, 15, 80), to_fourier=to_fourier, sample_fourier=sample, nt=nt, no_downloads=nt)
+        if seed % 2:
+            s = oblique(s, 15.0 + 5 * seed)
+    else:
+        s = make(tmp_path, "r", ixmax=20, iymax=12, izmax=80, pmlzi=9, pmlzf=71, hi=36, dh=8, dx=6, eps_bbox=3.0,
+                 box=(1, 20, 1, 12, 14, 66), to_fourier=to_fourier, sample_fourier=sample, nt=nt, no_downloads=nt)
+    monkeypatch.setenv("IRIS_GRAPH", "0")
+    ref, _ = run(s, precision, cuts)
+    monkeypatch.setenv("IRIS_GRAPH", "1")
+    got, info = run(s, precision, cuts)
+    for k, (a, b) in enumerate(zip(got, ref)):
+        assert np.array_equal(a, b), (k, info)
