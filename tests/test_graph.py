@@ -202,3 +202,25 @@ def test_graph_random_cadences_and_cuts(tmp_path, monkeypatch, seed):
     got, info = run(s, precision, cuts)
     for k, (a, b) in enumerate(zip(got, ref)):
         assert np.array_equal(a, b), (k, info)
+
+
+def test_graph_bitwise_at_full_size(monkeypatch):
+    """512 x 512 x 1024, FP32: 200 steps with a captured period replayed
+    (and the UPML variant, the single-step length groups and the fold inside
+    the graph) equal the launches one by one."""
+    print("[gpu-test] graph full size", flush=True)
+    import bench
+    s = bench.make_setup(4, nof=4, nt=200, no_downloads=200)
+    out = {}
+    for g in ("0", "1"):
+        monkeypatch.setenv("IRIS_GRAPH", g)
+        with Engine(s, precision=4) as eng:
+            eng.run_to(200)
+            p = s.params
+            out[g] = (eng.flux(1), eng.graph_info(),
+                      eng.plane(Engine.PLANE_E + 0, 3, 500, (1, p.nx), (1, p.ny)),
+                      eng.plane(Engine.PLANE_H + 1, 2, 256, (1, p.nx), (1, p.nz)))
+    assert out["1"][1]["replays"] >= 1, out["1"][1]
+    assert np.abs(out["0"][0]).max() > 0
+    for k in (0, 2, 3):
+        assert np.array_equal(out["0"][k], out["1"][k]), k
