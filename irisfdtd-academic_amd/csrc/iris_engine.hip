@@ -797,8 +797,8 @@ int graph_period_run(iris_handle* h, int it, const PassLists& L, bool* shifted) 
   if (h->gexec && std::equal(key, key + 12, h->g_key) && (it - h->g_it0) % h->p.sample_fourier == 0) {
     hipLaunchKernelGGL(k_set_shift, dim3(1), dim3(64), 0, h->stream, h->tfc, it - h->g_it0);
     HIP_TRY(h, hipGetLastError());
+    *shifted = true;  // the caller resets the shift before any eager step, also after an error here
     HIP_TRY(h, hipGraphLaunch(h->gexec, h->stream));
-    *shifted = true;
     h->it = it + P - 1;
     h->g_replays++;
     return 0;
@@ -868,7 +868,10 @@ int run_steps(iris_handle* h, int it_first, int it_last) {
   L.ngroups = (int)h->rest_groups.size();
   while (graph_fits<T>(h, it, it_last)) {
     int rc = graph_period_run<T>(h, it, L, &shifted);
-    if (rc) return rc;
+    if (rc) {
+      (void)unshift();
+      return rc;
+    }
     it += graph_period<T>(h);
   }
   if (int rc = unshift()) return rc;
@@ -2431,7 +2434,10 @@ int batch_run2(iris_batch* b, int it_first, int it_last) {
     if (graph_ok && pp == 0 && b->own[0]->nsamp == 0 && it >= p.to_fourier && it_last - it + 1 >= P + 1) {
       if (b->gexec && (it - b->g_it0) % p.sample_fourier == 0) {
         if ((rc = set_shift(it - b->g_it0))) return rc;
-        if (hipGraphLaunch(b->gexec, b->stream) != hipSuccess) return bfail(b, IRIS_EDEVICE, "hipGraphLaunch failed");
+        if (hipGraphLaunch(b->gexec, b->stream) != hipSuccess) {
+          (void)set_shift(0);
+          return bfail(b, IRIS_EDEVICE, "hipGraphLaunch failed");
+        }
         b->g_replays++;
         for (iris_handle* h : b->own) h->g_replays++;
         it += P;
