@@ -238,7 +238,9 @@ int iris_autotune(iris_handle* h, int32_t steps, int64_t info[9]);
  * launches, where the period starts with an empty ring at or after
  * to_fourier and a whole period fits in the call.  Results are bitwise those
  * of the launches one by one.  Not while profiling or recording snapshots;
- * IRIS_GRAPH=0 at iris_create turns it off.  iris_batch_step does the same
+ * IRIS_GRAPH=0 at iris_create turns it off; so does a profiler's tool
+ * library in the environment (ROCP_TOOL_LIBRARIES, set by rocprofv3) unless
+ * IRIS_GRAPH=1.  iris_batch_step does the same
  * for the pairs of steps of its two-step handles, with one graph per batch.
  * info[0] = 1 when the handle has a graph captured, info[1] = replays so far
  * that ran the handle's steps (its own graph's and a batch's), info[2] = P.  No counterpart in the
@@ -303,7 +305,9 @@ int iris_batch_create(iris_handle* const* handles, int32_t n, iris_batch** out);
 
 /* Steps it_first..it_last of every handle of the batch (all at step
  * it_first-1, it_last within each nt).  Queued on the batch's stream; later
- * calls on a handle wait for it. */
+ * calls on a handle wait for it.  IRIS_ESTATE when a handle's pass layout
+ * was rebuilt after iris_batch_create (iris_autotune): the batch's tables
+ * hold the old one. */
 int iris_batch_step(iris_batch* b, int32_t it_first, int32_t it_last);
 
 /* info[0] = handles, info[1] = planes per z-chunk of the batched pass,
@@ -316,6 +320,25 @@ int iris_batch_destroy(iris_batch* b);
 /* Message for the last failure of the batch (or of the last failed
  * iris_batch_create when b is NULL).  Never NULL. */
 const char* iris_batch_last_error(const iris_batch* b);
+
+/* Incident field of an oblique TF/SF source (ratio_angle != 1).  The
+ * reference evaluates its Gaussian pulse along u = (cos D, 0, sin D),
+ * D = ratio_angle*pi/2 (code/tfsf.f90:231-355), and itself only runs
+ * ratio_angle = 1 (code/files.f90:267).  Off normal incidence the engine
+ * replaces that pulse by the solution of its own update for the Bloch mode
+ * exp(i ak_x ix / nx) in the source medium (a 1D auxiliary grid with
+ * absorbing ends, driven by the pulse's time signal), so that every
+ * frequency satisfies the x-period's Bloch condition and travels at its own
+ * angle (sin theta = kx / (k n), kx = ak_x / (nx Q)); ux, uz only scale the
+ * amplitude.  Needs plane_wave = 1, the TF/SF plane below the structure and
+ * a real Bloch phase in x only (IRIS_EUNSUP otherwise); IRIS_EINVAL at
+ * ratio_angle = 1, where the reference's analytic pulse is used.  out
+ * receives (nt+1) x 2 x 3 complex values: for step it = 0..nt, plane
+ * iz_min-1 then iz_min, the components 1, 3, 5 (i_pol = 2: Ex, Ez, Hy) or
+ * 2, 4, 6 (i_pol = 1: Ey, Hx, Hz) at ix = 0 as the TF/SF corrections read
+ * them after step it (times exp(i ak_x ix / nx) at ix).  Host computation
+ * only (no device work).  No counterpart in the reference. */
+int iris_incident_table(const iris_params* p, double* out);
 
 /* Library version string. */
 const char* iris_version(void);

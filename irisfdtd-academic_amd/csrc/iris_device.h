@@ -99,6 +99,12 @@ struct TfsfConst {
   // steps added to every step index the kernels are given: 0, except while a
   // captured graph of steps replays at a later step than it was captured at
   int shift;
+  // oblique incidence: the incident field matched to the Bloch condition
+  // (iris_incident.h), (nt + 1) x 2 planes (iz_min - 1, iz_min) x 3
+  // components at ix = 0, times e^{i phix ix} at ix; null for the
+  // reference's analytic pulse
+  const double2* tab;
+  double phix;
 };
 
 }  // namespace iris

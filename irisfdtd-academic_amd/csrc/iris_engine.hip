@@ -26,6 +26,7 @@
 
 #include "../../include/iris.h"
 #include "iris_adjoint.h"
+#include "iris_incident.h"
 #include "iris_kernels.h"
 
 using namespace iris;
@@ -167,6 +168,7 @@ struct iris_handle {
   int tx2p = 0;             // tile columns of the UPML variant
   int nt2 = 0;              // two-step pass: streaming stores of its outputs
   PassConfig pass;          // the layout the lists were built for
+  int64_t pass_gen = 0;     // bumped by every rebuild of the pass lists
   // single steps of the two-step layout's other chunks when they come in
   // several lengths: one batched launch per class (lean, general) over the
   // length groups, with per-group arguments from a table per buffer
@@ -255,6 +257,7 @@ struct iris_handle {
 // A batch of handles stepped together by the batched fused pass.
 struct iris_batch {
   std::vector<iris_handle*> hs;     // all handles of the batch
+  std::vector<int64_t> pass_gen;    // their pass_gen when the batch was made (its tables hold those lists)
   std::vector<iris_handle*> one;    // those the batched single-step pass advances
   std::vector<iris_handle*> own;    // those that take their own two-step passes
   int prec = 4;
@@ -1111,6 +1114,7 @@ int build_passes(iris_handle* h, const PassConfig& c) {
   int rc;
   HIP_TRY(h, hipStreamSynchronize(h->stream));
   graph_release(h);  // captured with the old lists
+  ++h->pass_gen;     // a batch made from the old lists refuses to step
   h->pass = c;
   h->xcd_rows = c.xcd_rows;
   h->xcd_rows2 = c.xcd_rows2;
@@ -1445,7 +1449,18 @@ int create_impl(iris_handle* h, const double* eps_inv, const double* fpmn, const
   h->grd_tf = dim3((p.nx + 1 + tbx - 1) / tbx, (p.ny + 1 + tby - 1) / tby, 1);
   h->dtcq2 = ((p.dt * p.c0) / p.q) * ((p.dt * p.c0) / p.q);
   {
-    const TfsfConst t = tfsf_const(h);
+    TfsfConst t = tfsf_const(h);
+    if (incident_matched(p)) {
+      // off normal incidence: the incident field matched to the Bloch condition
+      if (const char* why = incident_unsupported(p)) return fail(h, IRIS_EUNSUP, "%s", why);
+      const std::vector<zcplx> tab = incident_table(p);
+      void* dtab = nullptr;
+      if ((rc = dalloc(h, &dtab, sizeof(zcplx) * tab.size()))) return rc;
+      HIP_TRY(h, hipMemcpy(dtab, tab.data(), sizeof(zcplx) * tab.size(), hipMemcpyHostToDevice));
+      t.tab = (const double2*)dtab;
+      t.phix = p.ak[0][0] / p.nx;
+      t.mode2d = 0;  // the corrections of 3D grids (exact for a discrete wave, also on 1D/2D grids)
+    }
     if ((rc = dalloc(h, (void**)&h->tfc, sizeof(TfsfConst)))) return rc;
     HIP_TRY(h, hipMemcpyAsync(h->tfc, &t, sizeof t, hipMemcpyHostToDevice, h->stream));
   }
@@ -2697,8 +2712,13 @@ int iris_create(const iris_params* p, const double* eps_inv, const double* fpzmn
     const char* tyenv = std::getenv("IRIS_ADJ_TY");
     if (tyenv && (std::atoi(tyenv) == 7 || std::atoi(tyenv) == 8 || std::atoi(tyenv) == 15))
       h->adj_ty = std::atoi(tyenv);
+    // Graphs are off under a profiler's tool library (rocprofv3 sets
+    // ROCP_TOOL_LIBRARIES): a kernel trace of many handles each replaying
+    // its own graph ended in a host-side fault inside the HIP runtime with
+    // no fault of a kernel; IRIS_GRAPH=1 keeps them on regardless.
     const char* genv = std::getenv("IRIS_GRAPH");
-    h->graph_on = !(genv && genv[0] == '0');
+    const char* tool = std::getenv("ROCP_TOOL_LIBRARIES");
+    h->graph_on = genv ? genv[0] != '0' : !(tool && tool[0]);
   }
   (void)hipGetDevice(&h->device);
   if (hipDeviceGetAttribute(&h->ncu, hipDeviceAttributeMultiprocessorCount, h->device) != hipSuccess || h->ncu < 1)
@@ -3107,6 +3127,7 @@ int iris_batch_create(iris_handle* const* handles, int32_t n, iris_batch** out) 
   }
   iris_batch* b = new iris_batch();
   b->hs.assign(handles, handles + n);
+  for (int i = 0; i < n; ++i) b->pass_gen.push_back(handles[i]->pass_gen);
   b->prec = handles[0]->prec;
   (void)hipSetDevice(handles[0]->device);
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
@@ -3139,6 +3160,9 @@ int iris_batch_step(iris_batch* b, int32_t it_first, int32_t it_last) {
       return bfail(b, IRIS_ESTATE, "iris_batch_step: DFT rings out of step");
     if (h->ck_seg > 0)
       return bfail(b, IRIS_EUNSUP, "iris_batch_step: handle %zu keeps gradient snapshots (step it alone)", i);
+    if (h->pass_gen != b->pass_gen[i])
+      return bfail(b, IRIS_ESTATE, "iris_batch_step: the pass layout of handle %zu changed after iris_batch_create "
+                   "(iris_autotune); create the batch again", i);
   }
   if (it_last < it_first) return 0;
   (void)hipSetDevice(b->hs[0]->device);
@@ -3164,6 +3188,17 @@ int iris_batch_destroy(iris_batch* b) {
 
 const char* iris_batch_last_error(const iris_batch* b) { return b ? b->err.c_str() : g_create_error.c_str(); }
 
-const char* iris_version(void) { return "iris-mi355x 0.1 (gfx950)"; }
+int iris_incident_table(const iris_params* p, double* out) {
+  if (!p || !out) return fail(nullptr, IRIS_EINVAL, "iris_incident_table: NULL argument");
+  if (!incident_matched(*p))
+    return fail(nullptr, IRIS_EINVAL, "iris_incident_table: normal incidence takes the reference's analytic pulse");
+  if (const char* why = incident_unsupported(*p)) return fail(nullptr, IRIS_EUNSUP, "iris_incident_table: %s", why);
+  if (p->nt < 0 || p->nx < 1) return fail(nullptr, IRIS_EINVAL, "iris_incident_table: bad nt or nx");
+  const std::vector<zcplx> tab = incident_table(*p);
+  std::memcpy(out, tab.data(), sizeof(zcplx) * tab.size());
+  return 0;
+}
+
+const char* iris_version(void) { return "iris-mi355x 0.2 (gfx950)"; }
 
 }  // extern "C"

@@ -204,6 +204,15 @@ __global__ __launch_bounds__(256) void k_hpass(StepArgs<T> a) {
 
 // incident values of comps {1,3,5} (i_pol=2) or {2,4,6} (i_pol=1) at (ix, iz)
 __device__ inline void tfsf_incident(const TfsfConst& t, int it, int ix, int iz, double2* inc) {
+  if (t.tab) {
+    // matched oblique field: the table's value at ix = 0 times e^{i phix ix}
+    const double2* v = t.tab + ((int64_t)(it + t.shift) * 2 + (iz - (t.iz_min - 1))) * 3;
+    double sn, cs;
+    sincos(t.phix * (double)ix, &sn, &cs);
+    const double2 ph = make_double2(cs, sn);
+    for (int k = 0; k < 3; ++k) inc[k] = cmul(v[k], ph);
+    return;
+  }
   const double c0dt_it = t.c0dt * (double)(it + t.shift);
   const double A = (t.q * (double)(ix - t.ixo)) * t.ux + (t.q * ((double)iz - t.zopump)) * t.uz;
   const double kzz = t.ko * ((t.sq * t.sign_kz) * A - c0dt_it);

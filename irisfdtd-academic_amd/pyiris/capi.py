@@ -127,6 +127,8 @@ def load_library(path=None):
     lib.iris_profile_read.restype = ct.c_int
     lib.iris_version.argtypes = []
     lib.iris_version.restype = ct.c_char_p
+    lib.iris_incident_table.argtypes = [P, ct.c_void_p]
+    lib.iris_incident_table.restype = ct.c_int
     _lib = lib
     return lib
 
@@ -136,10 +138,24 @@ EXPORTED = ["iris_create", "iris_step", "iris_sync", "iris_flux", "iris_dft_poin
             "iris_destroy", "iris_last_error", "iris_version", "iris_profile", "iris_profile_read",
             "iris_pass_info", "iris_autotune", "iris_graph_info", "iris_checkpoint", "iris_checkpoint_info",
             "iris_adjoint",
-            "iris_batch_create", "iris_batch_step", "iris_batch_info", "iris_batch_destroy", "iris_batch_last_error"]
+            "iris_batch_create", "iris_batch_step", "iris_batch_info", "iris_batch_destroy", "iris_batch_last_error",
+            "iris_incident_table"]
 
 KERNELS = ["epass", "hpass", "tfsf", "dft_sample", "dft_accum", "flux", "yee", "yee2", "yee_rest", "adjoint",
            "checkpoint", "yee2_pml"]
+
+
+def incident_table(params):
+    """The engine's incident field for an oblique source (ratio_angle != 1;
+    iris_incident_table): complex (nt+1, 2, 3) -- step, plane (iz_min-1,
+    iz_min), components 1,3,5 (i_pol=2) or 2,4,6 (i_pol=1) at ix = 0.  Host
+    computation only."""
+    lib = load_library()
+    out = np.zeros(2 * (params.nt + 1) * 6, dtype=np.float64)
+    rc = lib.iris_incident_table(ct.byref(params), _p(out))
+    if rc:
+        raise IrisError(f"iris_incident_table failed ({rc}): {lib.iris_last_error(None).decode()}")
+    return out.view(np.complex128).reshape(params.nt + 1, 2, 3)
 
 
 class Engine:

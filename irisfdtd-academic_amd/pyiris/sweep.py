@@ -18,8 +18,8 @@ in [0, 60] degrees.
 metasurface() and angle_averaged_gradient() are the pieces of an
 angle-averaged gradient: a structure whose eps_r is drawn per voxel block,
 and the mean over incidence angles of the summed transmission with its
-gradient with respect to the structure's eps_inv (one forward run and one
-adjoint per angle).
+gradient with respect to the structure's eps_inv or eps_r (one forward run
+and one adjoint per angle, one angle at a time).
 """
 import math
 import os
@@ -99,15 +99,29 @@ def metasurface(setup, block=32, eps_range=(1.0, 6.0), seed=5):
     return with_structure(setup, 1.0 / eps)
 
 
-def angle_averaged_gradient(setup, thetas, precision=4, nobox=1, segment=None, record_bytes=-1):
+def angle_averaged_gradient(setup, thetas, precision=4, nobox=1, segment=None, record_bytes=-1, wrt="eps_inv"):
     """Mean over the incidence angles thetas (degrees, xz plane, matching
     Bloch phase) of sum_w T(w), T = J6/(J5+J6) of box nobox, and its gradient
-    with respect to the real eps_inv of the structure cells, (hf-hi+1, ny, nx).
-    Each angle is one forward run and one adjoint (pyiris.adjoint.flux)."""
+    with respect to the structure cells, (hf-hi+1, ny, nx): of their real
+    eps_inv (wrt="eps_inv") or of their eps_r (wrt="eps", dT/d eps =
+    -dT/d eps_inv / eps^2 by the chain rule through eps_inv = 1/eps).
+
+    The angles run one after another: each forward run and its adjoint
+    (pyiris.adjoint.flux) finish, and the engine with its snapshots and
+    records is freed, before the next angle's engine is made, so the device
+    memory held does not grow with the number of angles.  The gradients add
+    up in the leaf tensor."""
     import torch
     from .adjoint import flux, structure_of, transmission
-    w = torch.tensor(structure_of(setup).real, requires_grad=True)
-    obj = sum(transmission(flux(w, oblique(setup, th), precision, nobox, segment, record_bytes)).sum()
-              for th in thetas) / len(thetas)
-    obj.backward()
-    return float(obj.detach()), w.grad.numpy().copy()
+    if wrt not in ("eps_inv", "eps"):
+        raise ValueError(f"wrt must be 'eps_inv' or 'eps', got {wrt!r}")
+    w0 = structure_of(setup).real
+    leaf = torch.tensor(w0 if wrt == "eps_inv" else 1.0 / w0, requires_grad=True)
+    total = 0.0
+    for th in thetas:
+        w = leaf if wrt == "eps_inv" else 1.0 / leaf
+        obj = transmission(flux(w, oblique(setup, th), precision, nobox, segment, record_bytes)).sum() / len(thetas)
+        obj.backward()
+        total += float(obj.detach())
+        del obj, w
+    return total, leaf.grad.numpy().copy()

@@ -78,6 +78,10 @@ typedef struct oracle {
      x in [-1, nx+2] (y-independent: u%y = 0) */
   cx *tfsf;                    /* [plane 0/1][comp 0..5][ix+1] */
   int tfsf_nxs;
+  /* a tabulated incident field in place of tfsf_gauss (oracle_set_incident):
+     (nt+1) x 2 planes x 3 comps at ix = 0, times exp(i phix ix) at ix */
+  cx *inc_tab;
+  double phix;
   /* Fourier boxes */
   int npts[2][3];
   int *pto[2][3];              /* (npts,4): comp, x, y, z */
@@ -172,12 +176,23 @@ oracle* oracle_create(const iris_params* p, const double* eps_inv, const double*
   return o;
 }
 
+/* Take the incident field from a table ((nt+1) x 2 x 3 complex, the layout
+ * of iris_incident_table) instead of the analytic pulse; the corrections are
+ * those of 3D grids (code/tfsf.f90:76-119) on every grid. */
+void oracle_set_incident(oracle* o, const double* table) {
+  const size_t n = ((size_t)o->p.nt + 1) * 6;
+  free(o->inc_tab);
+  o->inc_tab = (cx*)malloc(sizeof(cx) * n);
+  memcpy(o->inc_tab, table, sizeof(cx) * n);
+  o->phix = o->p.ak[0][0] / o->p.nx;
+}
+
 void oracle_destroy(oracle* o) {
   if (!o) return;
   free(o->e_cur); free(o->e_prev); free(o->h_cur); free(o->h_prev);
   for (int k = 0; k < 2; k++) { free(o->d_cur[k]); free(o->d_prev[k]); free(o->b_cur[k]); free(o->b_prev[k]); }
   free(o->fpzmn); free(o->fmzmn); free(o->fpzmx); free(o->fmzmx);
-  free(o->eps_inv); free(o->fw); free(o->tfsf);
+  free(o->eps_inv); free(o->fw); free(o->tfsf); free(o->inc_tab);
   for (int k = 0; k < 2; k++)
     for (int j = 0; j < 3; j++) { free(o->pto[k][j]); free(o->emg[k][j]); }
   free(o);
@@ -418,6 +433,23 @@ static void tfsf_gauss(oracle* o, int it) {
     for (size_t i = 0; i < 2 * 6 * (size_t)o->tfsf_nxs; i++) o->tfsf[i] = cmul_r(p->win, o->tfsf[i]);
 }
 
+/* Tabulated incident field (oracle_set_incident): the table's components
+ * 1, 3, 5 (i_pol = 2) or 2, 4, 6 (i_pol = 1) at ix = 0 of step it on both
+ * planes, times exp(i phix ix) at ix, with no other change. */
+static void tfsf_table(oracle* o, int it) {
+  const iris_params* p = &o->p;
+  const int c0 = p->i_pol == 2 ? 0 : 1;
+  memset(o->tfsf, 0, sizeof(cx) * 2 * 6 * (size_t)o->tfsf_nxs);
+  for (int plane = 0; plane < 2; plane++) {
+    const cx* v = o->inc_tab + ((size_t)it * 2 + plane) * 3;
+    for (int ix = -1; ix <= o->nx + 2; ix++) {
+      cx arg = {0.0, o->phix * (double)ix};
+      cx ph = cexp_ref(arg);
+      for (int k = 0; k < 3; k++) TF(o, plane, c0 + 2 * k, ix) = cmul(v[k], ph);
+    }
+  }
+}
+
 /* code/tfsf.f90:76-119 (the far-side block is disabled, :122) */
 static void tfsf_update_3d(oracle* o) {
   const iris_params* p = &o->p;
@@ -498,7 +530,10 @@ void oracle_step(oracle* o, int it_first, int it_last) {
     h_fdtd(o, dtcq2);
     upml_h(o);
     bc_min(o);
-    if (p->em_source) {
+    if (p->em_source && o->inc_tab) {
+      tfsf_table(o, it);
+      tfsf_update_3d(o);
+    } else if (p->em_source) {
       tfsf_gauss(o, it);
       if (o->ny != 1 && o->nx != 1) tfsf_update_3d(o);
       else if (p->i_pol == 2) tfsf_update_2d(o);

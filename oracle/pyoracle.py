@@ -45,6 +45,8 @@ def load():
     lib.oracle_diagnostics.restype = None
     lib.oracle_current_step.argtypes = [V]
     lib.oracle_current_step.restype = ct.c_int
+    lib.oracle_set_incident.argtypes = [V, V]
+    lib.oracle_set_incident.restype = None
     lib.oracle_destroy.argtypes = [V]
     lib.oracle_destroy.restype = None
     _lib = lib
@@ -108,6 +110,14 @@ class Oracle:
         out = np.zeros(4)
         self.lib.oracle_diagnostics(self._h, int(zi), int(zf), _p(out))
         return out
+
+    def set_incident(self, table):
+        """Tabulated incident field ((nt+1, 2, 3) complex, the layout of
+        iris_incident_table) in place of the analytic pulse."""
+        t = np.ascontiguousarray(table, dtype=np.complex128)
+        assert t.shape == (self.params.nt + 1, 2, 3), t.shape
+        self._tab = t
+        self.lib.oracle_set_incident(self._h, _p(t.view(np.float64)))
 
     def close(self):
         if self._h:

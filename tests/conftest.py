@@ -11,8 +11,8 @@ for p in (ROOT, PKG, os.path.join(ROOT, "oracle")):
     if p not in sys.path:
         sys.path.insert(0, p)
 
-GOLDEN_CASES = ["example_1", "example_2_short", "cfg1_short", "small3d_ipol2", "small3d_ipol1_cos_2box",
-                "slit2d_x", "slit2d_y", "substrate3d", "sdl3d", "sdl3d_hmod"]
+GOLDEN_CASES = ["example_1", "example_2_short", "cfg1_short", "cfg1_full", "cfg2_period2d", "small3d_ipol2",
+                "small3d_ipol1_cos_2box", "slit2d_x", "slit2d_y", "substrate3d", "sdl3d", "sdl3d_hmod"]
 
 
 def pytest_configure(config):

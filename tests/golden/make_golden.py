@@ -49,6 +49,20 @@ CASES = {
     "example_1": (lambda: example("example_1"), [3]),
     "example_2_short": (lambda: example("example_2", itmax_fs=7.7033), []),
     "cfg1_short": (lambda: inputs.render(nt=400, nof=16, dlambda=40.0, no_downloads=4), [3]),
+    # the dielectric thin slab at normal incidence at full length: 16 x 16 x
+    # 1024, eps_r 2.25 slab 100 nm thick, 8000 steps, DFT at 200 wavelengths
+    # in 400-1000 nm on a 1 x 1 column box, ten output blocks (the inputs'
+    # defaults); the pulse crosses the slab and both flux planes
+    "cfg1_full": (lambda: inputs.render(), [3]),
+    # one 600 nm period of the slit-array film, as a 2D grid (y uniform):
+    # 120 x 1 x 1024 cells, eps_r 4.0 film 200 nm (40 cells) thick on
+    # z = 492..531, one 100 nm (20-cell) air slit per period, 20000 steps,
+    # DFT at 100 wavelengths in 400-1000 nm on the full-period box, ten
+    # output blocks; emg of the z faces stored at every 10th wavelength
+    "cfg2_period2d": (lambda: inputs.render(
+        ixmax=120, iymax=1, izmax=1024, pmlzi=20, pmlzf=1005, hi=492, dh=40, dx=20, eps_bbox=4.0,
+        box=(1, 120, 1, 1, 25, 1000), nof=100, dlambda=600.0 / 99.0, lno=400.0, nt=20000,
+        no_downloads=10), [3]),
     "small3d_ipol2": (lambda: inputs.render(
         ixmax=8, iymax=6, izmax=96, pmlzi=10, pmlzf=86, hi=40, dh=12, dx=3, eps_bbox=2.25, tfsf_o=20,
         box=(2, 7, 1, 6, 15, 80), nof=4, nt=300, no_downloads=3, lno=500.0, dlambda=100.0, eps3=1.7), [2, 3]),
@@ -92,6 +106,9 @@ CASES = {
 }
 
 
+# cases whose stored emg keeps every n-th wavelength only (fixture size)
+EMG_STRIDE = {"cfg2_period2d": 10}
+
 # cases on which the reference binary stops with a runtime error; the text
 # that must appear in its error output
 EXPECTED_ABORT = {"loop_2runs": "ACTION= may not be changed on an open unit"}
@@ -122,7 +139,10 @@ def make(name):
             tot = pyoracle.read_scatt(os.path.join(run, f"{b}_scatt.dat"))
             out[f"jtot_b{b}"] = np.stack([v for _, v in tot], axis=0)
             for pair in pairs:
-                out[f"emg_b{b}_p{pair}"] = pyoracle.read_emg_dump(os.path.join(run, f"emg{b}_{pair}.bin"))
+                emg = pyoracle.read_emg_dump(os.path.join(run, f"emg{b}_{pair}.bin"))
+                out[f"emg_b{b}_p{pair}"] = emg[::EMG_STRIDE.get(name, 1)]
+        if name in EMG_STRIDE:
+            out["emg_stride"] = np.array([EMG_STRIDE[name]], dtype=np.int32)
         spec = np.loadtxt(os.path.join(run, "dielectric_layer_spectra.dat"))
         out["film_spectrum"] = spec
         for fn in sorted(os.listdir(run)):

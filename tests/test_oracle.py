@@ -18,8 +18,16 @@ import pyoracle
 from pyiris import inputs, load_input
 
 
-def _oracle_run(name):
+# the full-length goldens (thousands of steps with a 100-200 wavelength DFT):
+# the oracle is run to the end of their first output block only, to keep
+# the CPU suite short; the GPU tests compare the engine with every block
+FIRST_BLOCK_ONLY = {"cfg1_full", "cfg2_period2d"}
+
+
+def _oracle_run(name, nt=None):
     s = load_input(golden_path(name, "dat"))
+    if nt is not None:
+        s.params.nt = nt
     o = pyoracle.Oracle(s)
     res = run_with_cadence(o, s.params, s.params.nbox)
     return s, o, res
@@ -28,14 +36,17 @@ def _oracle_run(name):
 @pytest.mark.parametrize("name", GOLDEN_CASES)
 def test_oracle_matches_reference_golden_bitwise(name):
     g = load_golden(name)
-    s, o, res = _oracle_run(name)
+    short = name in FIRST_BLOCK_ONLY
+    nblk = 1 if short else len(g["it_b1"])
+    s, o, res = _oracle_run(name, int(g["it_b1"][0]) if short else None)
     for b in range(1, s.params.nbox + 1):
         its = [it for it, _ in res[b]]
-        assert its == g[f"it_b{b}"].tolist()
+        assert its == g[f"it_b{b}"].tolist()[:nblk]
         jab = np.stack([j for _, j in res[b]])
-        assert np.array_equal(jab, g[f"jabs_b{b}"]), f"box {b}: max diff {np.abs(jab - g[f'jabs_b{b}']).max()}"
+        ref = g[f"jabs_b{b}"][:nblk]
+        assert np.array_equal(jab, ref), f"box {b}: max diff {np.abs(jab - ref).max()}"
         for key in g:
-            if key.startswith(f"emg_b{b}_p"):
+            if key.startswith(f"emg_b{b}_p") and not short:
                 pair = int(key[-1])
                 assert np.array_equal(o.dft(b, pair), g[key]), key
     o.close()
