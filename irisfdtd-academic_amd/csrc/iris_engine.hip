@@ -1456,7 +1456,10 @@ int create_impl(iris_handle* h, const double* eps_inv, const double* fpmn, const
       const std::vector<zcplx> tab = incident_table(p);
       void* dtab = nullptr;
       if ((rc = dalloc(h, &dtab, sizeof(zcplx) * tab.size()))) return rc;
-      HIP_TRY(h, hipMemcpy(dtab, tab.data(), sizeof(zcplx) * tab.size(), hipMemcpyHostToDevice));
+      // on the handle's stream, after dalloc's zero fill there (a copy on the
+      // null stream is not ordered with this non-blocking stream)
+      HIP_TRY(h, hipMemcpyAsync(dtab, tab.data(), sizeof(zcplx) * tab.size(), hipMemcpyHostToDevice, h->stream));
+      HIP_TRY(h, hipStreamSynchronize(h->stream));  // `tab` is freed at the end of this block
       t.tab = (const double2*)dtab;
       t.phix = p.ak[0][0] / p.nx;
       t.mode2d = 0;  // the corrections of 3D grids (exact for a discrete wave, also on 1D/2D grids)
@@ -3151,18 +3154,24 @@ int iris_batch_create(iris_handle* const* handles, int32_t n, iris_batch** out) 
 int iris_batch_step(iris_batch* b, int32_t it_first, int32_t it_last) {
   if (!b) return bfail(nullptr, IRIS_EINVAL, "batch is NULL");
   for (size_t i = 0; i < b->hs.size(); ++i) {
+    // first: a rebuilt layout (iris_autotune) leaves the batch's own/one
+    // split and tables stale, so nothing below may rely on them
+    if (b->hs[i]->pass_gen != b->pass_gen[i])
+      return bfail(b, IRIS_ESTATE, "iris_batch_step: the pass layout of handle %zu changed after iris_batch_create "
+                   "(iris_autotune); create the batch again", i);
+  }
+  for (size_t i = 0; i < b->hs.size(); ++i) {
     const iris_handle* h = b->hs[i];
+    // the rings of each class (batched two-step handles, the others) fill together
+    const bool own = std::find(b->own.begin(), b->own.end(), h) != b->own.end();
+    const iris_handle* ring0 = own ? b->own[0] : b->one[0];
     if (h->it != it_first - 1)
       return bfail(b, IRIS_ESTATE, "iris_batch_step: handle %zu is at step %d, not %d", i, h->it, it_first - 1);
     if (it_last > h->p.nt) return bfail(b, IRIS_ESTATE, "iris_batch_step: it_last=%d is past nt=%d of handle %zu",
                                         it_last, h->p.nt, i);
-    if (h->nsamp != (h->two_step ? b->own[0] : b->one[0])->nsamp)
-      return bfail(b, IRIS_ESTATE, "iris_batch_step: DFT rings out of step");
+    if (h->nsamp != ring0->nsamp) return bfail(b, IRIS_ESTATE, "iris_batch_step: DFT rings out of step");
     if (h->ck_seg > 0)
       return bfail(b, IRIS_EUNSUP, "iris_batch_step: handle %zu keeps gradient snapshots (step it alone)", i);
-    if (h->pass_gen != b->pass_gen[i])
-      return bfail(b, IRIS_ESTATE, "iris_batch_step: the pass layout of handle %zu changed after iris_batch_create "
-                   "(iris_autotune); create the batch again", i);
   }
   if (it_last < it_first) return 0;
   (void)hipSetDevice(b->hs[0]->device);

@@ -40,8 +40,8 @@
 // film, which ring without decay, and the finite DFT window leaks their
 // ringing into the longest wavelengths.  The drive is therefore the pulse
 // with its spectrum raised-cosine filtered: zero below 1.1 w_c, whole above
-// 1.4 w_c (by FFT over a window four times the run, so that the filter's
-// tails stay clear of the run).  T and R are ratios per wavelength, so the
+// 1.4 w_c (by FFT over the run zero-padded to four times its length, so
+// that the filter's tails do not wrap around into it).  T and R are ratios per wavelength, so the
 // shaping leaves them unchanged where the drive has power; wavelengths above
 // lambda_c / 1.1 (lambda_c = 2 pi / kx) get none.
 #pragma once
@@ -146,8 +146,9 @@ inline std::vector<zcplx> incident_table(const iris_params& p) {
   const double tau = dk * sq / (p.c0 * p.dt);  // envelope width in steps
   const double cos_t = std::sqrt(std::max(0.0, 1.0 - std::pow(phi / (p.q * ko * sq), 2)));
   const double vz = (p.c0 * p.dt / p.q) / sq * std::max(cos_t, 0.2);  // cells per step along z
-  double it_src = A0 * sq / (p.c0 * p.dt) - (S + 1) / vz;
-  it_src = std::max(it_src, 6.0 * tau);
+  // the drive peaks so that the pulse passes iz_min when the reference
+  // pulse's envelope peaks there
+  const double it_src = std::max(A0 * sq / (p.c0 * p.dt) - (S + 1) / vz, 0.0);
   const double w0 = ko * p.c0 * p.dt;  // carrier phase per step
   std::vector<zcplx> g(nt + 1, zcplx(0.0, 0.0));
   for (int it = 1; it <= nt; ++it) {

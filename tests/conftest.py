@@ -75,6 +75,18 @@ def dft_rel_err(a, b):
     return float((num / den).max())
 
 
+def oracle_for(setup):
+    """The oracle of `setup`, driven off normal incidence (ratio_angle != 1)
+    by the engine's matched incident field (iris_incident_table) as the
+    engine is."""
+    import pyoracle
+    o = pyoracle.Oracle(setup)
+    if setup.params.ratio_angle != 1.0:
+        from pyiris.capi import incident_table
+        o.set_incident(incident_table(setup.params))
+    return o
+
+
 @pytest.fixture(scope="session")
 def ref_bin_available():
     import pyoracle

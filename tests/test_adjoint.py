@@ -30,8 +30,9 @@ from torch_fdtd import TorchFDTD
 AK = (0.3, -0.2, 0.0)
 
 
-def make_setup(tmp_path, case):
-    """Small 3D grids with a structure layer (eps_inv drawn per cell)."""
+def make_setup(tmp_path, case, **over):
+    """Small 3D grids with a structure layer (eps_inv drawn per cell); `over`
+    replaces input values."""
     if case == "slab_ipol2":
         kw = dict(ixmax=8, iymax=6, izmax=64, pmlzi=8, pmlzf=57, hi=28, dh=8, dx=3, eps_bbox=2.5, tfsf_o=14,
                   box=(1, 8, 1, 6, 12, 50), nof=3, nt=160, no_downloads=160, lno=450.0, dlambda=100.0)
@@ -47,6 +48,7 @@ def make_setup(tmp_path, case):
                   box=(1, 70, 1, 21, 15, 80), nof=4, nt=200, no_downloads=200, lno=450.0, dlambda=100.0)
     else:
         raise KeyError(case)
+    kw.update(over)
     s = load_input(write_input(tmp_path, inputs.render(**kw)))
     if case == "bloch_lossy_ipol1":
         p = s.params.copy()
@@ -294,7 +296,9 @@ def test_two_step_adjoint_with_bloch_phase(tmp_path, monkeypatch, precision):
     from pyiris.sweep import oblique
     print(f"[gpu-test] two-step adjoint bloch precision={precision}", flush=True)
     monkeypatch.setenv("IRIS_TWO_STEP_UPML", "1")   # the UPML slabs in the two-step pass too
-    s = oblique(make_setup(tmp_path, "two_step"), 30.0)
+    # 700 steps: the pulse has crossed the structure and reached both flux
+    # faces (the objective's weights sit on J5 and J6)
+    s = oblique(make_setup(tmp_path, "two_step", nt=700, no_downloads=700), 30.0)
     p = s.params
     w = np.zeros((p.nof, 6))
     w[:, 4:] = weights_for(p)[:, 4:]

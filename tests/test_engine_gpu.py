@@ -25,7 +25,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from conftest import (GOLDEN_CASES, PKG, dft_rel_err, face_rel_err, golden_path, load_golden,
+from conftest import (GOLDEN_CASES, PKG, dft_rel_err, face_rel_err, golden_path, load_golden, oracle_for,
                       run_with_cadence, write_input)
 import pyoracle
 from pyiris import Engine, IrisError, inputs, load_input
@@ -505,7 +505,7 @@ def test_two_step_pass_with_bloch_phase(tmp_path, name, precision, width):
     for k in one["dft"]:
         assert _nrel(two["dft"][k], one["dft"][k]) < tol, f"dft {k}"
     if precision == 8:
-        o = pyoracle.Oracle(s)
+        o = oracle_for(s)
         ro = run_with_cadence(o, s.params, s.params.nbox)
         o.close()
         for b in ro:
