@@ -153,6 +153,19 @@ int iris_set_field(iris_handle* h, int32_t which, const double* in);
 int iris_get_plane(iris_handle* h, int32_t what, int32_t axis, int32_t index, const int32_t range[4],
                    double* out);
 
+/* iris_get_plane without any device or host allocation.  No counterpart in
+ * the reference; for callers that cannot allocate (code under HIP graph
+ * capture elsewhere in the process, tight output loops, harnesses that
+ * forbid allocation).  Same arguments, layout and errors as iris_get_plane,
+ * plus IRIS_EINVAL when the plane holds more than 1048576 (2^20) values:
+ * the values are staged in a fixed 16 MiB device array that the library
+ * keeps on every GPU it runs on until the process ends.  Every call on one
+ * GPU shares that GPU's array, so calls that overlap on one GPU (from
+ * several threads, with handles on the same device) race; calls on one
+ * handle are ordered by its stream.  Synchronous. */
+int iris_get_plane_noalloc(iris_handle* h, int32_t what, int32_t axis, int32_t index, const int32_t range[4],
+                           double* out);
+
 /* Field monitors after the step most recently completed, over x in 1..nx,
  * y in 1..ny, z in zi..zf (within pmlzi+1..pmlzf-1, outside the UPML):
  *   out[0] = sum Re(eps_i conj(E_i^n) E_i^(n-1))    (electric part)

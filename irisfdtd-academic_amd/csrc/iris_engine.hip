@@ -20,6 +20,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <utility>
 #include <string>
 #include <vector>
@@ -44,6 +45,28 @@ constexpr int kFoldWaves = 16, kFoldGroups = 8;
 template <typename T> constexpr int dft_slots() { return sizeof(T) == 8 ? kDftSlots64 : kDftSlots32; }
 
 thread_local std::string g_create_error = "no error";
+
+// Staging of iris_get_plane_noalloc: a fixed device array instead of the
+// handle's growable buffer, for callers that may not allocate.  In host code
+// `scratch` names a host-side stand-in; the device address of each GPU's
+// copy comes from hipGetSymbolAddress, once per device (at iris_create, with
+// that device current), and only that address is used.
+constexpr int64_t kPlaneScratch = int64_t(1) << 20;  // complex values (16 MiB of double2)
+static __device__ __align__(256) double2 scratch[kPlaneScratch];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Resolve the current device's address of `scratch` (no-op when known).
+hipError_t scratch_setup(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[dev]) return hipSuccess;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e == hipSuccess) g_scratch_ptr[dev] = ptr;
+  return e;
+}
 
 struct DeviceBox {
   BoxGeom g{};
@@ -2726,6 +2749,11 @@ int iris_create(const iris_params* p, const double* eps_inv, const double* fpzmn
   (void)hipGetDevice(&h->device);
   if (hipDeviceGetAttribute(&h->ncu, hipDeviceAttributeMultiprocessorCount, h->device) != hipSuccess || h->ncu < 1)
     h->ncu = 256;
+  e = scratch_setup(h->device);
+  if (e != hipSuccess) {
+    delete h;
+    return fail(nullptr, IRIS_EDEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+  }
   e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     delete h;
@@ -3013,6 +3041,56 @@ int iris_get_plane(iris_handle* h, int32_t what, int32_t axis, int32_t index, co
     for (int64_t i = 0; i < np; ++i) {
       out[2 * i] = buf[(size_t)i].x;
       out[2 * i + 1] = buf[(size_t)i].y;
+    }
+  }
+  return 0;
+}
+
+int iris_get_plane_noalloc(iris_handle* h, int32_t what, int32_t axis, int32_t index, const int32_t range[4],
+                           double* out) {
+  if (!h || !range || !out) return fail(h, IRIS_EINVAL, "bad argument");
+  const iris_params& p = h->p;
+  if (what < 0 || what > 8) return fail(h, IRIS_EINVAL, "iris_get_plane_noalloc: what must be 0..8");
+  if (axis < 1 || axis > 3) return fail(h, IRIS_EINVAL, "iris_get_plane_noalloc: axis must be 1, 2 or 3");
+  const int n[3] = {p.nx, p.ny, p.nz};
+  const int fa = axis == 1 ? 1 : 0, fb = axis == 3 ? 1 : 2;  // free axes
+  if (index < 1 || index > n[axis - 1] || range[0] < 1 || range[0] > range[1] || range[1] > n[fa] ||
+      range[2] < 1 || range[2] > range[3] || range[3] > n[fb])
+    return fail(h, IRIS_EINVAL, "iris_get_plane_noalloc: index %d or range [%d,%d]x[%d,%d] outside 1..n of axis %d",
+                index, range[0], range[1], range[2], range[3], axis);
+  const int na = range[1] - range[0] + 1, nb = range[3] - range[2] + 1;
+  const int64_t np = (int64_t)na * nb;
+  if (np > kPlaneScratch)
+    return fail(h, IRIS_EINVAL, "iris_get_plane_noalloc: %lld values exceed the %lld of its fixed staging array "
+                "(use iris_get_plane)", (long long)np, (long long)kPlaneScratch);
+  const int base = what / 3, comp = what % 3;
+  const void* f;
+  if (base == 0) f = h->fe;
+  else if (base == 2) f = h->fh;
+  else if (h->fused) f = h->gh;  // the fused pass read H(it-1) from here and wrote H(it) into fh
+  else return fail(h, IRIS_EUNSUP, "iris_get_plane_noalloc: H of the previous step is kept only by the fused pass");
+  void* stage = g_scratch_ptr[h->device];  // resolved at iris_create on this device
+  if (!stage) return fail(h, IRIS_ESTATE, "iris_get_plane_noalloc: staging array of device %d unknown", h->device);
+  const char* src = (const char*)f + h->csize * (size_t)h->cs * comp;
+  const dim3 grid((unsigned)((np + 255) / 256));
+  if (h->prec == 8)
+    hipLaunchKernelGGL(k_plane<double>, grid, dim3(256), 0, h->stream, (const double2*)src, axis, index, range[0], na,
+                       range[2], nb, h->sx, h->sxy, h->xoff, (double2*)stage);
+  else
+    hipLaunchKernelGGL(k_plane<float>, grid, dim3(256), 0, h->stream, (const float2*)src, axis, index, range[0], na,
+                       range[2], nb, h->sx, h->sxy, h->xoff, (float2*)stage);
+  HIP_TRY(h, hipGetLastError());
+  HIP_TRY(h, hipMemcpyAsync(out, stage, h->csize * (size_t)np, hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  if (h->prec == 4) {
+    // widen the float2 values in place, from the last one down (value i
+    // moves from bytes 8i to 16i, never over a value not yet moved)
+    unsigned char* bytes = (unsigned char*)out;
+    for (int64_t i = np - 1; i >= 0; --i) {
+      float v[2];
+      std::memcpy(v, bytes + 8 * i, sizeof v);
+      const double w[2] = {v[0], v[1]};
+      std::memcpy(bytes + 16 * i, w, sizeof w);
     }
   }
   return 0;

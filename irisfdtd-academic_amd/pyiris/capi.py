@@ -91,6 +91,8 @@ def load_library(path=None):
     lib.iris_set_field.restype = ct.c_int
     lib.iris_get_plane.argtypes = [H, ct.c_int32, ct.c_int32, ct.c_int32, ct.c_void_p, ct.c_void_p]
     lib.iris_get_plane.restype = ct.c_int
+    lib.iris_get_plane_noalloc.argtypes = [H, ct.c_int32, ct.c_int32, ct.c_int32, ct.c_void_p, ct.c_void_p]
+    lib.iris_get_plane_noalloc.restype = ct.c_int
     lib.iris_diagnostics.argtypes = [H, ct.c_int32, ct.c_int32, ct.c_void_p]
     lib.iris_diagnostics.restype = ct.c_int
     lib.iris_pass_info.argtypes = [H, ct.c_void_p]
@@ -139,7 +141,7 @@ EXPORTED = ["iris_create", "iris_step", "iris_sync", "iris_flux", "iris_dft_poin
             "iris_pass_info", "iris_autotune", "iris_graph_info", "iris_checkpoint", "iris_checkpoint_info",
             "iris_adjoint",
             "iris_batch_create", "iris_batch_step", "iris_batch_info", "iris_batch_destroy", "iris_batch_last_error",
-            "iris_incident_table"]
+            "iris_incident_table", "iris_get_plane_noalloc"]
 
 KERNELS = ["epass", "hpass", "tfsf", "dft_sample", "dft_accum", "flux", "yee", "yee2", "yee_rest", "adjoint",
            "checkpoint", "yee2_pml"]
@@ -235,6 +237,18 @@ class Engine:
         out = np.zeros(2 * max(na, 0) * max(nb, 0), dtype=np.float64)
         self._check(self.lib.iris_get_plane(self._h, int(what), int(axis), int(index), rng, _p(out)),
                     "iris_get_plane")
+        return out.view(np.complex128).reshape(nb, na)
+
+    def plane_noalloc(self, what, axis, index, a, b, out=None):
+        """plane() through iris_get_plane_noalloc: no allocation in the
+        library (planes of at most 2^20 values); `out` (float64, 2*na*nb) may
+        be given to avoid one here too."""
+        na, nb = a[1] - a[0] + 1, b[1] - b[0] + 1
+        rng = (ct.c_int32 * 4)(a[0], a[1], b[0], b[1])
+        if out is None:
+            out = np.zeros(2 * max(na, 0) * max(nb, 0), dtype=np.float64)
+        self._check(self.lib.iris_get_plane_noalloc(self._h, int(what), int(axis), int(index), rng, _p(out)),
+                    "iris_get_plane_noalloc")
         return out.view(np.complex128).reshape(nb, na)
 
     def diagnostics(self, zi=None, zf=None):
