@@ -600,7 +600,7 @@ COPY_BYTES = 1 << 31
 CLASSES = [
     ("yee2_batch", r"k_yee2_batch"),
     ("yee2_pml", r"k_yee2<\w+, \w+, true\b"),
-    ("yee2", r"k_yee2<"),
+    ("yee2", r"k_yee2<|k_yee2p<"),
     ("yee_batch", r"k_yee_fused_batch"),
     ("yee", r"k_yee_fused<"),
     ("epass", r"k_epass"),
@@ -651,9 +651,10 @@ def kernel_class(name):
 
 def precision_of(name):
     """Field precision from the kernel's template arguments (None for the
-    kernels that have none, such as the phase table)."""
+    kernels that have none, such as the phase table; k_yee2p holds FP32
+    fields only)."""
     tmpl = name.split("(")[0]
-    if "<float" in tmpl:
+    if "<float" in tmpl or "k_yee2p<" in tmpl:
         return 4
     if "<double" in tmpl:
         return 8
@@ -702,7 +703,10 @@ def _read_counter(d, counter):
 
 
 def report_pmc(fetch_dir, write_dir, steps, out_dir, cmd_fetch, cmd_write):
-    res, nlaunch = {}, {}
+    # launches of a class: the dispatches of its most frequent kernel (the
+    # two-step pass of FP32 fields over nx = 512 is two kernels a launch:
+    # 124-cell tiles of two cells a lane and the strip of 16 columns beside)
+    res, per_kernel = {}, {}
     for counter, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
         rows = _read_counter(d, counter)
         copies = [float(r["Counter_Value"]) for r in rows if kernel_class(r["Kernel_Name"]) == "copy"]
@@ -717,8 +721,10 @@ def report_pmc(fetch_dir, write_dir, steps, out_dir, cmd_fetch, cmd_write):
                 continue
             sums[(cls, prec)] = sums.get((cls, prec), 0.0) + float(r["Counter_Value"]) * 1024.0 * scale
             if counter == "FETCH_SIZE":
-                nlaunch[(cls, prec)] = nlaunch.get((cls, prec), 0) + 1
+                k = per_kernel.setdefault((cls, prec), {})
+                k[r["Kernel_Name"]] = k.get(r["Kernel_Name"], 0) + 1
         res[counter] = {"copy_kib": ref_kib, "scale": scale, "bytes": sums}
+    nlaunch = {k: max(v.values()) for k, v in per_kernel.items()}
     os.makedirs(out_dir, exist_ok=True)
     md = ["Counter runs of `python3 bench.py --pmc-probe` (512x512x1024 slit-array workload,",
           f"{steps} steps per precision).", ""]

@@ -234,14 +234,15 @@ int iris_pass_info(const iris_handle* h, int64_t info[6]);
  * tile widths of both, the dealing of tiles to XCDs -- and keeps the
  * fastest.  Only layouts with the
  * same results bit for bit are compared (with a Bloch phase the choice of the
- * two-step pass stays as it is).  The field state is restored and the handle
+ * two-step pass, of its UPML variant and the tile widths of both stay as they
+ * are).  The field state is restored and the handle
  * stays at step 0.  Needs device memory for one copy of the state; call it
  * before iris_checkpoint.  info[0] = two-step pass on, info[1] = UPML planes
  * in it, info[2] / info[3] = planes per single-step / two-step chunk,
  * info[4] / info[5] = tile rows per XCD run of the single-step / two-step
  * pass, info[6] = layouts timed, info[7] = ns per step of the one kept,
- * info[8] = footprint cells per row of a two-step tile (64 or 32; 0 when
- * the two-step pass is off). */
+ * info[8] = footprint cells per row of a two-step tile (64, 32, or 128 for
+ * two cells a lane with FP32 fields; 0 when the two-step pass is off). */
 int iris_autotune(iris_handle* h, int32_t steps, int64_t info[9]);
 
 /* Steps as a captured HIP graph.  iris_step runs whole periods of steps --

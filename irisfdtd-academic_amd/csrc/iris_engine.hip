@@ -29,6 +29,7 @@
 #include "iris_adjoint.h"
 #include "iris_incident.h"
 #include "iris_kernels.h"
+#include "iris_yee2p.h"
 
 using namespace iris;
 
@@ -126,7 +127,7 @@ struct PassConfig {
   int zc2 = 0;        // planes per two-step chunk (0: 2*zc)
   int xcd_rows = 0;   // tile rows per XCD run of the single-step pass (0: identity order)
   int xcd_rows2 = 4;  // the same for the two-step pass
-  int w2 = 0;         // two-step tile width, 64 or 32 cells (0: 64)
+  int w2 = 0;         // two-step tile width: 64, 32 or 128 (two cells a lane, FP32 fields); 0: by the grid
   int w2p = 0;        // the same for its UPML variant (0: 32)
   int nt2 = -1;       // two-step pass streams its outputs past the L2: 0 no, 1 yes, -1 up to 160 x 160 cells a plane
 };
@@ -189,6 +190,11 @@ struct iris_handle {
   int t2w = 64;             // two-step tile width (footprint cells per row)
   int t2wp = 64;            // the same for its UPML variant
   int tx2p = 0;             // tile columns of the UPML variant
+  // 128-wide tiles of two cells a lane cover x in 1..124*tx2 when nx is not
+  // a multiple of 124: the other columns take a strip of the one-cell
+  // kernel's tiles (width w2s, tx2s x ty2s tiles, footprint from x = x0s)
+  bool t2split = false;
+  int tx2s = 0, ty2s = 0, w2s = 0, x0s = 0;
   int nt2 = 0;              // two-step pass: streaming stores of its outputs
   PassConfig pass;          // the layout the lists were built for
   int64_t pass_gen = 0;     // bumped by every rebuild of the pass lists
@@ -302,11 +308,13 @@ struct iris_batch {
   // the two-step handles (`own`), batched: pairs of steps by one launch per
   // kernel over all of them.  Tables per pair parity (the pair runs from the
   // current buffers to the third ones, the next pair back).
-  // by kernels: Bloch phase (bit 0), 32-cell tiles of the plain pass (bit 1) and of its UPML variant (bit 2)
-  std::vector<iris_handle*> own2[8];
-  void* y2[8][2] = {};                // Batch2Inst<T>[2][n]: [group][upml variant], pair parity outer
-  int* y2wg[8][2] = {};               // workgroup prefixes
-  int y2nwg[8][2] = {};
+  // by kernels: Bloch phase (bit 0), 32-cell tiles of the plain pass (bit 1) and of its UPML variant (bit 2),
+  // 128-cell tiles of two cells a lane of the plain pass (bit 3); groups 16..31
+  // hold the strips beside such tiles (bit 0 phase, bit 1 32-cell tiles)
+  std::vector<iris_handle*> own2[32];
+  void* y2[32][2] = {};               // Batch2Inst<T>[2][n]: [group][upml variant], pair parity outer
+  int* y2wg[32][2] = {};              // workgroup prefixes
+  int y2nwg[32][2] = {};
   void* rinst = nullptr;              // BatchInst<T>[2][nr]: their single-step chunks, per (handle, length group)
   int nr = 0;
   int* rwg[2] = {nullptr, nullptr};   // per class (lean, general)
@@ -515,6 +523,23 @@ template <typename T, bool PH, bool PML>
 void launch_yee2(const iris_handle* h, const StepArgs<T>& a, cplx<T>* eb, cplx<T>* hb, cplx<T>* db,
                  const int* chunks, const TileMap& tm, const Yee2Samples<T>& smp) {
   const dim3 grid((unsigned)tm.nblk);
+  if constexpr (sizeof(T) == 4 && !PML) {
+    if (h->t2w == 128) {
+      hipLaunchKernelGGL((k_yee2p<PH>), grid, dim3(64, kP2R), 0, h->stream, a, eb, hb, chunks, tm, smp);
+      if (h->t2split) {
+        // the strip of columns beside the 124-cell tiles, same chunks and buffers
+        const int nch = tm.nblk / (tm.tx * tm.ty);
+        const TileMap ts{h->tx2s, h->ty2s, h->tx2s * h->ty2s * nch, h->tx2s * h->xcd_rows2, h->x0s};
+        if (h->w2s == 32)
+          hipLaunchKernelGGL((k_yee2<T, PH, false, 32>), dim3((unsigned)ts.nblk), dim3(kT2X, Yee2Shape<T, false, 32>::R),
+                             0, h->stream, a, eb, hb, db, chunks, ts, smp);
+        else
+          hipLaunchKernelGGL((k_yee2<T, PH, false, 64>), dim3((unsigned)ts.nblk), dim3(kT2X, Yee2Shape<T, false, 64>::R),
+                             0, h->stream, a, eb, hb, db, chunks, ts, smp);
+      }
+      return;
+    }
+  }
   if ((PML ? h->t2wp : h->t2w) == 32)
     hipLaunchKernelGGL((k_yee2<T, PH, PML, 32>), grid, dim3(kT2X, Yee2Shape<T, PML, 32>::R), 0, h->stream, a, eb,
                        hb, db, chunks, tm, smp);
@@ -1112,6 +1137,15 @@ template <typename T>
 T2Geom t2geom(int W, bool pml) {
   T2Geom g;
   g.W = W;
+  if (W == 128) {
+    // two x-cells per lane (k_yee2p): FP32 fields, plain chunks, one row per wave
+    g.R = kP2R;
+    g.per_cu = 1;
+    g.RY = kP2R;
+    g.OX = kP2OX;
+    g.OY = kP2OY;
+    return g;
+  }
   if (W == 32) {
     g.R = pml ? Yee2Shape<T, true, 32>::R : Yee2Shape<T, false, 32>::R;
     g.per_cu = pml ? Yee2Shape<T, true, 32>::WG_PER_CU : Yee2Shape<T, false, 32>::WG_PER_CU;
@@ -1209,8 +1243,20 @@ int build_passes(iris_handle* h, const PassConfig& c) {
     // FP32 fields the 32-wide tiles (8 waves, two workgroups a CU) measured
     // within noise of the 64-wide ones at 512 x 512 x 1024 and 12% faster at
     // 128 x 128 x 768; on the 64-grating sweep, batched, 3% slower.
-    int W = (c.w2 == 32 || c.w2 == 64) ? c.w2 : 64;
-    if (!two_step_fits(p, t2geom<T>(W, false)) && two_step_fits(p, t2geom<T>(96 - W, false))) W = 96 - W;
+    // 128: two x-cells per lane (k_yee2p), with FP32 fields on grids of even
+    // nx only (a lane's pair of cells must not straddle the x period), and
+    // the default there: 124 cells a tile row, the r = nx mod 124 columns
+    // beside taken by 32-wide tiles (r <= 29) or 64-wide ones.  Measured at
+    // ny = 512, nz = 1024 against 64-wide tiles: 21% faster at nx = 64, 13%
+    // at 100, 5% at 128, 13% at 250 and 256, 8% at 376, 22-28% at 496, 17%
+    // at 512, 15% at 1024; 14% slower at 160 (a 64-wide strip over 36
+    // columns beside one tile column), the case left at 64.
+    const bool pair_ok = sizeof(T) == 4 && p.nx % 2 == 0 && two_step_fits(p, t2geom<T>(128, false));
+    const int r124 = p.nx % kP2OX;
+    const bool pair_default = pair_ok && (p.nx < kP2OX || r124 <= 29 || p.nx >= 4 * kP2OX);
+    int W = (c.w2 == 32 || c.w2 == 64 || (c.w2 == 128 && pair_ok)) ? c.w2 : (c.w2 == 0 && pair_default ? 128 : 64);
+    if (W != 128 && !two_step_fits(p, t2geom<T>(W, false)) && two_step_fits(p, t2geom<T>(96 - W, false)))
+      W = 96 - W;
     h->t2w = W;
     // the UPML variant's own width, 32 by default: its slabs' launch is a
     // few rounds of workgroups, and with FP32 fields 32-wide tiles of 12
@@ -1393,6 +1439,21 @@ int build_passes(iris_handle* h, const PassConfig& c) {
       h->ty2 = g2.ty(p.ny);
       h->tx2p = g2p.tx(p.nx);
       h->ty2p = g2p.ty(p.ny);
+      h->t2split = false;
+      if (W == 128 && p.nx >= kP2OX && p.nx % kP2OX != 0) {
+        // whole 124-cell tiles over x in 1..124*tx2, the rest (r columns) by
+        // a strip of 32- or 64-wide tiles: a fifth 128-wide tile column
+        // over 16 columns (nx = 512) would idle most of its lanes
+        const int r = p.nx % kP2OX;
+        const int ws = r <= t2geom<T>(32, false).OX ? 32 : 64;
+        const T2Geom gs = t2geom<T>(ws, false);
+        h->t2split = true;
+        h->tx2 = p.nx / kP2OX;
+        h->w2s = ws;
+        h->tx2s = (r + gs.OX - 1) / gs.OX;
+        h->ty2s = gs.ty(p.ny);
+        h->x0s = kP2OX * h->tx2;
+      }
       dfree(h, (void**)&h->chunks2);
       dfree(h, (void**)&h->chunks2p);
       dfree(h, (void**)&h->chunks_rest);
@@ -2102,15 +2163,19 @@ int autotune_impl(iris_handle* h, int steps, int64_t info[9]) {
   if (!rc) rc = build_passes<T>(h, best);
   const bool two = h->two_step;
   if (!rc && two) {  // tile width and output stores of the two-step pass
+    // with a Bloch phase the tile width stays: the lanes that stand for
+    // cells across the x period take their neighbours from other lanes or
+    // from the ghost planes depending on the tiles, the same values up to
+    // the rounding of the phase factors
     const int w0 = h->t2w;
-    for (int v : {64, 32}) {
+    for (int v : {64, 32, 128}) {
       PassConfig c = best;
       c.w2 = v;
-      if (v != w0) consider(c);
+      if (!h->bloch && v != w0 && (v != 128 || (sizeof(T) == 4 && h->p.nx % 2 == 0))) consider(c);
     }
     if (!rc) rc = build_passes<T>(h, best);
     const int wp0 = h->t2wp;
-    if (!rc && h->upml2p > 0)
+    if (!rc && h->upml2p > 0 && !h->bloch)
       for (int v : {64, 32}) {
         PassConfig c = best;
         c.w2p = v;
@@ -2252,11 +2317,12 @@ template <typename T>
 int batch_tables2(iris_batch* b) {
   using C = cplx<T>;
   if (b->own.empty()) return 0;
-  for (int bl = 0; bl < 8; ++bl) {
+  for (int bl = 0; bl < 32; ++bl) {
     const std::vector<iris_handle*>& hs = b->own2[bl];
     const int n = (int)hs.size();
     if (!n) continue;
-    for (int pm = 0; pm < 2; ++pm) {
+    const bool strip = bl >= 16;
+    for (int pm = 0; pm < (strip ? 1 : 2); ++pm) {
       std::vector<Batch2Inst<T>> I((size_t)2 * n);
       for (int pp = 0; pp < 2; ++pp)
         for (int i = 0; i < n; ++i) {
@@ -2270,8 +2336,8 @@ int batch_tables2(iris_batch* b) {
           B.hb = (C*)h->gh;
           B.db = pm ? (C*)h->dn : nullptr;
           B.chunks = pm ? h->chunks2p : h->chunks2;
-          const int tx = pm ? h->tx2p : h->tx2, ty = pm ? h->ty2p : h->ty2;
-          B.tm = TileMap{tx, ty, tx * ty * (pm ? h->n2p : h->n2), tx * h->xcd_rows2};
+          const int tx = strip ? h->tx2s : (pm ? h->tx2p : h->tx2), ty = strip ? h->ty2s : (pm ? h->ty2p : h->ty2);
+          B.tm = TileMap{tx, ty, tx * ty * (pm ? h->n2p : h->n2), tx * h->xcd_rows2, strip ? h->x0s : 0};
           B.smp = Yee2Samples<T>{};
           B.smp.nbox = h->p.nbox;
           for (int bx = 0; bx < h->p.nbox; ++bx) {
@@ -2331,15 +2397,20 @@ int batch_tables2(iris_batch* b) {
 template <typename T>
 int batch_create2(iris_batch* b) {
   int rc;
-  for (iris_handle* h : b->own)
-    b->own2[(h->bloch ? 1 : 0) + (h->t2w == 32 ? 2 : 0) + (h->t2wp == 32 ? 4 : 0)].push_back(h);
-  for (int bl = 0; bl < 8; ++bl) {
+  for (iris_handle* h : b->own) {
+    b->own2[(h->bloch ? 1 : 0) + (h->t2w == 32 ? 2 : 0) + (h->t2wp == 32 ? 4 : 0) + (h->t2w == 128 ? 8 : 0)]
+        .push_back(h);
+    if (h->t2w == 128 && h->t2split) b->own2[16 + (h->bloch ? 1 : 0) + (h->w2s == 32 ? 2 : 0)].push_back(h);
+  }
+  for (int bl = 0; bl < 32; ++bl) {
     const std::vector<iris_handle*>& hs = b->own2[bl];
     if (hs.empty()) continue;
-    for (int pm = 0; pm < 2; ++pm) {
+    const bool strip = bl >= 16;
+    for (int pm = 0; pm < (strip ? 1 : 2); ++pm) {
       std::vector<int> wg = {0};
       for (iris_handle* h : hs) {
-        const int64_t nb = (int64_t)(pm ? h->tx2p : h->tx2) * (pm ? h->ty2p : h->ty2) * (pm ? h->n2p : h->n2);
+        const int64_t nb = strip ? (int64_t)h->tx2s * h->ty2s * h->n2
+                                 : (int64_t)(pm ? h->tx2p : h->tx2) * (pm ? h->ty2p : h->ty2) * (pm ? h->n2p : h->n2);
         wg.push_back(wg.back() + (int)((nb + kNumXcd - 1) / kNumXcd * kNumXcd));
       }
       if ((rc = balloc(b, (void**)&b->y2wg[bl][pm], sizeof(int) * wg.size())) ||
@@ -2402,6 +2473,12 @@ int batch_sample2(iris_batch* b, int half, int par, int n) {
 template <typename T, bool PH, bool PML>
 void launch_yee2_batch(const iris_batch* b, int W, unsigned nblk, const Batch2Inst<T>* I, const int* wg, int n,
                        int slot) {
+  if constexpr (sizeof(T) == 4 && !PML) {
+    if (W == 128) {
+      hipLaunchKernelGGL((k_yee2p_batch<PH>), dim3(nblk), dim3(64, kP2R), 0, b->stream, I, wg, n, slot);
+      return;
+    }
+  }
   if (W == 32)
     hipLaunchKernelGGL((k_yee2_batch<T, PH, PML, 32>), dim3(nblk), dim3(kT2X, Yee2Shape<T, PML, 32>::R), 0,
                        b->stream, I, wg, n, slot);
@@ -2436,13 +2513,13 @@ int batch_run2(iris_batch* b, int it_first, int it_last) {
     const int slot = s1 ? b->own[0]->nsamp : -1;
     int rc;
     rest(0, it);
-    for (int bl = 0; bl < 8; ++bl) {
+    for (int bl = 0; bl < 32; ++bl) {
       const int n = (int)b->own2[bl].size();
       for (int pm = 0; pm < 2; ++pm) {
         if (!b->y2nwg[bl][pm]) continue;
         const Batch2Inst<T>* I = (const Batch2Inst<T>*)b->y2[bl][pm] + (size_t)pp * n;
         const int* w = b->y2wg[bl][pm];
-        const int W = (bl & (pm ? 4 : 2)) ? 32 : 64;
+        const int W = bl >= 16 ? ((bl & 2) ? 32 : 64) : ((!pm && (bl & 8)) ? 128 : ((bl & (pm ? 4 : 2)) ? 32 : 64));
         const unsigned g = (unsigned)b->y2nwg[bl][pm];
         if (bl & 1) {
           if (pm) launch_yee2_batch<T, true, true>(b, W, g, I, w, n, slot);

@@ -408,6 +408,8 @@ struct TileMap {
   int tx, ty;  // tiles along x and y
   int nblk;    // workgroups in the launch (tx * ty * chunks)
   int run;     // tiles per run (0: identity order)
+  int x0 = 0;  // two-step pass: x of the footprint's first cell in tile column 0, less its usual place
+               // (a strip of columns beside tiles of another kernel)
 };
 constexpr int kNumXcd = 8;
 

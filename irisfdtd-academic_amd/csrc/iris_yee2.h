@@ -179,7 +179,7 @@ template <typename T> struct Yee2Samples {
 template <typename T, bool PH, bool PML, int kT2W>
 __device__ __forceinline__ void yee2_tile(const StepArgs<T>& a, cplx<T>* __restrict__ eb, cplx<T>* __restrict__ hb,
                                           cplx<T>* __restrict__ db, const int* __restrict__ chunks, int bx, int by,
-                                          int bz, const Yee2Samples<T>& smp, int slot) {
+                                          int bz, const Yee2Samples<T>& smp, int slot, int xbase) {
   using C = cplx<T>;
   constexpr int kT2R = Yee2Shape<T, PML, kT2W>::RY, kT2OX = Yee2Shape<T, PML, kT2W>::OX,
                 kT2OY = Yee2Shape<T, PML, kT2W>::OY;
@@ -203,7 +203,7 @@ __device__ __forceinline__ void yee2_tile(const StepArgs<T>& a, cplx<T>* __restr
   const bool exp_below = (craw & kChunkExportBelow) != 0, exp_above = (craw & kChunkExportAbove) != 0;
 
   // footprint cell of this lane and its periodic source
-  const int xg = bx * kT2OX + cx, yg = by * kT2OY + cy;  // x0-1+cx, y0-1+cy
+  const int xg = xbase + bx * kT2OX + cx, yg = by * kT2OY + cy;  // x0-1+cx, y0-1+cy
   const int xs = xg < 1 ? xg + nx : (xg > nx ? xg - nx : xg);
   const int ys = yg < 1 ? yg + ny : (yg > ny ? yg - ny : yg);
   const bool own = cx >= 1 && cx <= kT2OX && cy >= 1 && cy <= kT2OY && xg <= nx && yg <= ny;
@@ -399,7 +399,7 @@ __global__ __launch_bounds__((kT2X * Yee2Shape<T, PML, W>::R), (Yee2Shape<T, PML
     const int* __restrict__ chunks, TileMap tm, Yee2Samples<T> smp) {
   int bx, by, bz;
   tile_of(tm, blockIdx.x, &bx, &by, &bz);
-  yee2_tile<T, PH, PML, W>(a, eb, hb, db, chunks, bx, by, bz, smp, 0);
+  yee2_tile<T, PH, PML, W>(a, eb, hb, db, chunks, bx, by, bz, smp, 0, tm.x0);
 }
 
 // Batched two-step pass: the chunks of many grids in one launch.  Entry i of
@@ -429,7 +429,7 @@ __global__ __launch_bounds__((kT2X * Yee2Shape<T, PML, W>::R), (Yee2Shape<T, PML
   if (L >= I.tm.nblk) return;  // padding (no barrier has been reached)
   int bx, by, bz;
   tile_of(I.tm, L, &bx, &by, &bz);
-  yee2_tile<T, PH, PML, W>(I.a, I.eb, I.hb, I.db, I.chunks, bx, by, bz, I.smp, slot);
+  yee2_tile<T, PH, PML, W>(I.a, I.eb, I.hb, I.db, I.chunks, bx, by, bz, I.smp, slot, I.tm.x0);
 }
 
 // DFT samples of one step from a buffer that holds that step only on the

@@ -40,8 +40,8 @@ def mixed_setups(tmp_path):
                      eps_bbox=2.25, tfsf_o=20, box=(1, 70, 1, 21, 15, 80), nof=4), 20.0),
         oblique(make(tmp_path, "g", ixmax=130, iymax=40, izmax=80, pmlzi=9, pmlzf=71, hi=36, dh=8, dx=20,
                      eps_bbox=3.0, tfsf_o=18, box=(1, 130, 1, 40, 13, 66), nof=3), 40.0),
-        make(tmp_path, "h", ixmax=100, iymax=33, izmax=88, pmlzi=12, pmlzf=76, hi=40, dh=6, dx=10, eps_bbox=2.0,
-             tfsf_o=20, box=(1, 100, 1, 33, 15, 72), nof=3),
+        make(tmp_path, "h", ixmax=190, iymax=33, izmax=88, pmlzi=12, pmlzf=76, hi=40, dh=6, dx=10, eps_bbox=2.0,
+             tfsf_o=20, box=(1, 190, 1, 33, 15, 72), nof=3),
     ]
 
 
@@ -72,7 +72,7 @@ def test_grating_sweep_draws(tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("width", ["auto", "32", "32/64"])
+@pytest.mark.parametrize("width", ["auto", "32", "32/64", "128"])
 @pytest.mark.parametrize("upml", ["auto", "1"])
 @pytest.mark.parametrize("precision", [8, 4])
 def test_batch_is_bitwise_each_run_alone(tmp_path, monkeypatch, precision, upml, width):

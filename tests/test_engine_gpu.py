@@ -405,7 +405,12 @@ TWO_STEP_CASES = {
     "lossy_structure": dict(ixmax=64, iymax=16, izmax=128, pmlzi=10, pmlzf=118, hi=50, dh=30, dx=8,
                             eps_bbox=complex(4.0, 0.6), tfsf_o=20, box=(1, 64, 1, 16, 15, 110), nof=2, nt=50,
                             no_downloads=2),
+    # 128-cell tiles of two cells a lane over x in 1..124 and a strip of
+    # 64-cell tiles (two of them) over the other 64 columns
+    "strip64": dict(ixmax=188, iymax=10, izmax=100, pmlzi=10, pmlzf=90, hi=46, dh=8, dx=30, eps_bbox=3.0,
+                    tfsf_o=20, box=(2, 187, 1, 10, 15, 85), nof=2, nt=50, no_downloads=2),
     # region III different from I, box faces inside the top plain region
+    # (with 128-cell tiles a strip of one 32-cell tile over x in 125..130)
     "substrate": dict(ixmax=130, iymax=24, izmax=140, pmlzi=10, pmlzf=130, hi=40, dh=10, dx=20, eps_bbox=4.0,
                       eps3=2.25, tfsf_o=20, box=(10, 120, 3, 22, 60, 110), nof=2, nt=64, no_downloads=2),
 }
@@ -441,11 +446,12 @@ def _run_two_step(setup, precision, two_step, zchunk, width=0):
 @pytest.mark.parametrize("name", list(TWO_STEP_CASES))
 @pytest.mark.parametrize("precision", [8, 4])
 @pytest.mark.parametrize("zchunk", [1, 5, 16])
-@pytest.mark.parametrize("width", [64, 32])
+@pytest.mark.parametrize("width", [64, 32, 128])
 def test_two_step_pass_is_bitwise_two_single_steps(tmp_path, name, precision, zchunk, width):
     """Two steps per pass over the plain z-chunks (tiles of 64 or 32 cells a
-    row) give bit for bit the fields, DFT accumulators, fluxes and
-    previous-step H of single steps."""
+    row, or with FP32 fields and even nx 128 cells of two per lane) give bit
+    for bit the fields, DFT accumulators, fluxes and previous-step H of
+    single steps."""
     _announce(f"two-step {name} prec={precision} zchunk={zchunk} width={width}")
     s = load_input(write_input(tmp_path, inputs.render(**TWO_STEP_CASES[name])))
     one = _run_two_step(s, precision, False, zchunk)
@@ -477,9 +483,9 @@ def _nrel(a, b):
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
 
 
-@pytest.mark.parametrize("name", ["ipol2_2box", "ipol1_stride", "oblique", "narrow_x"])
+@pytest.mark.parametrize("name", ["ipol2_2box", "ipol1_stride", "oblique", "narrow_x", "substrate", "strip64"])
 @pytest.mark.parametrize("precision", [8, 4])
-@pytest.mark.parametrize("width", [64, 32])
+@pytest.mark.parametrize("width", [64, 32, 128])
 def test_two_step_pass_with_bloch_phase(tmp_path, name, precision, width):
     """With a Bloch phase in x and y the two-step pass's image lanes carry
     the phase: fields, DFT accumulators and fluxes equal those of single
