@@ -190,11 +190,10 @@ struct iris_handle {
   int t2w = 64;             // two-step tile width (footprint cells per row)
   int t2wp = 64;            // the same for its UPML variant
   int tx2p = 0;             // tile columns of the UPML variant
-  // 128-wide tiles of two cells a lane cover x in 1..124*tx2 when nx is not
-  // a multiple of 124: the other columns take a strip of the one-cell
-  // kernel's tiles (width w2s, tx2s x ty2s tiles, footprint from x = x0s)
-  bool t2split = false;
-  int tx2s = 0, ty2s = 0, w2s = 0, x0s = 0;
+  // two cells a lane (t2w = 128): full tiles (tx2 x ty2, 124 cells a row)
+  // over x in 1..124*tx2 and narrow tiles (tx2n x ty2n, p2ox cells a row in
+  // rows of p2sl lanes, p2ns rows a wave) over the columns from x0n + 1
+  int p2ox = 0, p2sl = 0, p2ns = 0, tx2n = 0, ty2n = 0, x0n = 0;
   int nt2 = 0;              // two-step pass: streaming stores of its outputs
   PassConfig pass;          // the layout the lists were built for
   int64_t pass_gen = 0;     // bumped by every rebuild of the pass lists
@@ -309,12 +308,14 @@ struct iris_batch {
   // kernel over all of them.  Tables per pair parity (the pair runs from the
   // current buffers to the third ones, the next pair back).
   // by kernels: Bloch phase (bit 0), 32-cell tiles of the plain pass (bit 1) and of its UPML variant (bit 2),
-  // 128-cell tiles of two cells a lane of the plain pass (bit 3); groups 16..31
-  // hold the strips beside such tiles (bit 0 phase, bit 1 32-cell tiles)
-  std::vector<iris_handle*> own2[32];
-  void* y2[32][2] = {};               // Batch2Inst<T>[2][n]: [group][upml variant], pair parity outer
-  int* y2wg[32][2] = {};              // workgroup prefixes
-  int y2nwg[32][2] = {};
+  // 128-cell tiles of two cells a lane of the plain pass (bit 3)
+  std::vector<iris_handle*> own2[16];
+  // table entries of a group's launch: (handle, part), the part being the
+  // full (0) or narrow (1) tiles of the two-cells-a-lane kernel, else 0
+  std::vector<std::pair<iris_handle*, int>> y2e[16][2];
+  void* y2[16][2] = {};               // Batch2Inst<T>[2][entries]: [group][upml variant], pair parity outer
+  int* y2wg[16][2] = {};              // workgroup prefixes
+  int y2nwg[16][2] = {};
   void* rinst = nullptr;              // BatchInst<T>[2][nr]: their single-step chunks, per (handle, length group)
   int nr = 0;
   int* rwg[2] = {nullptr, nullptr};   // per class (lean, general)
@@ -518,28 +519,26 @@ template <typename T>
 int dft_flush(iris_handle* h);
 
 // One launch of the two-step kernel (variant PH, PML) at the handle's tile
-// width, tm.nblk workgroups.
+// width over nch chunks (tile map tm; with two cells a lane the handle's
+// full and narrow tiles instead).
 template <typename T, bool PH, bool PML>
 void launch_yee2(const iris_handle* h, const StepArgs<T>& a, cplx<T>* eb, cplx<T>* hb, cplx<T>* db,
-                 const int* chunks, const TileMap& tm, const Yee2Samples<T>& smp) {
-  const dim3 grid((unsigned)tm.nblk);
+                 const int* chunks, const TileMap& tm, int nch, const Yee2Samples<T>& smp) {
   if constexpr (sizeof(T) == 4 && !PML) {
     if (h->t2w == 128) {
-      hipLaunchKernelGGL((k_yee2p<PH>), grid, dim3(64, kP2R), 0, h->stream, a, eb, hb, chunks, tm, smp);
-      if (h->t2split) {
-        // the strip of columns beside the 124-cell tiles, same chunks and buffers
-        const int nch = tm.nblk / (tm.tx * tm.ty);
-        const TileMap ts{h->tx2s, h->ty2s, h->tx2s * h->ty2s * nch, h->tx2s * h->xcd_rows2, h->x0s};
-        if (h->w2s == 32)
-          hipLaunchKernelGGL((k_yee2<T, PH, false, 32>), dim3((unsigned)ts.nblk), dim3(kT2X, Yee2Shape<T, false, 32>::R),
-                             0, h->stream, a, eb, hb, db, chunks, ts, smp);
-        else
-          hipLaunchKernelGGL((k_yee2<T, PH, false, 64>), dim3((unsigned)ts.nblk), dim3(kT2X, Yee2Shape<T, false, 64>::R),
-                             0, h->stream, a, eb, hb, db, chunks, ts, smp);
-      }
+      const P2Part p0{TileMap{h->tx2, h->ty2, h->tx2 * h->ty2 * nch, h->tx2 * h->xcd_rows2, 0}, kP2OX, 64, 1};
+      const P2Part p1{TileMap{h->tx2n, h->ty2n, h->tx2n * h->ty2n * nch, h->tx2n * h->xcd_rows2, h->x0n}, h->p2ox,
+                      h->p2sl, h->p2ns};
+      const int npad0 = p1.tm.nblk ? (p0.tm.nblk + kNumXcd - 1) / kNumXcd * kNumXcd : p0.tm.nblk;
+      const unsigned nblk = (unsigned)(npad0 + p1.tm.nblk);
+      if (nblk)
+        hipLaunchKernelGGL((k_yee2p<PH>), dim3(nblk), dim3(64, kP2R), 0, h->stream, a, (float2*)eb, (float2*)hb,
+                           chunks, p0, p1, npad0, smp);
       return;
     }
   }
+  (void)nch;
+  const dim3 grid((unsigned)tm.nblk);
   if ((PML ? h->t2wp : h->t2w) == 32)
     hipLaunchKernelGGL((k_yee2<T, PH, PML, 32>), grid, dim3(kT2X, Yee2Shape<T, PML, 32>::R), 0, h->stream, a, eb,
                        hb, db, chunks, tm, smp);
@@ -640,9 +639,9 @@ int launch_double(iris_handle* h, int it, const PassLists& L, bool dft) {
     const StepArgs<T> a2 = step_args_on<T>(h, h->fe, h->fh, h->ke, h->kh, h->d, h->b, h->dn, h->bn);
     const TileMap tm{h->tx2, h->ty2, h->tx2 * h->ty2 * L.n2, h->tx2 * h->xcd_rows2};
     if (h->bloch)
-      launch_yee2<T, true, false>(h, a2, (C*)h->ge, (C*)h->gh, nullptr, L.chunks2, tm, smp);
+      launch_yee2<T, true, false>(h, a2, (C*)h->ge, (C*)h->gh, nullptr, L.chunks2, tm, L.n2, smp);
     else
-      launch_yee2<T, false, false>(h, a2, (C*)h->ge, (C*)h->gh, nullptr, L.chunks2, tm, smp);
+      launch_yee2<T, false, false>(h, a2, (C*)h->ge, (C*)h->gh, nullptr, L.chunks2, tm, L.n2, smp);
   }
   if (L.n2p) {
     // the UPML variant: D and B of step n+2 into the third aux buffers
@@ -650,9 +649,9 @@ int launch_double(iris_handle* h, int it, const PassLists& L, bool dft) {
     const StepArgs<T> a2 = step_args_on<T>(h, h->fe, h->fh, h->ke, h->kh, h->d, h->b, h->kd, h->kb);
     const TileMap tm{h->tx2p, h->ty2p, h->tx2p * h->ty2p * L.n2p, h->tx2p * h->xcd_rows2};
     if (h->bloch)
-      launch_yee2<T, true, true>(h, a2, (C*)h->ge, (C*)h->gh, (C*)h->dn, L.chunks2p, tm, smp);
+      launch_yee2<T, true, true>(h, a2, (C*)h->ge, (C*)h->gh, (C*)h->dn, L.chunks2p, tm, L.n2p, smp);
     else
-      launch_yee2<T, false, true>(h, a2, (C*)h->ge, (C*)h->gh, (C*)h->dn, L.chunks2p, tm, smp);
+      launch_yee2<T, false, true>(h, a2, (C*)h->ge, (C*)h->gh, (C*)h->dn, L.chunks2p, tm, L.n2p, smp);
   }
   if (L.nlean + L.ngen) {
     ProfScope ps(h, IRIS_K_YEE_REST);
@@ -1158,6 +1157,60 @@ T2Geom t2geom(int W, bool pml) {
   g.OY = g.RY - 3;
   return g;
 }
+// Tiles of the two-cells-a-lane kernel (k_yee2p) over a grid of even nx:
+// full tiles (124 output cells a row, 9 rows) over x in 1 .. 124 tx, and
+// narrow tiles over the r = nx - 124 tx columns beside them (all of a grid
+// narrower than 124): tiles of ox output cells (even) in rows of
+// sl = ox/2 + 2 lanes, ns rows a wave, 12 ns - 3 output rows; the shape with
+// the fewest workgroups a chunk among those whose footprint stays within one
+// period of the grid.
+struct P2Layout {
+  int tx = 0, ty = 0;
+  int ox = 0, sl = 0, ns = 0, txn = 0, tyn = 0, x0n = 0;
+  bool ok = false;
+  int64_t tiles() const { return (int64_t)tx * ty + (int64_t)txn * tyn; }
+};
+P2Layout p2_layout(int nx, int ny) {
+  P2Layout L;
+  if (nx < 2 || nx % 2 || ny < 1) return L;
+  auto fits_y = [&](int oy, int ry, int ty) { return (int64_t)(ty - 1) * oy + ry - 1 <= 2LL * ny; };
+  L.tx = nx / kP2OX;
+  if (L.tx) {
+    L.ty = (ny + kP2OY - 1) / kP2OY;
+    if (!fits_y(kP2OY, kP2R, L.ty)) return L;
+  }
+  const int r = nx - kP2OX * L.tx;
+  if (r) {
+    int64_t best = -1;
+    for (int m = 1; m <= 8; ++m) {
+      const int ox = ((r + m - 1) / m + 1) / 2 * 2;
+      const int sl = ox / 2 + 2;
+      if (sl > 64) continue;
+      const int txn = (r + ox - 1) / ox;
+      if ((int64_t)kP2OX * L.tx + (int64_t)txn * ox + 2 > 2LL * nx) continue;
+      for (int ns = 64 / sl; ns >= 1; --ns) {
+        const int ry = kP2R * ns, oy = ry - 3;
+        const int tyn = (ny + oy - 1) / oy;
+        if (!fits_y(oy, ry, tyn)) continue;
+        const int64_t wg = (int64_t)txn * tyn;
+        if (best < 0 || wg < best) {
+          best = wg;
+          L.ox = ox;
+          L.sl = sl;
+          L.ns = ns;
+          L.txn = txn;
+          L.tyn = tyn;
+        }
+        break;
+      }
+    }
+    if (best < 0) return L;
+    L.x0n = kP2OX * L.tx;
+  }
+  L.ok = true;
+  return L;
+}
+
 bool two_step_fits(const iris_params& p, const T2Geom& g) {
   return (int64_t)(g.tx(p.nx) - 1) * g.OX + g.W - 1 <= 2LL * p.nx &&
          (int64_t)(g.ty(p.ny) - 1) * g.OY + g.RY - 1 <= 2LL * p.ny;
@@ -1245,15 +1298,14 @@ int build_passes(iris_handle* h, const PassConfig& c) {
     // 128 x 128 x 768; on the 64-grating sweep, batched, 3% slower.
     // 128: two x-cells per lane (k_yee2p), with FP32 fields on grids of even
     // nx only (a lane's pair of cells must not straddle the x period), and
-    // the default there: 124 cells a tile row, the r = nx mod 124 columns
-    // beside taken by 32-wide tiles (r <= 29) or 64-wide ones.  Measured at
-    // ny = 512, nz = 1024 against 64-wide tiles: 21% faster at nx = 64, 13%
-    // at 100, 5% at 128, 13% at 250 and 256, 8% at 376, 22-28% at 496, 17%
-    // at 512, 15% at 1024; 14% slower at 160 (a 64-wide strip over 36
-    // columns beside one tile column), the case left at 64.
-    const bool pair_ok = sizeof(T) == 4 && p.nx % 2 == 0 && two_step_fits(p, t2geom<T>(128, false));
-    const int r124 = p.nx % kP2OX;
-    const bool pair_default = pair_ok && (p.nx < kP2OX || r124 <= 29 || p.nx >= 4 * kP2OX);
+    // the default there: 124 cells a tile row, the columns beside (and grids
+    // narrower than that) in narrow tiles of the same launch (p2_layout).
+    // With 124-cell tiles alone, measured at ny = 512, nz = 1024 against
+    // 64-wide tiles: 21% faster at nx = 64, 13% at 100, 5% at 128, 13% at
+    // 250 and 256, 8% at 376, 22-28% at 496, 17% at 512, 15% at 1024.
+    const P2Layout pl = p2_layout(p.nx, p.ny);
+    const bool pair_ok = sizeof(T) == 4 && pl.ok;
+    const bool pair_default = pair_ok;
     int W = (c.w2 == 32 || c.w2 == 64 || (c.w2 == 128 && pair_ok)) ? c.w2 : (c.w2 == 0 && pair_default ? 128 : 64);
     if (W != 128 && !two_step_fits(p, t2geom<T>(W, false)) && two_step_fits(p, t2geom<T>(96 - W, false)))
       W = 96 - W;
@@ -1267,14 +1319,16 @@ int build_passes(iris_handle* h, const PassConfig& c) {
     if (!two_step_fits(p, t2geom<T>(Wp, true)) && two_step_fits(p, t2geom<T>(96 - Wp, true))) Wp = 96 - Wp;
     h->t2wp = Wp;
     const T2Geom g2 = t2geom<T>(W, false), g2p = t2geom<T>(Wp, true);
+    // two-step tiles a chunk of planes
+    const int64_t t2 = W == 128 ? pl.tiles() : (int64_t)g2.tx(p.nx) * g2.ty(p.ny);
     bool want = c.two != 0;
     if (c.two < 0) {
-      const double two = (double)g2.tx(p.nx) * g2.ty(p.ny) * g2.R / 2.0;
+      const double two = (double)t2 * g2.R / 2.0;
       const double one = (double)((p.nx + kFTX - 1) / kFTX) * ((p.ny + kFTY - 1) / kFTY) * (kFTY + 1);
       if (two > 0.95 * one) want = false;
     }
     h->bloch = !(p.ak[0][0] == 0.0 && p.ak[0][1] == 0.0 && p.ak[1][0] == 0.0 && p.ak[1][1] == 0.0);
-    const bool shape = h->fused && want && two_step_fits(p, g2) && p.nz >= 6;
+    const bool shape = h->fused && want && (W == 128 ? pl.ok : two_step_fits(p, g2)) && p.nz >= 6;
     const bool shape_pml = two_step_fits(p, g2p);
     auto tfsf_plane = [&](int z) { return p.em_source && (z == p.iz_min - 1 || z == p.iz_min); };
     auto upml_plane = [&](int z) { return z <= p.pmlzi || z >= p.pmlzf; };
@@ -1341,7 +1395,6 @@ int build_passes(iris_handle* h, const PassConfig& c) {
       }
       return runs;
     };
-    const int64_t t2 = (int64_t)g2.tx(p.nx) * g2.ty(p.ny);
     auto rounds = [&](const std::vector<int>& runs, int L) {
       int64_t nch = 0;
       for (int r : runs) nch += (r + L - 1) / L;
@@ -1439,20 +1492,16 @@ int build_passes(iris_handle* h, const PassConfig& c) {
       h->ty2 = g2.ty(p.ny);
       h->tx2p = g2p.tx(p.nx);
       h->ty2p = g2p.ty(p.ny);
-      h->t2split = false;
-      if (W == 128 && p.nx >= kP2OX && p.nx % kP2OX != 0) {
-        // whole 124-cell tiles over x in 1..124*tx2, the rest (r columns) by
-        // a strip of 32- or 64-wide tiles: a fifth 128-wide tile column
-        // over 16 columns (nx = 512) would idle most of its lanes
-        const int r = p.nx % kP2OX;
-        const int ws = r <= t2geom<T>(32, false).OX ? 32 : 64;
-        const T2Geom gs = t2geom<T>(ws, false);
-        h->t2split = true;
-        h->tx2 = p.nx / kP2OX;
-        h->w2s = ws;
-        h->tx2s = (r + gs.OX - 1) / gs.OX;
-        h->ty2s = gs.ty(p.ny);
-        h->x0s = kP2OX * h->tx2;
+      h->p2ox = h->p2sl = h->p2ns = h->tx2n = h->ty2n = h->x0n = 0;
+      if (W == 128) {
+        h->tx2 = pl.tx;
+        h->ty2 = pl.ty;
+        h->p2ox = pl.ox;
+        h->p2sl = pl.sl;
+        h->p2ns = pl.ns;
+        h->tx2n = pl.txn;
+        h->ty2n = pl.tyn;
+        h->x0n = pl.x0n;
       }
       dfree(h, (void**)&h->chunks2);
       dfree(h, (void**)&h->chunks2p);
@@ -1803,9 +1852,9 @@ int adjoint_double(iris_handle* h, int n, int nobox, int face_on, const void* re
     const TileMap tm{h->tx2, h->ty2, h->tx2 * h->ty2 * L.n2, h->tx2 * h->xcd_rows2};
     Yee2Samples<T> none{};
     if (h->bloch)
-      launch_yee2<T, true, false>(h, a2, (C*)h->ae2, (C*)h->ah2, nullptr, (const int*)L.chunks2, tm, none);
+      launch_yee2<T, true, false>(h, a2, (C*)h->ae2, (C*)h->ah2, nullptr, (const int*)L.chunks2, tm, L.n2, none);
     else
-      launch_yee2<T, false, false>(h, a2, (C*)h->ae2, (C*)h->ah2, nullptr, (const int*)L.chunks2, tm, none);
+      launch_yee2<T, false, false>(h, a2, (C*)h->ae2, (C*)h->ah2, nullptr, (const int*)L.chunks2, tm, L.n2, none);
   }
   const StepArgs<T> a3 = adj_args<T>(h, h->ae2, h->ah2, h->ae3, h->ah3, h->ad2, h->ab2, h->ad, h->ab);
   adj_sources<T>(h, a3, n - 1, nobox, face_on);
@@ -2317,16 +2366,16 @@ template <typename T>
 int batch_tables2(iris_batch* b) {
   using C = cplx<T>;
   if (b->own.empty()) return 0;
-  for (int bl = 0; bl < 32; ++bl) {
-    const std::vector<iris_handle*>& hs = b->own2[bl];
-    const int n = (int)hs.size();
-    if (!n) continue;
-    const bool strip = bl >= 16;
-    for (int pm = 0; pm < (strip ? 1 : 2); ++pm) {
+  for (int bl = 0; bl < 16; ++bl) {
+    for (int pm = 0; pm < 2; ++pm) {
+      const auto& es = b->y2e[bl][pm];
+      const int n = (int)es.size();
+      if (!n) continue;
       std::vector<Batch2Inst<T>> I((size_t)2 * n);
       for (int pp = 0; pp < 2; ++pp)
         for (int i = 0; i < n; ++i) {
-          iris_handle* h = hs[i];
+          iris_handle* h = es[i].first;
+          const int part = es[i].second;
           Batch2Inst<T>& B = I[(size_t)pp * n + i];
           void *A0 = pp ? h->ke : h->fe, *A1 = pp ? h->kh : h->fh, *C0 = pp ? h->fe : h->ke, *C1 = pp ? h->fh : h->kh;
           void *AD = pp ? h->kd : h->d, *AB = pp ? h->kb : h->b, *CD = pp ? h->d : h->kd, *CB = pp ? h->b : h->kb;
@@ -2336,8 +2385,19 @@ int batch_tables2(iris_batch* b) {
           B.hb = (C*)h->gh;
           B.db = pm ? (C*)h->dn : nullptr;
           B.chunks = pm ? h->chunks2p : h->chunks2;
-          const int tx = strip ? h->tx2s : (pm ? h->tx2p : h->tx2), ty = strip ? h->ty2s : (pm ? h->ty2p : h->ty2);
-          B.tm = TileMap{tx, ty, tx * ty * (pm ? h->n2p : h->n2), tx * h->xcd_rows2, strip ? h->x0s : 0};
+          const int nch = pm ? h->n2p : h->n2;
+          if (part) {
+            B.tm = TileMap{h->tx2n, h->ty2n, h->tx2n * h->ty2n * nch, h->tx2n * h->xcd_rows2, h->x0n};
+            B.ox = h->p2ox;
+            B.sl = h->p2sl;
+            B.ns = h->p2ns;
+          } else {
+            const int tx = pm ? h->tx2p : h->tx2, ty = pm ? h->ty2p : h->ty2;
+            B.tm = TileMap{tx, ty, tx * ty * nch, tx * h->xcd_rows2, 0};
+            B.ox = kP2OX;
+            B.sl = 64;
+            B.ns = 1;
+          }
           B.smp = Yee2Samples<T>{};
           B.smp.nbox = h->p.nbox;
           for (int bx = 0; bx < h->p.nbox; ++bx) {
@@ -2398,23 +2458,27 @@ template <typename T>
 int batch_create2(iris_batch* b) {
   int rc;
   for (iris_handle* h : b->own) {
-    b->own2[(h->bloch ? 1 : 0) + (h->t2w == 32 ? 2 : 0) + (h->t2wp == 32 ? 4 : 0) + (h->t2w == 128 ? 8 : 0)]
-        .push_back(h);
-    if (h->t2w == 128 && h->t2split) b->own2[16 + (h->bloch ? 1 : 0) + (h->w2s == 32 ? 2 : 0)].push_back(h);
+    const int bl = (h->bloch ? 1 : 0) + (h->t2w == 32 ? 2 : 0) + (h->t2wp == 32 ? 4 : 0) + (h->t2w == 128 ? 8 : 0);
+    b->own2[bl].push_back(h);
+    if (h->n2) {
+      if (h->t2w != 128 || h->tx2) b->y2e[bl][0].push_back({h, 0});
+      if (h->t2w == 128 && h->tx2n) b->y2e[bl][0].push_back({h, 1});
+    }
+    if (h->n2p) b->y2e[bl][1].push_back({h, 0});
   }
-  for (int bl = 0; bl < 32; ++bl) {
-    const std::vector<iris_handle*>& hs = b->own2[bl];
-    if (hs.empty()) continue;
-    const bool strip = bl >= 16;
-    for (int pm = 0; pm < (strip ? 1 : 2); ++pm) {
+  for (int bl = 0; bl < 16; ++bl) {
+    for (int pm = 0; pm < 2; ++pm) {
+      const auto& es = b->y2e[bl][pm];
+      if (es.empty()) continue;
       std::vector<int> wg = {0};
-      for (iris_handle* h : hs) {
-        const int64_t nb = strip ? (int64_t)h->tx2s * h->ty2s * h->n2
-                                 : (int64_t)(pm ? h->tx2p : h->tx2) * (pm ? h->ty2p : h->ty2) * (pm ? h->n2p : h->n2);
+      for (const auto& e : es) {
+        iris_handle* h = e.first;
+        const int64_t nb = e.second ? (int64_t)h->tx2n * h->ty2n * h->n2
+                                    : (int64_t)(pm ? h->tx2p : h->tx2) * (pm ? h->ty2p : h->ty2) * (pm ? h->n2p : h->n2);
         wg.push_back(wg.back() + (int)((nb + kNumXcd - 1) / kNumXcd * kNumXcd));
       }
       if ((rc = balloc(b, (void**)&b->y2wg[bl][pm], sizeof(int) * wg.size())) ||
-          (rc = balloc(b, &b->y2[bl][pm], sizeof(Batch2Inst<T>) * 2 * hs.size())))
+          (rc = balloc(b, &b->y2[bl][pm], sizeof(Batch2Inst<T>) * 2 * es.size())))
         return rc;
       BHIP_TRY(b, hipMemcpy(b->y2wg[bl][pm], wg.data(), sizeof(int) * wg.size(), hipMemcpyHostToDevice));
       b->y2nwg[bl][pm] = wg.back();
@@ -2513,13 +2577,13 @@ int batch_run2(iris_batch* b, int it_first, int it_last) {
     const int slot = s1 ? b->own[0]->nsamp : -1;
     int rc;
     rest(0, it);
-    for (int bl = 0; bl < 32; ++bl) {
-      const int n = (int)b->own2[bl].size();
+    for (int bl = 0; bl < 16; ++bl) {
       for (int pm = 0; pm < 2; ++pm) {
         if (!b->y2nwg[bl][pm]) continue;
+        const int n = (int)b->y2e[bl][pm].size();
         const Batch2Inst<T>* I = (const Batch2Inst<T>*)b->y2[bl][pm] + (size_t)pp * n;
         const int* w = b->y2wg[bl][pm];
-        const int W = bl >= 16 ? ((bl & 2) ? 32 : 64) : ((!pm && (bl & 8)) ? 128 : ((bl & (pm ? 4 : 2)) ? 32 : 64));
+        const int W = (!pm && (bl & 8)) ? 128 : ((bl & (pm ? 4 : 2)) ? 32 : 64);
         const unsigned g = (unsigned)b->y2nwg[bl][pm];
         if (bl & 1) {
           if (pm) launch_yee2_batch<T, true, true>(b, W, g, I, w, n, slot);

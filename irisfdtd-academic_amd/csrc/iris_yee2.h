@@ -417,6 +417,7 @@ template <typename T> struct Batch2Inst {
   const int* chunks;
   TileMap tm;
   Yee2Samples<T> smp;  // ring bases (slot 0)
+  int ox = 0, sl = 0, ns = 0;  // tile shape of the two-cells-a-lane kernel (k_yee2p_batch)
 };
 
 template <typename T, bool PH, bool PML, int W>

@@ -15,6 +15,16 @@
 //   - LDS carries only the y-neighbours, and of those only the components a
 //     neighbour reads (comps 3 and 1: the +y/-y terms of the curls), for E1,
 //     H1 and E2 double-buffered by plane parity: 144 KB, one barrier a plane.
+// Narrow tiles.  A tile row of ox output cells takes sl = ox/2 + 2 lanes;
+// a wave holds ns = 64/sl such rows side by side (segments), so the
+// workgroup covers 12 ns footprint rows and outputs ox x (12 ns - 3) cells.
+// The full tile is ox = 124, sl = 64, ns = 1.  Narrow tiles take the
+// columns beside the full tiles when nx is not a multiple of 124 (16 of 512)
+// and whole grids narrower than 124, in the same launch.  The DPP shifts
+// cross from one segment into the next; as at the wave's ends, what they
+// bring is read only by the halo cells at the segment's ends.  Lanes beyond
+// the last segment stand in for the first lane of the tile's last row (in
+// bounds) and write nothing.
 // Pairs never straddle the period boundary when nx is even (cell a is odd),
 // so both cells of a lane are inside, or both are images on the same side
 // with one Bloch factor.  The shift beyond the wave's ends (lane 0's -x,
@@ -56,17 +66,32 @@ __device__ __forceinline__ void pst_nt(float2* f, int64_t i, P2 v) {
 // DPP wave shifts: the value of the next lane (wave_shl:1) and of the
 // previous lane (wave_shr:1); 0 beyond the wave's ends.  Every lane of the
 // wave must be active where they are used.
-__device__ __forceinline__ float from_next(float v) { return __builtin_amdgcn_update_dpp(0.0f, v, 0x130, 0xf, 0xf, false); }
-__device__ __forceinline__ float from_prev(float v) { return __builtin_amdgcn_update_dpp(0.0f, v, 0x138, 0xf, 0xf, false); }
+// bound_ctrl set: an invalid source lane reads 0 with no old value to set up
+// (measured 5-6% faster per launch at 512 x 512 x 1024 than keeping an old
+// value of 0 in a register)
+__device__ __forceinline__ float from_next(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xf, 0xf, true));
+}
+__device__ __forceinline__ float from_prev(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
+}
 // the +x neighbours of a lane's cells: its own cell b, the next lane's cell a
 __device__ __forceinline__ P2 xplus(P2 v) { return P2{v.b, make_float2(from_next(v.a.x), from_next(v.a.y))}; }
 // the -x neighbours: the previous lane's cell b, its own cell a
 __device__ __forceinline__ P2 xminus(P2 v) { return P2{make_float2(from_prev(v.b.x), from_prev(v.b.y)), v.a}; }
 
-constexpr int kP2R = 12;            // waves (footprint rows) per workgroup
-constexpr int kP2W = 128;           // footprint cells per row
-constexpr int kP2OX = kP2W - 4;     // output cells per row (x0 .. x0+123)
-constexpr int kP2OY = kP2R - 3;     // output rows
+constexpr int kP2R = 12;            // waves per workgroup (footprint rows of a full tile)
+constexpr int kP2W = 128;           // footprint cells per row of a full tile
+constexpr int kP2OX = kP2W - 4;     // its output cells per row (x0 .. x0+123)
+constexpr int kP2OY = kP2R - 3;     // its output rows
+
+// One shape of tiles in a launch: the tile map (tm.x0: x of the first
+// output cell of tile column 0, less 1), ox output cells per tile row, sl
+// lanes per row, ns rows per wave.
+struct P2Part {
+  TileMap tm;
+  int ox, sl, ns;
+};
 
 // E update of a lane's two cells outside the UPML (e_update_plain per cell)
 __device__ __forceinline__ void p_e_update(const StepArgs<float>& a, int64_t col, int z, const P2 e[3], const P2 c[3],
@@ -93,12 +118,16 @@ __device__ __forceinline__ void p_e_update(const StepArgs<float>& a, int64_t col
 template <bool PH>
 __device__ __forceinline__ void yee2p_tile(const StepArgs<float>& a, float2* __restrict__ eb, float2* __restrict__ hb,
                                            const int* __restrict__ chunks, int bx, int by, int bz,
-                                           const Yee2Samples<float>& smp, int slot) {
+                                           const Yee2Samples<float>& smp, int slot, int x0, int ox, int sl, int ns) {
   using C = float2;
-  __shared__ P2 s1[2][2][kP2R][64];  // E1 comps 3, 1 of planes z (z & 1) and z-1
-  __shared__ P2 sh[2][2][kP2R][64];  // H1 comps 3, 1
-  __shared__ P2 s2[2][2][kP2R][64];  // E2 comps 3, 1
-  const int cx = threadIdx.x, cy = threadIdx.y;
+  constexpr int NL = kP2R * 64;  // LDS entries per plane and component: rows x lanes of a row
+  __shared__ P2 s1[2][2][NL];    // E1 comps 3, 1 of planes z (z & 1) and z-1
+  __shared__ P2 sh[2][2][NL];    // H1 comps 3, 1
+  __shared__ P2 s2[2][2][NL];    // E2 comps 3, 1
+  const int lane = threadIdx.x, seg = lane / sl;
+  const bool idle = seg >= ns;
+  const int ry = kP2R * ns, oy = ry - 3;  // footprint and output rows of the tile
+  const int cx = idle ? 0 : lane - seg * sl, cy = idle ? ry - 1 : threadIdx.y * ns + seg;
   const int nx = a.nx, ny = a.ny;
   const int64_t cs = a.cs, sxy = a.sxy;
   const int craw = chunks[bz];
@@ -107,13 +136,15 @@ __device__ __forceinline__ void yee2p_tile(const StepArgs<float>& a, float2* __r
   const bool exp_below = (craw & kChunkExportBelow) != 0, exp_above = (craw & kChunkExportAbove) != 0;
 
   // cell a of this lane (odd x) and its periodic source; cell b is the next
-  const int xg = bx * kP2OX - 1 + 2 * cx, yg = by * kP2OY + cy;
+  const int xg = x0 + bx * ox - 1 + 2 * cx, yg = by * oy + cy;
   const int xs = xg < 1 ? xg + nx : (xg > nx ? xg - nx : xg);
   const int ys = yg < 1 ? yg + ny : (yg > ny ? yg - ny : yg);
-  const bool own = cx >= 1 && cx <= kP2OX / 2 && cy >= 1 && cy <= kP2OY && xg <= nx && yg <= ny;
+  const bool own = !idle && cx >= 1 && cx <= ox / 2 && cy >= 1 && cy <= oy && xg <= nx && yg <= ny;
   const int64_t col = (int64_t)(xs + a.xoff) + a.sx * ys;
   const int64_t colym = col - a.sx;  // y-1 (the ghost y = 0 when ys = 1)
-  const int wp = min(cy + 1, kP2R - 1), wm = max(cy - 1, 0);
+  // LDS entries of this lane's row and of the rows above and below (clamped
+  // into the tile: edge rows compute values nobody uses)
+  const int li = cy * sl + cx, lp = min(cy + 1, ry - 1) * sl + cx, lm = max(cy - 1, 0) * sl + cx;
   const C* __restrict__ e0 = a.e;
   const C* __restrict__ h0 = a.h;
   C phi = mk<float>(1.0f, 0.0f);
@@ -162,7 +193,7 @@ __device__ __forceinline__ void yee2p_tile(const StepArgs<float>& a, float2* __r
     {
       const P2 e3xp = xplus(e1p[2]), e2xp = xplus(e1p[1]);
       if (z >= z0) {
-        const P2 e3yp = s1[pm][0][wp][cx], e1yp = s1[pm][1][wp][cx];
+        const P2 e3yp = s1[pm][0][lp], e1yp = s1[pm][1][lp];
         const P2 c0 = padd(psub(psub(e3yp, e1p[2]), e1[1]), e1p[1]);
         const P2 c1 = padd(psub(psub(e1[0], e1p[0]), e3xp), e1p[2]);
         const P2 c2 = padd(psub(psub(e2xp, e1p[1]), e1yp), e1p[0]);
@@ -171,17 +202,19 @@ __device__ __forceinline__ void yee2p_tile(const StepArgs<float>& a, float2* __r
         h1n[2] = psub(hz1[2], pscale(a.dtcq2, c2));
       }
     }
-    s1[pz][0][cy][cx] = e1[2];
-    s1[pz][1][cy][cx] = e1[0];
-    sh[pz][0][cy][cx] = h1n[2];
-    sh[pz][1][cy][cx] = h1n[0];
+    if (!idle) {
+      s1[pz][0][li] = e1[2];
+      s1[pz][1][li] = e1[0];
+      sh[pz][0][li] = h1n[2];
+      sh[pz][1][li] = h1n[0];
+    }
     __syncthreads();
     // ---- E2(z-1) from E1(z-1), H1(z-1) (own, -x by DPP, -y from LDS), H1(z-2)
     P2 e2n[3] = {};
     {
       const P2 h3xm = xminus(h1n[2]), h2xm = xminus(h1n[1]);
       if (z >= z0 + 1) {
-        const P2 h3ym = sh[pz][0][wm][cx], h1ym = sh[pz][1][wm][cx];
+        const P2 h3ym = sh[pz][0][lm], h1ym = sh[pz][1][lm];
         const P2 c[3] = {padd(psub(psub(h1n[2], h3ym), h1n[1]), h1pp[1]),
                          padd(psub(psub(h1n[0], h1pp[0]), h1n[2]), h3xm),
                          padd(psub(psub(h1n[1], h2xm), h1n[0]), h1ym)};
@@ -193,7 +226,7 @@ __device__ __forceinline__ void yee2p_tile(const StepArgs<float>& a, float2* __r
     {
       const P2 e3xp = xplus(e2p[2]), e2xp = xplus(e2p[1]);
       if (z >= z0 + 2) {
-        const P2 e3yp = s2[pm][0][wp][cx], e1yp = s2[pm][1][wp][cx];
+        const P2 e3yp = s2[pm][0][lp], e1yp = s2[pm][1][lp];
         const P2 c0 = padd(psub(psub(e3yp, e2p[2]), e2n[1]), e2p[1]);
         const P2 c1 = padd(psub(psub(e2n[0], e2p[0]), e3xp), e2p[2]);
         const P2 c2 = padd(psub(psub(e2xp, e2p[1]), e1yp), e2p[0]);
@@ -202,8 +235,10 @@ __device__ __forceinline__ void yee2p_tile(const StepArgs<float>& a, float2* __r
         h2n[2] = psub(h1pp[2], pscale(a.dtcq2, c2));
       }
     }
-    s2[pz][0][cy][cx] = e2n[2];
-    s2[pz][1][cy][cx] = e2n[0];
+    if (!idle) {
+      s2[pz][0][li] = e2n[2];
+      s2[pz][1][li] = e2n[0];
+    }
 
     if (own) {
       // step n+2 of the chunk's planes, with H's min-side ghost planes
@@ -284,16 +319,24 @@ __device__ __forceinline__ void yee2p_tile(const StepArgs<float>& a, float2* __r
   }
 }
 
+// Workgroups 0 .. p0.tm.nblk-1 are tiles of p0, the next npad0 - p0.tm.nblk
+// return at once (the count of p0 rounded up to the XCD count, so that the
+// tile order's XCD dealing holds in both parts), the rest tiles of p1.
 template <bool PH>
 __global__ __launch_bounds__(64 * kP2R, 3) void k_yee2p(StepArgs<float> a, float2* __restrict__ eb,
                                                         float2* __restrict__ hb, const int* __restrict__ chunks,
-                                                        TileMap tm, Yee2Samples<float> smp) {
+                                                        P2Part p0, P2Part p1, int npad0, Yee2Samples<float> smp) {
+  int L = blockIdx.x;
+  const bool second = L >= npad0;
+  if (!second && L >= p0.tm.nblk) return;  // padding (no barrier has been reached)
+  if (second) L -= npad0;
+  const P2Part& P = second ? p1 : p0;
   int bx, by, bz;
-  tile_of(tm, blockIdx.x, &bx, &by, &bz);
-  yee2p_tile<PH>(a, eb, hb, chunks, bx, by, bz, smp, 0);
+  tile_of(P.tm, L, &bx, &by, &bz);
+  yee2p_tile<PH>(a, eb, hb, chunks, bx, by, bz, smp, 0, P.tm.x0, P.ox, P.sl, P.ns);
 }
 
-// the batched form (Batch2Inst tables, as k_yee2_batch)
+// the batched form (Batch2Inst tables, as k_yee2_batch; one entry per part)
 template <bool PH>
 __global__ __launch_bounds__(64 * kP2R, 3) void k_yee2p_batch(const Batch2Inst<float>* __restrict__ inst,
                                                               const int* __restrict__ wg_start, int ninst, int slot) {
@@ -304,7 +347,7 @@ __global__ __launch_bounds__(64 * kP2R, 3) void k_yee2p_batch(const Batch2Inst<f
   if (L >= I.tm.nblk) return;  // padding (no barrier has been reached)
   int bx, by, bz;
   tile_of(I.tm, L, &bx, &by, &bz);
-  yee2p_tile<PH>(I.a, I.eb, I.hb, I.chunks, bx, by, bz, I.smp, slot);
+  yee2p_tile<PH>(I.a, I.eb, I.hb, I.chunks, bx, by, bz, I.smp, slot, I.tm.x0, I.ox, I.sl, I.ns);
 }
 
 }  // namespace iris

@@ -115,11 +115,14 @@ def test_batch_is_bitwise_each_run_alone(tmp_path, monkeypatch, precision, upml,
 
 @pytest.mark.gpu
 def test_batch_then_alone(tmp_path):
-    """Handles leave a batch in a state they can go on from alone."""
+    """Handles leave a batch in a state they can go on from alone: the same
+    calls alone give the same results bit for bit.  (Where a Bloch phase
+    meets the two-step pass, as on grid "a", where the calls cut the pairs
+    of steps changes the rounding, so the reference cuts its calls alike.)"""
     print("[gpu-test] batch then alone", flush=True)
     setups = mixed_setups(tmp_path)[:3]
     nt = setups[0].params.nt
-    ref = [alone(s, 4, [(1, nt)]) for s in setups]
+    ref = [alone(s, 4, [(1, 61), (62, nt)]) for s in setups]
     engines = [Engine(s, precision=4) for s in setups]
     try:
         b = Batch(engines)

@@ -386,7 +386,8 @@ def test_bloch_phases_match_oracle(tmp_path, fused):
 
 TWO_STEP_CASES = {
     # narrower than a tile: one 64-lane tile over 40 cells in x (lanes 41..63
-    # stand for cells 1..23), and 9 rows in y under a 16- (12-) row tile
+    # stand for cells 1..23), and 9 rows in y under a 16- (12-) row tile;
+    # with two cells a lane one narrow tile of 40 cells, two rows a wave
     "narrow_x": dict(ixmax=40, iymax=21, izmax=96, pmlzi=10, pmlzf=86, hi=44, dh=10, dx=8, eps_bbox=2.25,
                      tfsf_o=20, box=(1, 40, 2, 20, 15, 80), nof=3, nt=61, no_downloads=3),
     "narrow_y": dict(ixmax=70, iymax=9, izmax=96, pmlzi=10, pmlzf=86, hi=44, dh=10, dx=8, eps_bbox=3.0,
@@ -405,12 +406,13 @@ TWO_STEP_CASES = {
     "lossy_structure": dict(ixmax=64, iymax=16, izmax=128, pmlzi=10, pmlzf=118, hi=50, dh=30, dx=8,
                             eps_bbox=complex(4.0, 0.6), tfsf_o=20, box=(1, 64, 1, 16, 15, 110), nof=2, nt=50,
                             no_downloads=2),
-    # 128-cell tiles of two cells a lane over x in 1..124 and a strip of
-    # 64-cell tiles (two of them) over the other 64 columns
+    # 128-cell tiles of two cells a lane over x in 1..124 and narrow tiles of
+    # 64 output cells over the other 64 columns, in the same launch
     "strip64": dict(ixmax=188, iymax=10, izmax=100, pmlzi=10, pmlzf=90, hi=46, dh=8, dx=30, eps_bbox=3.0,
                     tfsf_o=20, box=(2, 187, 1, 10, 15, 85), nof=2, nt=50, no_downloads=2),
     # region III different from I, box faces inside the top plain region
-    # (with 128-cell tiles a strip of one 32-cell tile over x in 125..130)
+    # (with two cells a lane: narrow tiles of 6 cells, four rows a wave, over
+    # x in 125..130)
     "substrate": dict(ixmax=130, iymax=24, izmax=140, pmlzi=10, pmlzf=130, hi=40, dh=10, dx=20, eps_bbox=4.0,
                       eps3=2.25, tfsf_o=20, box=(10, 120, 3, 22, 60, 110), nof=2, nt=64, no_downloads=2),
 }
