@@ -309,7 +309,15 @@ int iris_adjoint(iris_handle* h, int32_t nobox, const double* weights, double* g
  * the whole batch allows.  Runs that the two-step pass covers take it
  * batched as well: each pair of steps is one launch per kernel over all of
  * them (their single-step chunks, plain and UPML two-step chunks, DFT
- * samples).  Results are bitwise those of stepping each handle alone. ---- */
+ * samples).  The batch lays its handles out for itself while it holds them:
+ * z-chunks as long as a large grid's (the launches span every handle), and
+ * the UPML slabs of two-step runs in the two-step pass (its UPML variant),
+ * which a small grid alone leaves to single steps; iris_batch_destroy puts
+ * each handle's own layout back.  Results are bitwise those of stepping each
+ * handle alone, except on the UPML planes of two-step runs with a Bloch
+ * phase, which then round as the two-step pass does (equal to rounding);
+ * IRIS_BATCH_UPML=0 keeps each handle's own choice there, and every result
+ * bitwise. ---- */
 typedef struct iris_batch iris_batch;
 
 /* Group n handles (created with iris_create, same precision and device, the
@@ -329,6 +337,8 @@ int iris_batch_step(iris_batch* b, int32_t it_first, int32_t it_last);
  * handles that take the (batched) two-step pass. */
 int iris_batch_info(const iris_batch* b, int64_t info[5]);
 
+/* Waits for the batch's work, puts each handle's own layout back (not where
+ * it was rebuilt since, e.g. by iris_autotune) and frees the batch. */
 int iris_batch_destroy(iris_batch* b);
 
 /* Message for the last failure of the batch (or of the last failed

@@ -286,6 +286,9 @@ struct iris_handle {
 struct iris_batch {
   std::vector<iris_handle*> hs;     // all handles of the batch
   std::vector<int64_t> pass_gen;    // their pass_gen when the batch was made (its tables hold those lists)
+  // the handles' own layouts, put back by iris_batch_destroy (the batch lays
+  // them out for itself: see batch_layout)
+  std::vector<PassConfig> saved;
   std::vector<iris_handle*> one;    // those the batched single-step pass advances
   std::vector<iris_handle*> own;    // those that take their own two-step passes
   int prec = 4;
@@ -2683,13 +2686,41 @@ int batch_run2(iris_batch* b, int it_first, int it_last) {
   return 0;
 }
 
+// The batch's layout of a handle.  Alone, a small grid takes short z-chunks
+// so that its launches have enough workgroups, and leaves its UPML slabs to
+// single steps (a launch of the UPML variant over a few dozen workgroups
+// took longer than the single steps).  In a batch the launches span every
+// handle, so the z-chunks are those of a large grid (64-plane two-step
+// chunks, 32-plane single-step ones; the chunk length changes no result) and
+// the UPML slabs go into the two-step pass's UPML variant wherever it applies
+// (bitwise the single steps without a Bloch phase; with one, equal to
+// rounding, as the two-step pass is).  IRIS_BATCH_UPML=0 keeps what each
+// handle alone does there, so that every result is bitwise that of the
+// handle alone.
+PassConfig batch_layout(const iris_handle* h) {
+  PassConfig c = h->pass;
+  c.zc = std::min(32, h->p.nz);
+  c.zc2 = 64;
+  const char* e = std::getenv("IRIS_BATCH_UPML");
+  c.upml = (e && e[0] == '0') ? (h->n2p > 0 ? 1 : 0) : 1;
+  return c;
+}
+
 template <typename T>
 int batch_create_impl(iris_batch* b) {
+  int rc;
+  for (iris_handle* h : b->hs) {
+    b->saved.push_back(h->pass);
+    if ((rc = build_passes<T>(h, batch_layout(h)))) {
+      b->err = h->err;
+      return rc;
+    }
+    b->pass_gen.push_back(h->pass_gen);
+  }
   // handles the two-step pass covers keep it (their own passes, as alone);
   // the batched single-step pass takes the others
   for (iris_handle* h : b->hs) (h->two_step ? b->own : b->one).push_back(h);
   const int n = (int)b->one.size();
-  int rc;
   // z-chunks as long as the whole batch still gives every CU many workgroups
   int64_t tiles = 0;
   int nzmax = 0;
@@ -2821,6 +2852,22 @@ void batch_release(iris_batch* b) {
   b->done = nullptr;
   if (b->stream) (void)hipStreamDestroy(b->stream);
   b->stream = nullptr;
+}
+
+// The handles' own layouts back, after the batch's work has finished (a
+// handle whose layout was rebuilt since the batch took it keeps the new one).
+int batch_restore(iris_batch* b) {
+  int rc = 0;
+  for (size_t i = 0; i < b->saved.size(); ++i) {
+    iris_handle* h = b->hs[i];
+    if (i < b->pass_gen.size() && h->pass_gen != b->pass_gen[i]) continue;
+    const int r = h->prec == 8 ? build_passes<double>(h, b->saved[i]) : build_passes<float>(h, b->saved[i]);
+    if (r && !rc) {
+      rc = r;
+      b->err = h->err;
+    }
+  }
+  return rc;
 }
 
 void release(iris_handle* h) {
@@ -3349,7 +3396,6 @@ int iris_batch_create(iris_handle* const* handles, int32_t n, iris_batch** out) 
   }
   iris_batch* b = new iris_batch();
   b->hs.assign(handles, handles + n);
-  for (int i = 0; i < n; ++i) b->pass_gen.push_back(handles[i]->pass_gen);
   b->prec = handles[0]->prec;
   (void)hipSetDevice(handles[0]->device);
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
@@ -3363,6 +3409,7 @@ int iris_batch_create(iris_handle* const* handles, int32_t n, iris_batch** out) 
   if (rc) {
     g_create_error = b->err;
     batch_release(b);
+    batch_restore(b);
     delete b;
     return rc;
   }
@@ -3410,8 +3457,9 @@ int iris_batch_info(const iris_batch* b, int64_t info[5]) {
 int iris_batch_destroy(iris_batch* b) {
   if (!b) return 0;
   batch_release(b);
+  const int rc = batch_restore(b);
   delete b;
-  return 0;
+  return rc;
 }
 
 const char* iris_batch_last_error(const iris_batch* b) { return b ? b->err.c_str() : g_create_error.c_str(); }

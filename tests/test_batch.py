@@ -5,7 +5,12 @@ The batched pass performs, per workgroup, exactly the work of the single-grid
 pass, so every instance's fields, DFT accumulators and fluxes must equal those
 of stepping that handle alone, bit for bit, whatever mix of grid shapes,
 polarisations, Bloch phases, structures and boxes the batch holds (and
-whether the handle alone would take the two-step pass).
+whether the handle alone would take the two-step pass).  A batch lays its
+handles out with long z-chunks (which change no result) and the UPML slabs in
+the two-step pass; with a Bloch phase the latter rounds differently from
+single steps, so the tests either run the handles alone with the UPML
+variant too (IRIS_TWO_STEP_UPML=1) or keep each handle's own choice in the
+batch (IRIS_BATCH_UPML=0).
 """
 import numpy as np
 import pytest
@@ -83,6 +88,10 @@ def test_batch_is_bitwise_each_run_alone(tmp_path, monkeypatch, precision, upml,
     print(f"[gpu-test] batch vs alone precision={precision} upml={upml} width={width}", flush=True)
     if upml != "auto":
         monkeypatch.setenv("IRIS_TWO_STEP_UPML", upml)
+    else:
+        # the batch keeps each handle's own use of the UPML variant (by
+        # default it takes the slabs wherever the variant applies)
+        monkeypatch.setenv("IRIS_BATCH_UPML", "0")
     if width != "auto":
         w, wp = (width.split("/") + [width])[:2]
         monkeypatch.setenv("IRIS_TILE_W2", w)
@@ -114,12 +123,13 @@ def test_batch_is_bitwise_each_run_alone(tmp_path, monkeypatch, precision, upml,
 
 
 @pytest.mark.gpu
-def test_batch_then_alone(tmp_path):
+def test_batch_then_alone(tmp_path, monkeypatch):
     """Handles leave a batch in a state they can go on from alone: the same
     calls alone give the same results bit for bit.  (Where a Bloch phase
     meets the two-step pass, as on grid "a", where the calls cut the pairs
     of steps changes the rounding, so the reference cuts its calls alike.)"""
     print("[gpu-test] batch then alone", flush=True)
+    monkeypatch.setenv("IRIS_BATCH_UPML", "0")
     setups = mixed_setups(tmp_path)[:3]
     nt = setups[0].params.nt
     ref = [alone(s, 4, [(1, 61), (62, nt)]) for s in setups]
@@ -160,10 +170,12 @@ def test_batch_rejects(tmp_path):
 
 
 @pytest.mark.gpu
-def test_tuned_handles_in_a_batch(tmp_path):
+def test_tuned_handles_in_a_batch(tmp_path, monkeypatch):
     """Handles whose layouts iris_autotune chose (each its own) batch
-    together and still give the fluxes of the default runs alone."""
+    together and still give the fluxes of the default runs alone (the batch
+    keeping each handle's own use of the UPML variant)."""
     print("[gpu-test] tuned handles in a batch", flush=True)
+    monkeypatch.setenv("IRIS_BATCH_UPML", "0")
     setups = mixed_setups(tmp_path)
     nt = setups[0].params.nt
     ref = [alone(s, 4, [(1, nt)]) for s in setups]

@@ -17,7 +17,6 @@ the oblique source against the oracle.
     angles' gradients taken one by one;
   - a batch refuses to step after one of its handles was retuned.
 """
-import ctypes as ct
 
 import numpy as np
 import pytest
@@ -72,20 +71,24 @@ def test_oblique_engine_matches_oracle(tmp_path, case, i_pol):
     o.close()
 
 
-def test_grating_sweep_of_64_batched_is_each_alone():
+def test_grating_sweep_of_64_batched_is_each_alone(monkeypatch):
     """Config-3 sweep: 64 gratings with their Bloch phases and matched oblique
     sources, FP32, 200 steps: the batch's fluxes and fields equal each run's
-    alone bit for bit."""
+    alone bit for bit, the run alone taking its UPML slabs into the two-step
+    pass as the batch does (with a Bloch phase that rounds differently from
+    single steps; the batch's long z-chunks change no result)."""
     _announce("64-grating sweep: batch vs alone")
     setups, drawn = grating_sweep(n=64, seed=3, nz=768, nt=200, nof=50, precision=4)
     assert len({(nx, ny) for nx, ny, _ in drawn}) > 40
     ref = []
+    monkeypatch.setenv("IRIS_TWO_STEP_UPML", "1")
     for s in setups:
         with Engine(s, precision=4) as eng:
             eng.run_to(100)
             eng.flux(1)
             eng.run_to(200)
             ref.append((eng.flux(1), eng.field(0)))
+    monkeypatch.delenv("IRIS_TWO_STEP_UPML")
     engines = [Engine(s, precision=4) for s in setups]
     try:
         with Batch(engines) as b:
@@ -170,7 +173,12 @@ def test_gradient_workload_finite_differences():
 
 
 def _free_device_bytes():
-    hip = ct.CDLL("libamdhip64.so")
+    """Free device memory, read through the HIP runtime the engine's library
+    is linked against (libamdhip64.so.7, already loaded with it).  torch
+    ships a HIP runtime of its own, and a second runtime started in a
+    process where the engine's has started sees no device."""
+    import ctypes as ct
+    hip = ct.CDLL("libamdhip64.so.7")
     free, total = ct.c_size_t(), ct.c_size_t()
     assert hip.hipMemGetInfo(ct.byref(free), ct.byref(total)) == 0
     return free.value
