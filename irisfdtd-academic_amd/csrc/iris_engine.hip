@@ -126,10 +126,11 @@ struct PassConfig {
   int upml = -1;      // its UPML variant: 0 off, 1 wherever it applies, -1 where its launch fills the CUs
   int zc2 = 0;        // planes per two-step chunk (0: 2*zc)
   int xcd_rows = 0;   // tile rows per XCD run of the single-step pass (0: identity order)
-  int xcd_rows2 = 4;  // the same for the two-step pass
+  int xcd_rows2 = -1; // the same for the two-step pass (-1: 2 with two cells a lane, else 4)
   int w2 = 0;         // two-step tile width: 64, 32 or 128 (two cells a lane, FP32 fields); 0: by the grid
   int w2p = 0;        // the same for its UPML variant (0: 32)
   int nt2 = -1;       // two-step pass streams its outputs past the L2: 0 no, 1 yes, -1 up to 160 x 160 cells a plane
+  int p2ox = 0;       // two cells a lane: output cells a row of the main tiles (124, 60, 28; 0: by the grid)
 };
 
 struct iris_handle {
@@ -190,10 +191,11 @@ struct iris_handle {
   int t2w = 64;             // two-step tile width (footprint cells per row)
   int t2wp = 64;            // the same for its UPML variant
   int tx2p = 0;             // tile columns of the UPML variant
-  // two cells a lane (t2w = 128): full tiles (tx2 x ty2, 124 cells a row)
-  // over x in 1..124*tx2 and narrow tiles (tx2n x ty2n, p2ox cells a row in
-  // rows of p2sl lanes, p2ns rows a wave) over the columns from x0n + 1
-  int p2ox = 0, p2sl = 0, p2ns = 0, tx2n = 0, ty2n = 0, x0n = 0;
+  // two cells a lane (t2w = 128): main tiles (tx2 x ty2, p2ox cells a row
+  // in rows of p2sl lanes, p2ns rows a wave) over x in 1..p2ox*tx2 and
+  // narrow tiles (tx2n x ty2n, the same for p2oxn, p2sln, p2nsn) over the
+  // columns from x0n + 1
+  int p2ox = 0, p2sl = 0, p2ns = 0, p2oxn = 0, p2sln = 0, p2nsn = 0, tx2n = 0, ty2n = 0, x0n = 0;
   int nt2 = 0;              // two-step pass: streaming stores of its outputs
   PassConfig pass;          // the layout the lists were built for
   int64_t pass_gen = 0;     // bumped by every rebuild of the pass lists
@@ -529,9 +531,10 @@ void launch_yee2(const iris_handle* h, const StepArgs<T>& a, cplx<T>* eb, cplx<T
                  const int* chunks, const TileMap& tm, int nch, const Yee2Samples<T>& smp) {
   if constexpr (sizeof(T) == 4 && !PML) {
     if (h->t2w == 128) {
-      const P2Part p0{TileMap{h->tx2, h->ty2, h->tx2 * h->ty2 * nch, h->tx2 * h->xcd_rows2, 0}, kP2OX, 64, 1};
-      const P2Part p1{TileMap{h->tx2n, h->ty2n, h->tx2n * h->ty2n * nch, h->tx2n * h->xcd_rows2, h->x0n}, h->p2ox,
-                      h->p2sl, h->p2ns};
+      const P2Part p0{TileMap{h->tx2, h->ty2, h->tx2 * h->ty2 * nch, h->tx2 * h->xcd_rows2, 0}, h->p2ox, h->p2sl,
+                      h->p2ns};
+      const P2Part p1{TileMap{h->tx2n, h->ty2n, h->tx2n * h->ty2n * nch, h->tx2n * h->xcd_rows2, h->x0n}, h->p2oxn,
+                      h->p2sln, h->p2nsn};
       const int npad0 = p1.tm.nblk ? (p0.tm.nblk + kNumXcd - 1) / kNumXcd * kNumXcd : p0.tm.nblk;
       const unsigned nblk = (unsigned)(npad0 + p1.tm.nblk);
       if (nblk)
@@ -1105,6 +1108,7 @@ PassConfig pass_config_env(const iris_params& p) {
   num("IRIS_XCD_ROWS2", &c.xcd_rows2, 0);
   num("IRIS_TILE_W2", &c.w2, 0);
   num("IRIS_TILE_W2P", &c.w2p, 0);
+  num("IRIS_P2_OX", &c.p2ox, 0);
   {
     const char* e = std::getenv("IRIS_NT_STORE2");
     if (e && (e[0] == '0' || e[0] == '1')) c.nt2 = e[0] - '0';
@@ -1160,58 +1164,77 @@ T2Geom t2geom(int W, bool pml) {
   g.OY = g.RY - 3;
   return g;
 }
-// Tiles of the two-cells-a-lane kernel (k_yee2p) over a grid of even nx:
-// full tiles (124 output cells a row, 9 rows) over x in 1 .. 124 tx, and
-// narrow tiles over the r = nx - 124 tx columns beside them (all of a grid
-// narrower than 124): tiles of ox output cells (even) in rows of
-// sl = ox/2 + 2 lanes, ns rows a wave, 12 ns - 3 output rows; the shape with
-// the fewest workgroups a chunk among those whose footprint stays within one
-// period of the grid.
+// Tiles of the two-cells-a-lane kernel (k_yee2p) over a grid of even nx.
+// A tile of ox output cells a row takes rows of sl = ox/2 + 2 lanes, ns rows
+// a wave (12 ns footprint rows, 12 ns - 3 output rows).  Main tiles of one
+// shape cover x in 1 .. ox tx; narrow tiles (another shape) the r = nx - ox tx
+// columns beside them, or all of a grid narrower than a main tile.  The
+// main shapes tried fill a wave: ox = 124 (one row a wave), 60 (two), 28
+// (four); 60 and 28 output 1260 cells a workgroup against 1116 (fewer halo
+// rows: 21 of 24 against 9 of 12).  The layout kept has the fewest
+// workgroups a chunk among those whose footprint stays within one period of
+// the grid (ties: the wider main tile).
 struct P2Layout {
-  int tx = 0, ty = 0;
-  int ox = 0, sl = 0, ns = 0, txn = 0, tyn = 0, x0n = 0;
+  int ox = 0, sl = 0, ns = 0, tx = 0, ty = 0;             // main tiles
+  int oxn = 0, sln = 0, nsn = 0, txn = 0, tyn = 0, x0n = 0;  // narrow tiles
   bool ok = false;
   int64_t tiles() const { return (int64_t)tx * ty + (int64_t)txn * tyn; }
 };
-P2Layout p2_layout(int nx, int ny) {
-  P2Layout L;
-  if (nx < 2 || nx % 2 || ny < 1) return L;
-  auto fits_y = [&](int oy, int ry, int ty) { return (int64_t)(ty - 1) * oy + ry - 1 <= 2LL * ny; };
-  L.tx = nx / kP2OX;
-  if (L.tx) {
-    L.ty = (ny + kP2OY - 1) / kP2OY;
-    if (!fits_y(kP2OY, kP2R, L.ty)) return L;
+// rows of a tile shape that fit the grid's period in y: fewest tile rows
+int p2_rows(int ny, int sl, int* ns_out, int* ty_out) {
+  for (int ns = 64 / sl; ns >= 1; --ns) {
+    const int ry = kP2R * ns, oy = ry - 3;
+    const int ty = (ny + oy - 1) / oy;
+    if ((int64_t)(ty - 1) * oy + ry - 1 <= 2LL * ny) {
+      *ns_out = ns;
+      *ty_out = ty;
+      return 1;
+    }
   }
-  const int r = nx - kP2OX * L.tx;
-  if (r) {
-    int64_t best = -1;
-    for (int m = 1; m <= 8; ++m) {
-      const int ox = ((r + m - 1) / m + 1) / 2 * 2;
-      const int sl = ox / 2 + 2;
-      if (sl > 64) continue;
-      const int txn = (r + ox - 1) / ox;
-      if ((int64_t)kP2OX * L.tx + (int64_t)txn * ox + 2 > 2LL * nx) continue;
-      for (int ns = 64 / sl; ns >= 1; --ns) {
-        const int ry = kP2R * ns, oy = ry - 3;
-        const int tyn = (ny + oy - 1) / oy;
-        if (!fits_y(oy, ry, tyn)) continue;
-        const int64_t wg = (int64_t)txn * tyn;
-        if (best < 0 || wg < best) {
-          best = wg;
-          L.ox = ox;
-          L.sl = sl;
-          L.ns = ns;
+  return 0;
+}
+P2Layout p2_layout(int nx, int ny, int only_ox = 0) {
+  P2Layout best;
+  if (nx < 2 || nx % 2 || ny < 1) return best;
+  for (int oxm : {kP2OX, 60, 28}) {
+    if (only_ox && oxm != only_ox) continue;
+    P2Layout L;
+    L.tx = nx / oxm;
+    if (L.tx) {
+      L.ox = oxm;
+      L.sl = oxm / 2 + 2;
+      int ns, ty;
+      if (!p2_rows(ny, L.sl, &ns, &ty) || ns != 64 / L.sl) continue;  // main tiles fill their waves
+      L.ns = ns;
+      L.ty = ty;
+    }
+    const int r = nx - oxm * L.tx;
+    if (r) {
+      int64_t bn = -1;
+      for (int m = 1; m <= 8; ++m) {
+        const int ox = ((r + m - 1) / m + 1) / 2 * 2;
+        const int sl = ox / 2 + 2;
+        if (sl > 64) continue;
+        const int txn = (r + ox - 1) / ox;
+        if ((int64_t)oxm * L.tx + (int64_t)txn * ox + 2 > 2LL * nx) continue;
+        int ns, tyn;
+        if (!p2_rows(ny, sl, &ns, &tyn)) continue;
+        if (bn < 0 || (int64_t)txn * tyn < bn) {
+          bn = (int64_t)txn * tyn;
+          L.oxn = ox;
+          L.sln = sl;
+          L.nsn = ns;
           L.txn = txn;
           L.tyn = tyn;
         }
-        break;
       }
+      if (bn < 0) continue;
+      L.x0n = oxm * L.tx;
     }
-    if (best < 0) return L;
-    L.x0n = kP2OX * L.tx;
+    L.ok = true;
+    if (!best.ok || L.tiles() < best.tiles()) best = L;
   }
-  L.ok = true;
-  return L;
+  return best;
 }
 
 bool two_step_fits(const iris_params& p, const T2Geom& g) {
@@ -1230,7 +1253,7 @@ int build_passes(iris_handle* h, const PassConfig& c) {
   ++h->pass_gen;     // a batch made from the old lists refuses to step
   h->pass = c;
   h->xcd_rows = c.xcd_rows;
-  h->xcd_rows2 = c.xcd_rows2;
+  h->xcd_rows2 = c.xcd_rows2 >= 0 ? c.xcd_rows2 : 4;  // resolved below once the tile width is known
   // Streaming stores of the two-step outputs: measured 6-10% faster at
   // 128 x 128 x 768 and 3-5% slower at 512 x 512 x 1024 (FP32, same box)
   h->nt2 = c.nt2 >= 0 ? c.nt2 : (p.nx <= 160 && p.ny <= 160 ? 1 : 0);
@@ -1301,18 +1324,25 @@ int build_passes(iris_handle* h, const PassConfig& c) {
     // 128 x 128 x 768; on the 64-grating sweep, batched, 3% slower.
     // 128: two x-cells per lane (k_yee2p), with FP32 fields on grids of even
     // nx only (a lane's pair of cells must not straddle the x period), and
-    // the default there: 124 cells a tile row, the columns beside (and grids
-    // narrower than that) in narrow tiles of the same launch (p2_layout).
-    // With 124-cell tiles alone, measured at ny = 512, nz = 1024 against
-    // 64-wide tiles: 21% faster at nx = 64, 13% at 100, 5% at 128, 13% at
-    // 250 and 256, 8% at 376, 22-28% at 496, 17% at 512, 15% at 1024.
-    const P2Layout pl = p2_layout(p.nx, p.ny);
+    // the default there: main tiles of 124, 60 or 28 cells a row, the columns
+    // beside (and grids narrower than those) in narrow tiles of the same
+    // launch (p2_layout).  With 124-cell tiles alone, measured at ny = 512,
+    // nz = 1024 against 64-wide tiles: 21% faster at nx = 64, 13% at 100, 5%
+    // at 128, 13% at 250 and 256, 8% at 376, 22-28% at 496, 17% at 512, 15%
+    // at 1024.  At 512 x 512 x 1024 the 60-cell main tiles (216 workgroups a
+    // chunk against 236) measured within noise of the 124-cell ones, 28-cell
+    // ones 15% slower.
+    P2Layout pl = p2_layout(p.nx, p.ny, c.p2ox);
+    if (!pl.ok && c.p2ox) pl = p2_layout(p.nx, p.ny);
     const bool pair_ok = sizeof(T) == 4 && pl.ok;
     const bool pair_default = pair_ok;
     int W = (c.w2 == 32 || c.w2 == 64 || (c.w2 == 128 && pair_ok)) ? c.w2 : (c.w2 == 0 && pair_default ? 128 : 64);
     if (W != 128 && !two_step_fits(p, t2geom<T>(W, false)) && two_step_fits(p, t2geom<T>(96 - W, false)))
       W = 96 - W;
     h->t2w = W;
+    // tile rows per XCD run: 2 with two cells a lane (measured 3% faster
+    // than 4 at 512 x 512 x 1024 over four interleaved runs each)
+    if (c.xcd_rows2 < 0) h->xcd_rows2 = W == 128 ? 2 : 4;
     // the UPML variant's own width, 32 by default: its slabs' launch is a
     // few rounds of workgroups, and with FP32 fields 32-wide tiles of 12
     // waves cover a 512 x 512 plane in 450 workgroups against 513 (two slabs:
@@ -1495,13 +1525,16 @@ int build_passes(iris_handle* h, const PassConfig& c) {
       h->ty2 = g2.ty(p.ny);
       h->tx2p = g2p.tx(p.nx);
       h->ty2p = g2p.ty(p.ny);
-      h->p2ox = h->p2sl = h->p2ns = h->tx2n = h->ty2n = h->x0n = 0;
+      h->p2ox = h->p2sl = h->p2ns = h->p2oxn = h->p2sln = h->p2nsn = h->tx2n = h->ty2n = h->x0n = 0;
       if (W == 128) {
         h->tx2 = pl.tx;
         h->ty2 = pl.ty;
         h->p2ox = pl.ox;
         h->p2sl = pl.sl;
         h->p2ns = pl.ns;
+        h->p2oxn = pl.oxn;
+        h->p2sln = pl.sln;
+        h->p2nsn = pl.nsn;
         h->tx2n = pl.txn;
         h->ty2n = pl.tyn;
         h->x0n = pl.x0n;
@@ -2260,10 +2293,11 @@ int autotune_impl(iris_handle* h, int steps, int64_t info[9]) {
       c.zc2 = std::min(v, kChunkLenMask + 1);
       if (c.zc2 >= 2 && c.zc2 != zc2) consider(c);
     }
-    for (int v : {0, 2, 8}) {
+    const int xr2 = h->xcd_rows2;
+    for (int v : {0, 2, 4, 8}) {
       PassConfig c = best;
       c.xcd_rows2 = v;
-      if (v != best.xcd_rows2) consider(c);
+      if (v != xr2) consider(c);
     }
   }
   const int rc_run = rc;
@@ -2391,15 +2425,15 @@ int batch_tables2(iris_batch* b) {
           const int nch = pm ? h->n2p : h->n2;
           if (part) {
             B.tm = TileMap{h->tx2n, h->ty2n, h->tx2n * h->ty2n * nch, h->tx2n * h->xcd_rows2, h->x0n};
-            B.ox = h->p2ox;
-            B.sl = h->p2sl;
-            B.ns = h->p2ns;
+            B.ox = h->p2oxn;
+            B.sl = h->p2sln;
+            B.ns = h->p2nsn;
           } else {
             const int tx = pm ? h->tx2p : h->tx2, ty = pm ? h->ty2p : h->ty2;
             B.tm = TileMap{tx, ty, tx * ty * nch, tx * h->xcd_rows2, 0};
-            B.ox = kP2OX;
-            B.sl = 64;
-            B.ns = 1;
+            B.ox = h->p2ox;
+            B.sl = h->p2sl;
+            B.ns = h->p2ns;
           }
           B.smp = Yee2Samples<T>{};
           B.smp.nbox = h->p.nbox;

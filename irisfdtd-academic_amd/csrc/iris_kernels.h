@@ -419,7 +419,10 @@ __global__ __launch_bounds__(256) void k_diag_final(const double* __restrict__ p
 
 // Face fluxes, reference code/dataprocess.f90:241-271: block = (pair, iw,
 // half); writes the unscaled sum with the reference's sign into
-// out[iw*6 + face].
+// out[iw*6 + face].  Each lane sums a strided share of the face's points;
+// the 64 lanes of a wave reduce by shuffles (fixed butterfly order), the
+// four wave sums in order through LDS, so the result does not depend on
+// scheduling.
 __global__ __launch_bounds__(256) void k_flux(BoxGeom g, const double2* emg, int nof, int active_mask,
                                               double* out) {
   const int j = blockIdx.x / 2, half = blockIdx.x % 2, iw = blockIdx.y;
@@ -439,14 +442,15 @@ __global__ __launch_bounds__(256) void k_flux(BoxGeom g, const double2* emg, int
     const double v2 = Cc.x * D.x - Cc.y * (-D.y);
     s += v1 - v2;
   }
-  __shared__ double red[256];
-  red[threadIdx.x] = s;
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, 64);
+  __shared__ double red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
   __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
+  if (threadIdx.x == 0) {
+    const double t = (red[0] + red[1]) + (red[2] + red[3]);
+    out[iw * 6 + 2 * j + half] = half ? t : -t;
   }
-  if (threadIdx.x == 0) out[iw * 6 + 2 * j + half] = half ? red[0] : -red[0];
 }
 
 }  // namespace iris
