@@ -865,7 +865,9 @@ int graph_period_run(iris_handle* h, int it, const PassLists& L, bool* shifted) 
     return period_steps<T>(h, it, L);
   }
   // capture; the host state advances as the steps are recorded, so it is
-  // restored if the capture fails (nothing has run)
+  // restored if the capture fails (nothing has run).  A replay of the old
+  // graph may still be running: wait for it before its exec is destroyed.
+  if (h->gexec) HIP_TRY(h, hipStreamSynchronize(h->stream));
   graph_release(h);
   if (*shifted) {
     hipLaunchKernelGGL(k_set_shift, dim3(1), dim3(64), 0, h->stream, h->tfc, 0);
@@ -2665,6 +2667,9 @@ int batch_run2(iris_batch* b, int it_first, int it_last) {
       // capture (the host state advances as the pairs are recorded; it is
       // restored if the capture fails, since nothing has run)
       if (shifted && (rc = set_shift(0))) return rc;
+      // a replay of the old graph may still be running
+      if (b->gexec && hipStreamSynchronize(b->stream) != hipSuccess)
+        return bfail(b, IRIS_EDEVICE, "hipStreamSynchronize failed");
       if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
       if (b->graph) (void)hipGraphDestroy(b->graph);
       b->gexec = nullptr;
