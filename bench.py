@@ -43,6 +43,7 @@ fastest; --no-autotune keeps the default layout.
   python bench.py --angles-only [--angle-steps N] [--angles A]
   python bench.py --batch-only [--batch-steps N]
   python bench.py --pmc-probe [--fp64] [--pmc-steps N]     workload of the counter runs
+  python bench.py --bw-probe                               streaming bandwidth of plain kernels
   python bench.py --report-stats CSV [--command CMD] [--steps-fp32 N] [--steps-fp64 N]
   python bench.py --report-pmc FETCH_DIR WRITE_DIR --pmc-steps N --out-dir DIR
                   [--command-fetch CMD --command-write CMD]
@@ -623,6 +624,39 @@ CLASSES = [
 ]
 
 
+def bw_probe(gib=3, reps=5):
+    """Device-memory bandwidth a plain streaming kernel reaches on this part:
+    torch's elementwise kernels on `gib`-GiB buffers (no engine in this
+    process), best of `reps` of each: read only (a sum), copy (one read, one
+    write stream), and six read and six write streams at once (as the Yee
+    passes have).  The ceiling the Yee kernels are held against in practice;
+    the roofline fraction is taken against the 8 TB/s peak regardless."""
+    import torch
+    n = (gib << 30) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    six_a = [t for t in a.view(6, -1)]
+    six_b = [t for t in b.view(6, -1)]
+    def timed(fn, moved):
+        best = None
+        for _ in range(reps + 1):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1)
+            best = ms if best is None or ms < best else best
+        return round(moved / (best * 1e-3) / 1e9, 1)
+    out = {"buffer_bytes": n * 4,
+           "read_gbs": timed(lambda: a.sum(), n * 4),
+           "copy_gbs": timed(lambda: b.copy_(a), 2 * n * 4),
+           "six_streams_gbs": timed(lambda: [y.copy_(x) for x, y in zip(six_a, six_b)], 2 * n * 4),
+           "note": "best of %d; six streams: six copies of a sixth each, one after another" % reps}
+    del a, b
+    return out
+
+
 def pmc_probe(fp64, steps):
     """Workload of the counter runs: the calibration copy (COPY_BYTES device
     to device with hipMemcpy, which runs the runtime's copy kernel, three
@@ -659,9 +693,9 @@ def kernel_class(name):
 def precision_of(name):
     """Field precision from the kernel's template arguments (None for the
     kernels that have none, such as the phase table; k_yee2p holds FP32
-    fields only)."""
+    fields only, and k_dft_fold_mfma folds FP32 samples)."""
     tmpl = name.split("(")[0]
-    if "<float" in tmpl or "k_yee2p<" in tmpl:
+    if "<float" in tmpl or "k_yee2p<" in tmpl or "k_dft_fold_mfma<" in tmpl:
         return 4
     if "<double" in tmpl:
         return 8
@@ -790,6 +824,7 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=BATCH_STEPS)
     ap.add_argument("--cpu-only", action="store_true")
     ap.add_argument("--pmc-probe", action="store_true")
+    ap.add_argument("--bw-probe", action="store_true", help="device-memory bandwidth of torch's streaming kernels")
     ap.add_argument("--fp64", action="store_true")
     ap.add_argument("--pmc-steps", type=int, default=40)
     ap.add_argument("--report-stats", metavar="CSV")
@@ -814,6 +849,9 @@ def main():
         sys.exit(2)
     if a.pmc_probe:
         pmc_probe(a.fp64, a.pmc_steps)
+        return
+    if a.bw_probe:
+        print(json.dumps(bw_probe()), flush=True)
         return
     if a.cpu_only:
         print(json.dumps(cpu_baseline()), flush=True)
