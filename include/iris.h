@@ -228,6 +228,16 @@ int iris_profile_read(iris_handle* h, double* ms, int64_t* launches, int32_t res
  * reference; used to account the algorithmic bytes of the passes. */
 int iris_pass_info(const iris_handle* h, int64_t info[6]);
 
+/* Tiles of the two-step pass with two x-cells a lane (FP32 fields, even nx)
+ * over an nx x ny plane, as a handle would take them; host arithmetic only,
+ * no device.  only_ox: 124, 60 or 28 for that main shape alone, 0 for the
+ * default choice.  out[0..4] = main tiles: output cells a row, lanes a row,
+ * rows a wave, tile columns, tile rows; out[5..9] = the same for the narrow
+ * tiles beside them; out[10] = x of the narrow tiles' first column, less 1.
+ * IRIS_EINVAL where no layout fits (odd nx, or a grid too small).  No
+ * counterpart in the reference. */
+int iris_pair_layout(int32_t nx, int32_t ny, int32_t only_ox, int32_t out[11]);
+
 /* Layout tuner (no counterpart in the reference).  Before the first step,
  * times `steps` steps (2..1000) of candidate layouts of the step passes --
  * chunk lengths, the two-step pass and its UPML variant on or off, the

@@ -3365,6 +3365,15 @@ int iris_pass_info(const iris_handle* h, int64_t info[6]) {
   return 0;
 }
 
+int iris_pair_layout(int32_t nx, int32_t ny, int32_t only_ox, int32_t out[11]) {
+  if (!out) return fail(nullptr, IRIS_EINVAL, "iris_pair_layout: out is NULL");
+  const P2Layout L = p2_layout(nx, ny, only_ox);
+  if (!L.ok) return fail(nullptr, IRIS_EINVAL, "iris_pair_layout: no layout of two cells a lane fits %d x %d", nx, ny);
+  const int32_t v[11] = {L.ox, L.sl, L.ns, L.tx, L.ty, L.oxn, L.sln, L.nsn, L.txn, L.tyn, L.x0n};
+  std::copy(v, v + 11, out);
+  return 0;
+}
+
 int iris_graph_info(const iris_handle* h, int64_t info[3]) {
   if (!h || !info) return fail(nullptr, IRIS_EINVAL, "handle or info is NULL");
   info[0] = h->gexec ? 1 : 0;

@@ -130,6 +130,8 @@ def load_library(path=None):
     lib.iris_version.argtypes = []
     lib.iris_version.restype = ct.c_char_p
     lib.iris_incident_table.argtypes = [P, ct.c_void_p]
+    lib.iris_pair_layout.argtypes = [ct.c_int32, ct.c_int32, ct.c_int32, ct.c_void_p]
+    lib.iris_pair_layout.restype = ct.c_int
     lib.iris_incident_table.restype = ct.c_int
     _lib = lib
     return lib
@@ -141,10 +143,22 @@ EXPORTED = ["iris_create", "iris_step", "iris_sync", "iris_flux", "iris_dft_poin
             "iris_pass_info", "iris_autotune", "iris_graph_info", "iris_checkpoint", "iris_checkpoint_info",
             "iris_adjoint",
             "iris_batch_create", "iris_batch_step", "iris_batch_info", "iris_batch_destroy", "iris_batch_last_error",
-            "iris_incident_table", "iris_get_plane_noalloc"]
+            "iris_incident_table", "iris_get_plane_noalloc", "iris_pair_layout"]
 
 KERNELS = ["epass", "hpass", "tfsf", "dft_sample", "dft_accum", "flux", "yee", "yee2", "yee_rest", "adjoint",
            "checkpoint", "yee2_pml"]
+
+
+def pair_layout(nx, ny, only_ox=0):
+    """Tiles of the two-cells-a-lane two-step pass over an nx x ny plane
+    (iris_pair_layout; host arithmetic, no device): dict of the main and
+    narrow tile shapes, or None where none fits."""
+    lib = load_library()
+    out = (ct.c_int32 * 11)()
+    if lib.iris_pair_layout(nx, ny, only_ox, out) != 0:
+        return None
+    keys = ["ox", "sl", "ns", "tx", "ty", "oxn", "sln", "nsn", "txn", "tyn", "x0n"]
+    return dict(zip(keys, list(out)))
 
 
 def incident_table(params):
