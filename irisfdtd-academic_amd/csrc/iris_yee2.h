@@ -55,29 +55,56 @@ template <typename T, bool PML = false, int W = 64> struct Yee2Shape {
   static constexpr int WAVES_PER_EU = (R * WG_PER_CU + 3) / 4;
 };
 
+// Face membership of cell (x, y) in box g, 5 bits: on the bxi / bxf x face
+// (1, 2) or the byi / byf y face (4, 8) -- each within the box's range in the
+// other lateral coordinate -- and inside the box's x and y ranges (16: its z
+// faces).  Computed once per lane, so that the z march tests a plane with a
+// few instructions: edge lanes alone carry lateral bits, and the z faces are
+// two planes of the march.
+__device__ inline unsigned face_bits(const BoxGeom& g, int x, int y) {
+  const bool inx = x >= g.bxi && x <= g.bxf, iny = y >= g.byi && y <= g.byf;
+  unsigned f = 0;
+  if (g.npts[0] && iny) f |= (x == g.bxi ? 1u : 0u) | (x == g.bxf ? 2u : 0u);
+  if (g.npts[1] && inx) f |= (y == g.byi ? 4u : 0u) | (y == g.byf ? 8u : 0u);
+  if (g.npts[2] && inx && iny) f |= 16u;
+  return f;
+}
+// the bits of both boxes of a sampling set (5 a box)
+template <typename T> struct Yee2Samples;
+template <typename T>
+__device__ inline unsigned face_bits2(const Yee2Samples<T>& smp, int x, int y) {
+  unsigned f = 0;
+  for (int b = 0; b < 2; ++b)
+    if (b < smp.nbox) f |= face_bits(smp.g[b], x, y) << (5 * b);
+  return f;
+}
+
 // Index in the concatenated face-point list of box g (inverse of box_point,
 // reference code/dataprocess.f90:813-901): store E (base 0) or H (base 3) of
-// cell (x,y,z) into every face point the cell is.
+// cell (x,y,z) into every face point the cell is; f = the cell's face bits
+// for g.
 template <typename C>
-__device__ inline void sample_cell(const BoxGeom& g, C* __restrict__ samp, int x, int y, int z, int base,
+__device__ inline void sample_cell(const BoxGeom& g, C* __restrict__ samp, unsigned f, int x, int y, int z, int base,
                                    const C v[3]) {
-  if (g.npts[0] && y >= g.byi && y <= g.byf && z >= g.bzi && z <= g.bzf) {
-    const int64_t q = g.start[0] + 6 * ((int64_t)(y - g.byi) + (int64_t)g.nyb * (z - g.bzi)) + base;
-    if (x == g.bxi) { samp[q] = v[0]; samp[q + 1] = v[1]; samp[q + 2] = v[2]; }
-    if (x == g.bxf) {
-      const int64_t r = q + g.npts[0] / 2;
-      samp[r] = v[0]; samp[r + 1] = v[1]; samp[r + 2] = v[2];
+  if ((f & 15u) && z >= g.bzi && z <= g.bzf) {
+    if (f & 3u) {
+      const int64_t q = g.start[0] + 6 * ((int64_t)(y - g.byi) + (int64_t)g.nyb * (z - g.bzi)) + base;
+      if (f & 1u) { samp[q] = v[0]; samp[q + 1] = v[1]; samp[q + 2] = v[2]; }
+      if (f & 2u) {
+        const int64_t r = q + g.npts[0] / 2;
+        samp[r] = v[0]; samp[r + 1] = v[1]; samp[r + 2] = v[2];
+      }
+    }
+    if (f & 12u) {
+      const int64_t q = g.start[1] + 6 * ((int64_t)(x - g.bxi) + (int64_t)g.nxb * (z - g.bzi)) + base;
+      if (f & 4u) { samp[q] = v[0]; samp[q + 1] = v[1]; samp[q + 2] = v[2]; }
+      if (f & 8u) {
+        const int64_t r = q + g.npts[1] / 2;
+        samp[r] = v[0]; samp[r + 1] = v[1]; samp[r + 2] = v[2];
+      }
     }
   }
-  if (g.npts[1] && x >= g.bxi && x <= g.bxf && z >= g.bzi && z <= g.bzf) {
-    const int64_t q = g.start[1] + 6 * ((int64_t)(x - g.bxi) + (int64_t)g.nxb * (z - g.bzi)) + base;
-    if (y == g.byi) { samp[q] = v[0]; samp[q + 1] = v[1]; samp[q + 2] = v[2]; }
-    if (y == g.byf) {
-      const int64_t r = q + g.npts[1] / 2;
-      samp[r] = v[0]; samp[r + 1] = v[1]; samp[r + 2] = v[2];
-    }
-  }
-  if (g.npts[2] && x >= g.bxi && x <= g.bxf && y >= g.byi && y <= g.byf) {
+  if ((f & 16u) && (z == g.bzi || z == g.bzf)) {
     const int64_t q = g.start[2] + 6 * ((int64_t)(x - g.bxi) + (int64_t)g.nxb * (y - g.byi)) + base;
     if (z == g.bzi) { samp[q] = v[0]; samp[q + 1] = v[1]; samp[q + 2] = v[2]; }
     if (z == g.bzf) {
@@ -229,6 +256,7 @@ __device__ __forceinline__ void yee2_tile(const StepArgs<T>& a, cplx<T>* __restr
   };
 
   auto upml = [&](int z) { return PML && (z <= a.pmlzi || z >= a.pmlzf); };
+  const unsigned fbits = face_bits2(smp, xs, ys);  // this lane's faces of the DFT boxes
   auto auxi = [&](int z, int k) -> int64_t {  // aux index of component k of this lane's cell on UPML plane z
     const int top = z >= a.pmlzf;
     return a.aux(top, k, col + sxy * (top ? (z - a.pmlzf) : (z - 1)));
@@ -375,8 +403,9 @@ __device__ __forceinline__ void yee2_tile(const StepArgs<T>& a, cplx<T>* __restr
       for (int b = 0; slot >= 0 && b < smp.nbox; ++b) {
         if (!smp.samp[b]) continue;
         C* const sp = smp.samp[b] + (int64_t)slot * smp.g[b].ntot;
-        if (z <= z1 && z >= z0) sample_cell(smp.g[b], sp, xs, ys, z, 0, e1);
-        if (z - 1 >= z0 && z - 1 <= z1) sample_cell(smp.g[b], sp, xs, ys, z - 1, 3, h1n);
+        const unsigned f = (fbits >> (5 * b)) & 31u;
+        if (z <= z1 && z >= z0) sample_cell(smp.g[b], sp, f, xs, ys, z, 0, e1);
+        if (z - 1 >= z0 && z - 1 <= z1) sample_cell(smp.g[b], sp, f, xs, ys, z - 1, 3, h1n);
       }
     }
     // rotate the pipeline

@@ -160,6 +160,8 @@ __device__ __forceinline__ void yee2p_tile(const StepArgs<float>& a, float2* __r
     if (a.nt_out) pst_nt(f, i, v);
     else pst(f, i, v);
   };
+  // the DFT-box faces of this lane's cells a (bits 0..9) and b (10..19)
+  const unsigned fbits = face_bits2(smp, xs, ys) | (face_bits2(smp, xs + 1, ys) << 10);
 
   // pipeline registers: H0(z-1), E1(z-1), H1(z-2), E2(z-2)
   P2 hz1[3], e1p[3] = {}, h1pp[3] = {}, e2p[3] = {};
@@ -296,15 +298,16 @@ __device__ __forceinline__ void yee2p_tile(const StepArgs<float>& a, float2* __r
       for (int b = 0; slot >= 0 && b < smp.nbox; ++b) {
         if (!smp.samp[b]) continue;
         C* const sp = smp.samp[b] + (int64_t)slot * smp.g[b].ntot;
+        const unsigned fa = (fbits >> (5 * b)) & 31u, fb = (fbits >> (10 + 5 * b)) & 31u;
         if (z <= z1 && z >= z0) {
           const C va[3] = {e1[0].a, e1[1].a, e1[2].a}, vb[3] = {e1[0].b, e1[1].b, e1[2].b};
-          sample_cell(smp.g[b], sp, xs, ys, z, 0, va);
-          sample_cell(smp.g[b], sp, xs + 1, ys, z, 0, vb);
+          sample_cell(smp.g[b], sp, fa, xs, ys, z, 0, va);
+          sample_cell(smp.g[b], sp, fb, xs + 1, ys, z, 0, vb);
         }
         if (z - 1 >= z0 && z - 1 <= z1) {
           const C va[3] = {h1n[0].a, h1n[1].a, h1n[2].a}, vb[3] = {h1n[0].b, h1n[1].b, h1n[2].b};
-          sample_cell(smp.g[b], sp, xs, ys, z - 1, 3, va);
-          sample_cell(smp.g[b], sp, xs + 1, ys, z - 1, 3, vb);
+          sample_cell(smp.g[b], sp, fa, xs, ys, z - 1, 3, va);
+          sample_cell(smp.g[b], sp, fb, xs + 1, ys, z - 1, 3, vb);
         }
       }
     }
