@@ -404,9 +404,9 @@ __device__ inline void adj_e_item(const StepArgs<T>& a, int xs, int ys, int zs, 
   if (q.kind == kAdjPlain) {
     // X' = X + a curl_b(Y), as the forward E update of region I / III
     const T* ac = (zs < a.hi) ? a.a1 : a.a3;
-    x[0] = cadd(e0, cscale(ac[0], c0));
-    x[1] = cadd(e1, cscale(ac[1], c1));
-    x[2] = cadd(e2, cscale(ac[2], c2));
+    x[0] = cmadd<T>(e0, ac[0], c0);
+    x[1] = cmadd<T>(e1, ac[1], c1);
+    x[2] = cmadd<T>(e2, ac[2], c2);
     en[0] = x[0]; en[1] = x[1]; en[2] = x[2];
   } else {
     const C em[3] = {cadd(e0, c0), cadd(e1, c1), cadd(e2, c2)};
@@ -524,9 +524,9 @@ __global__ __launch_bounds__(kFTX * (TY + 1), 1) void k_adj_gen(
       if (adj_kind(a, zh) == kAdjPlain) {
         // Y' = Y - dtcq2 curl_f(X), as the forward H update, with the
         // min-side ghosts the forward passes read
-        hn[0] = csub(hl[0], cscale(a.dtcq2, c0));
-        hn[1] = csub(hl[1], cscale(a.dtcq2, c1));
-        hn[2] = csub(hl[2], cscale(a.dtcq2, c2));
+        hn[0] = cmsub<T>(hl[0], a.dtcq2, c0);
+        hn[1] = cmsub<T>(hl[1], a.dtcq2, c1);
+        hn[2] = cmsub<T>(hl[2], a.dtcq2, c2);
         if (xa == nx) store3_phase(a.hn, ih - nx, cs, a.fxm, hn);
         if (ya == ny) store3_phase(a.hn, ih - a.sx * ny, cs, a.fym, hn);
       } else {

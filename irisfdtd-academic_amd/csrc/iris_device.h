@@ -18,6 +18,18 @@ template <typename C> __host__ __device__ inline C csub(C a, C b) { C r; r.x = a
 // real * complex: the reference promotes the real to (r,0); for finite
 // operands that equals (r*re, r*im)
 template <typename C, typename T> __host__ __device__ inline C cscale(T s, C a) { C r; r.x = s * a.x; r.y = s * a.y; return r; }
+// The Yee updates e + s*c and h - s*c (real s): with FP32 fields one fused
+// multiply-add per component (one rounding; every FP32 pass forms them this
+// way, so the passes stay bitwise equal to each other), with FP64 fields the
+// reference's separate multiply and add.
+template <typename T> __device__ inline cplx<T> cmadd(cplx<T> e, T s, cplx<T> c) {
+  if constexpr (sizeof(T) == 4) return mk<T>(__builtin_fmaf(s, c.x, e.x), __builtin_fmaf(s, c.y, e.y));
+  else return cadd(e, cscale(s, c));
+}
+template <typename T> __device__ inline cplx<T> cmsub(cplx<T> h, T s, cplx<T> c) {
+  if constexpr (sizeof(T) == 4) return mk<T>(__builtin_fmaf(-s, c.x, h.x), __builtin_fmaf(-s, c.y, h.y));
+  else return csub(h, cscale(s, c));
+}
 // complex * complex as (ac-bd, ad+bc)
 template <typename C> __host__ __device__ inline C cmul(C a, C b) {
   C r; r.x = a.x * b.x - a.y * b.y; r.y = a.x * b.y + a.y * b.x; return r;

@@ -56,9 +56,9 @@ __device__ inline void e_cell(const StepArgs<T>& a, int64_t col, int iz, int64_t
     n[2] = cadd(cscale(P.ce[2], e2), cscale(P.ge[2], csub(cscale(P.fp[2], d2), cscale(P.fm[2], d2o))));
   } else if (iz < a.hi) {
     // region I, reference code/fdtd.f90:86-96 (off-diagonal terms are 0)
-    n[0] = cadd(e0, cscale(a.a1[0], c0));
-    n[1] = cadd(e1, cscale(a.a1[1], c1));
-    n[2] = cadd(e2, cscale(a.a1[2], c2));
+    n[0] = cmadd<T>(e0, a.a1[0], c0);
+    n[1] = cmadd<T>(e1, a.a1[1], c1);
+    n[2] = cmadd<T>(e2, a.a1[2], c2);
   } else if (iz <= a.hf) {
     // structure, reference code/fdtd.f90:115-117
     const int64_t eidx = col + a.sxy * (iz - a.hi);
@@ -69,15 +69,15 @@ __device__ inline void e_cell(const StepArgs<T>& a, int64_t col, int iz, int64_t
       n[1] = cadd(e1, cmul(ei, c1));
       n[2] = cadd(e2, cmul(ei, c2));
     } else {
-      n[0] = cadd(e0, cscale(er, c0));
-      n[1] = cadd(e1, cscale(er, c1));
-      n[2] = cadd(e2, cscale(er, c2));
+      n[0] = cmadd<T>(e0, er, c0);
+      n[1] = cmadd<T>(e1, er, c1);
+      n[2] = cmadd<T>(e2, er, c2);
     }
   } else {
     // region III, reference code/fdtd.f90:137-147
-    n[0] = cadd(e0, cscale(a.a3[0], c0));
-    n[1] = cadd(e1, cscale(a.a3[1], c1));
-    n[2] = cadd(e2, cscale(a.a3[2], c2));
+    n[0] = cmadd<T>(e0, a.a3[0], c0);
+    n[1] = cmadd<T>(e1, a.a3[1], c1);
+    n[2] = cmadd<T>(e2, a.a3[2], c2);
   }
 }
 
@@ -105,9 +105,9 @@ __device__ inline void h_cell(const StepArgs<T>& a, int64_t col, int iz, cplx<T>
     n[2] = cadd(cscale(P.ce[2], h2), cscale(P.gh[2], csub(cscale(P.fp[2], b2), cscale(P.fm[2], b2o))));
   } else {
     // bulk, reference code/fdtd.f90:191-195 (mu_inv_bbox = 1)
-    n[0] = csub(h0, cscale(a.dtcq2, c0));
-    n[1] = csub(h1, cscale(a.dtcq2, c1));
-    n[2] = csub(h2, cscale(a.dtcq2, c2));
+    n[0] = cmsub<T>(h0, a.dtcq2, c0);
+    n[1] = cmsub<T>(h1, a.dtcq2, c1);
+    n[2] = cmsub<T>(h2, a.dtcq2, c2);
   }
 }
 
@@ -379,9 +379,9 @@ __device__ inline void e_item(const StepArgs<T>& a, int xs, int ys, int zs, cplx
   } else {
     // bulk planes: region I or III only (reference code/fdtd.f90:86-96, :137-147)
     const T* ac = (zs < a.hi) ? a.a1 : a.a3;
-    n[0] = cadd(a.e[idx], cscale(ac[0], c0));
-    n[1] = cadd(a.e[idx + cs], cscale(ac[1], c1));
-    n[2] = cadd(a.e[idx + 2 * cs], cscale(ac[2], c2));
+    n[0] = cmadd<T>(a.e[idx], ac[0], c0);
+    n[1] = cmadd<T>(a.e[idx + cs], ac[1], c1);
+    n[2] = cmadd<T>(a.e[idx + 2 * cs], ac[2], c2);
   }
   hown[0] = h1;
   hown[1] = h2;
@@ -518,9 +518,9 @@ __device__ __forceinline__ void yee_fused_tile(const StepArgs<T>& a, const TfsfC
       if constexpr (GEN) {
         h_cell<T>(a, cola, zh, a1m, a2m, a3m, c0, c1, c2, hn);
       } else {
-        hn[0] = csub(a1m, cscale(a.dtcq2, c0));
-        hn[1] = csub(a2m, cscale(a.dtcq2, c1));
-        hn[2] = csub(a3m, cscale(a.dtcq2, c2));
+        hn[0] = cmsub<T>(a1m, a.dtcq2, c0);
+        hn[1] = cmsub<T>(a2m, a.dtcq2, c1);
+        hn[2] = cmsub<T>(a3m, a.dtcq2, c2);
       }
       C gx[3] = {cmul(a.fxm, hn[0]), cmul(a.fxm, hn[1]), cmul(a.fxm, hn[2])};
       C gy[3] = {cmul(a.fym, hn[0]), cmul(a.fym, hn[1]), cmul(a.fym, hn[2])};

@@ -122,9 +122,9 @@ __device__ inline void e_update_plain(const StepArgs<T>& a, int64_t col, int z, 
                                       const cplx<T> c[3], cplx<T> n[3]) {
   if (z < a.hi || z > a.hf) {
     const T* ac = (z < a.hi) ? a.a1 : a.a3;
-    n[0] = cadd(e[0], cscale(ac[0], c[0]));
-    n[1] = cadd(e[1], cscale(ac[1], c[1]));
-    n[2] = cadd(e[2], cscale(ac[2], c[2]));
+    n[0] = cmadd<T>(e[0], ac[0], c[0]);
+    n[1] = cmadd<T>(e[1], ac[1], c[1]);
+    n[2] = cmadd<T>(e[2], ac[2], c[2]);
   } else {
     const int64_t eidx = col + a.sxy * (z - a.hi);
     const T er = a.eps_re[eidx];
@@ -134,9 +134,9 @@ __device__ inline void e_update_plain(const StepArgs<T>& a, int64_t col, int z, 
       n[1] = cadd(e[1], cmul(ei, c[1]));
       n[2] = cadd(e[2], cmul(ei, c[2]));
     } else {
-      n[0] = cadd(e[0], cscale(er, c[0]));
-      n[1] = cadd(e[1], cscale(er, c[1]));
-      n[2] = cadd(e[2], cscale(er, c[2]));
+      n[0] = cmadd<T>(e[0], er, c[0]);
+      n[1] = cmadd<T>(e[1], er, c[1]);
+      n[2] = cmadd<T>(e[2], er, c[2]);
     }
   }
 }
@@ -309,9 +309,9 @@ __device__ __forceinline__ void yee2_tile(const StepArgs<T>& a, cplx<T>* __restr
         const C cc[3] = {c0, c1, c2};
         h_upml3<T>(a.pml[z - 1], hz1, cc, bold, b1n, h1n);
       } else {
-        h1n[0] = csub(hz1[0], cscale(a.dtcq2, c0));
-        h1n[1] = csub(hz1[1], cscale(a.dtcq2, c1));
-        h1n[2] = csub(hz1[2], cscale(a.dtcq2, c2));
+        h1n[0] = cmsub<T>(hz1[0], a.dtcq2, c0);
+        h1n[1] = cmsub<T>(hz1[1], a.dtcq2, c1);
+        h1n[2] = cmsub<T>(hz1[2], a.dtcq2, c2);
       }
     }
     // slot pz last held E1(z-2), read before the previous plane's second
@@ -350,9 +350,9 @@ __device__ __forceinline__ void yee2_tile(const StepArgs<T>& a, cplx<T>* __restr
         const C cc[3] = {c0, c1, c2};
         h_upml3<T>(a.pml[z - 2], h1pp, cc, b1pp, b2n, h2n);
       } else {
-        h2n[0] = csub(h1pp[0], cscale(a.dtcq2, c0));
-        h2n[1] = csub(h1pp[1], cscale(a.dtcq2, c1));
-        h2n[2] = csub(h1pp[2], cscale(a.dtcq2, c2));
+        h2n[0] = cmsub<T>(h1pp[0], a.dtcq2, c0);
+        h2n[1] = cmsub<T>(h1pp[1], a.dtcq2, c1);
+        h2n[2] = cmsub<T>(h1pp[2], a.dtcq2, c2);
       }
     }
     if constexpr (NB == 1) __syncthreads();  // the E2(z-2) and H1(z-1) neighbours have been read

@@ -45,6 +45,9 @@ __device__ __forceinline__ P2 padd(P2 u, P2 v) { return P2{cadd(u.a, v.a), cadd(
 __device__ __forceinline__ P2 psub(P2 u, P2 v) { return P2{csub(u.a, v.a), csub(u.b, v.b)}; }
 __device__ __forceinline__ P2 pscale(float s, P2 u) { return P2{cscale(s, u.a), cscale(s, u.b)}; }
 __device__ __forceinline__ P2 pmul(float2 f, P2 u) { return P2{cmul(f, u.a), cmul(f, u.b)}; }
+// e + s*c and h - s*c as fused multiply-adds (cmadd / cmsub per cell)
+__device__ __forceinline__ P2 pmadd(P2 e, float s, P2 c) { return P2{cmadd<float>(e.a, s, c.a), cmadd<float>(e.b, s, c.b)}; }
+__device__ __forceinline__ P2 pmsub(P2 h, float s, P2 c) { return P2{cmsub<float>(h.a, s, c.a), cmsub<float>(h.b, s, c.b)}; }
 
 __device__ __forceinline__ P2 pld(const float2* __restrict__ f, int64_t i) {
   const float4 v = *reinterpret_cast<const float4*>(f + i);
@@ -99,7 +102,7 @@ __device__ __forceinline__ void p_e_update(const StepArgs<float>& a, int64_t col
   if (z < a.hi || z > a.hf) {
     const float* ac = (z < a.hi) ? a.a1 : a.a3;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) n[k] = padd(e[k], pscale(ac[k], c[k]));
+    for (int k = 0; k < 3; ++k) n[k] = pmadd(e[k], ac[k], c[k]);
   } else {
     const int64_t eidx = col + a.sxy * (z - a.hi);
     const float2 er = *reinterpret_cast<const float2*>(a.eps_re + eidx);  // cells a, b
@@ -110,7 +113,7 @@ __device__ __forceinline__ void p_e_update(const StepArgs<float>& a, int64_t col
       for (int k = 0; k < 3; ++k) n[k] = P2{cadd(e[k].a, cmul(ea, c[k].a)), cadd(e[k].b, cmul(eb, c[k].b))};
     } else {
 #pragma unroll
-      for (int k = 0; k < 3; ++k) n[k] = P2{cadd(e[k].a, cscale(er.x, c[k].a)), cadd(e[k].b, cscale(er.y, c[k].b))};
+      for (int k = 0; k < 3; ++k) n[k] = P2{cmadd<float>(e[k].a, er.x, c[k].a), cmadd<float>(e[k].b, er.y, c[k].b)};
     }
   }
 }
@@ -199,9 +202,9 @@ __device__ __forceinline__ void yee2p_tile(const StepArgs<float>& a, float2* __r
         const P2 c0 = padd(psub(psub(e3yp, e1p[2]), e1[1]), e1p[1]);
         const P2 c1 = padd(psub(psub(e1[0], e1p[0]), e3xp), e1p[2]);
         const P2 c2 = padd(psub(psub(e2xp, e1p[1]), e1yp), e1p[0]);
-        h1n[0] = psub(hz1[0], pscale(a.dtcq2, c0));
-        h1n[1] = psub(hz1[1], pscale(a.dtcq2, c1));
-        h1n[2] = psub(hz1[2], pscale(a.dtcq2, c2));
+        h1n[0] = pmsub(hz1[0], a.dtcq2, c0);
+        h1n[1] = pmsub(hz1[1], a.dtcq2, c1);
+        h1n[2] = pmsub(hz1[2], a.dtcq2, c2);
       }
     }
     if (!idle) {
@@ -232,9 +235,9 @@ __device__ __forceinline__ void yee2p_tile(const StepArgs<float>& a, float2* __r
         const P2 c0 = padd(psub(psub(e3yp, e2p[2]), e2n[1]), e2p[1]);
         const P2 c1 = padd(psub(psub(e2n[0], e2p[0]), e3xp), e2p[2]);
         const P2 c2 = padd(psub(psub(e2xp, e2p[1]), e1yp), e2p[0]);
-        h2n[0] = psub(h1pp[0], pscale(a.dtcq2, c0));
-        h2n[1] = psub(h1pp[1], pscale(a.dtcq2, c1));
-        h2n[2] = psub(h1pp[2], pscale(a.dtcq2, c2));
+        h2n[0] = pmsub(h1pp[0], a.dtcq2, c0);
+        h2n[1] = pmsub(h1pp[1], a.dtcq2, c1);
+        h2n[2] = pmsub(h1pp[2], a.dtcq2, c2);
       }
     }
     if (!idle) {
