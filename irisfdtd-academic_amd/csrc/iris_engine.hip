@@ -519,7 +519,7 @@ void launch_fused_lists(iris_handle* h, const StepArgs<T>& a, int it, const int*
 }
 
 template <typename T>
-int dft_sample(iris_handle* h, int n);
+int dft_sample(iris_handle* h, int n, const unsigned char* xmask = nullptr);
 template <typename T>
 int dft_flush(iris_handle* h);
 
@@ -528,7 +528,7 @@ int dft_flush(iris_handle* h);
 // full and narrow tiles instead).
 template <typename T, bool PH, bool PML>
 void launch_yee2(const iris_handle* h, const StepArgs<T>& a, cplx<T>* eb, cplx<T>* hb, cplx<T>* db,
-                 const int* chunks, const TileMap& tm, int nch, const Yee2Samples<T>& smp) {
+                 const int* chunks, const TileMap& tm, int nch, const Yee2Samples<T>& smp, int slot, int slot2) {
   if constexpr (sizeof(T) == 4 && !PML) {
     if (h->t2w == 128) {
       const P2Part p0{TileMap{h->tx2, h->ty2, h->tx2 * h->ty2 * nch, h->tx2 * h->xcd_rows2, 0}, h->p2ox, h->p2sl,
@@ -539,7 +539,7 @@ void launch_yee2(const iris_handle* h, const StepArgs<T>& a, cplx<T>* eb, cplx<T
       const unsigned nblk = (unsigned)(npad0 + p1.tm.nblk);
       if (nblk)
         hipLaunchKernelGGL((k_yee2p<PH>), dim3(nblk), dim3(64, kP2R), 0, h->stream, a, (float2*)eb, (float2*)hb,
-                           chunks, p0, p1, npad0, smp);
+                           chunks, p0, p1, npad0, smp, slot, slot2);
       return;
     }
   }
@@ -547,10 +547,10 @@ void launch_yee2(const iris_handle* h, const StepArgs<T>& a, cplx<T>* eb, cplx<T
   const dim3 grid((unsigned)tm.nblk);
   if ((PML ? h->t2wp : h->t2w) == 32)
     hipLaunchKernelGGL((k_yee2<T, PH, PML, 32>), grid, dim3(kT2X, Yee2Shape<T, PML, 32>::R), 0, h->stream, a, eb,
-                       hb, db, chunks, tm, smp);
+                       hb, db, chunks, tm, smp, slot, slot2);
   else
     hipLaunchKernelGGL((k_yee2<T, PH, PML, 64>), grid, dim3(kT2X, Yee2Shape<T, PML, 64>::R), 0, h->stream, a, eb,
-                       hb, db, chunks, tm, smp);
+                       hb, db, chunks, tm, smp, slot, slot2);
 }
 
 // Argument table of the handle's single-step length groups for its current
@@ -602,11 +602,16 @@ int launch_double(iris_handle* h, int it, const PassLists& L, bool dft) {
   const iris_params& p = h->p;
   const bool s1 = dft && it >= p.to_fourier && (it % p.sample_fourier) == 0;
   const bool s2 = dft && it + 1 >= p.to_fourier && ((it + 1) % p.sample_fourier) == 0;
+  // ring slots: step n+1 (every face of the two-step kernels' own cells),
+  // step n+2 (their x faces; not when step n+1 fills the ring, whose fold
+  // must come first)
+  const int slot = s1 ? h->nsamp : -1;
+  const int slot2 = (s2 && h->nsamp + (s1 ? 1 : 0) < dft_slots<T>()) ? h->nsamp + (s1 ? 1 : 0) : -1;
   Yee2Samples<T> smp{};
   smp.nbox = p.nbox;
   for (int b = 0; b < p.nbox; ++b) {
     smp.g[b] = h->box[b].g;
-    smp.samp[b] = (s1 && h->box[b].g.ntot) ? (C*)h->box[b].samp + (int64_t)h->nsamp * h->box[b].g.ntot : nullptr;
+    smp.samp[b] = h->box[b].g.ntot ? (C*)h->box[b].samp : nullptr;
   }
   const RestGroup one{h->zchunk, 0, L.nlean, L.ngen};
   const RestGroup* groups = L.groups ? L.groups : &one;
@@ -645,9 +650,9 @@ int launch_double(iris_handle* h, int it, const PassLists& L, bool dft) {
     const StepArgs<T> a2 = step_args_on<T>(h, h->fe, h->fh, h->ke, h->kh, h->d, h->b, h->dn, h->bn);
     const TileMap tm{h->tx2, h->ty2, h->tx2 * h->ty2 * L.n2, h->tx2 * h->xcd_rows2};
     if (h->bloch)
-      launch_yee2<T, true, false>(h, a2, (C*)h->ge, (C*)h->gh, nullptr, L.chunks2, tm, L.n2, smp);
+      launch_yee2<T, true, false>(h, a2, (C*)h->ge, (C*)h->gh, nullptr, L.chunks2, tm, L.n2, smp, slot, slot2);
     else
-      launch_yee2<T, false, false>(h, a2, (C*)h->ge, (C*)h->gh, nullptr, L.chunks2, tm, L.n2, smp);
+      launch_yee2<T, false, false>(h, a2, (C*)h->ge, (C*)h->gh, nullptr, L.chunks2, tm, L.n2, smp, slot, slot2);
   }
   if (L.n2p) {
     // the UPML variant: D and B of step n+2 into the third aux buffers
@@ -655,9 +660,9 @@ int launch_double(iris_handle* h, int it, const PassLists& L, bool dft) {
     const StepArgs<T> a2 = step_args_on<T>(h, h->fe, h->fh, h->ke, h->kh, h->d, h->b, h->kd, h->kb);
     const TileMap tm{h->tx2p, h->ty2p, h->tx2p * h->ty2p * L.n2p, h->tx2p * h->xcd_rows2};
     if (h->bloch)
-      launch_yee2<T, true, true>(h, a2, (C*)h->ge, (C*)h->gh, (C*)h->dn, L.chunks2p, tm, L.n2p, smp);
+      launch_yee2<T, true, true>(h, a2, (C*)h->ge, (C*)h->gh, (C*)h->dn, L.chunks2p, tm, L.n2p, smp, slot, slot2);
     else
-      launch_yee2<T, false, true>(h, a2, (C*)h->ge, (C*)h->gh, (C*)h->dn, L.chunks2p, tm, L.n2p, smp);
+      launch_yee2<T, false, true>(h, a2, (C*)h->ge, (C*)h->gh, (C*)h->dn, L.chunks2p, tm, L.n2p, smp, slot, slot2);
   }
   if (L.nlean + L.ngen) {
     ProfScope ps(h, IRIS_K_YEE_REST);
@@ -688,7 +693,8 @@ int launch_double(iris_handle* h, int it, const PassLists& L, bool dft) {
   std::swap(h->b, h->kb);
   h->it = it;
   if (s2) {
-    int rc = dft_sample<T>(h, it + 2 - p.to_fourier);
+    // the x faces of the two-step planes are in slot2 already
+    int rc = dft_sample<T>(h, it + 2 - p.to_fourier, slot2 >= 0 ? L.zmask : nullptr);
     if (rc) return rc;
   }
   h->it = it + 1;
@@ -769,7 +775,7 @@ int dft_flush(iris_handle* h) {
 }
 
 template <typename T>
-int dft_sample(iris_handle* h, int n) {
+int dft_sample(iris_handle* h, int n, const unsigned char* xmask) {
   if (h->nsamp == 0) h->samp_n0 = n;
   StepArgs<T> a = step_args<T>(h);
   for (int b = 0; b < h->p.nbox; ++b) {
@@ -777,7 +783,7 @@ int dft_sample(iris_handle* h, int n) {
     if (B.g.ntot == 0) continue;
     ProfScope ps(h, IRIS_K_DFT_SAMPLE);
     hipLaunchKernelGGL(k_dft_sample<T>, dim3((unsigned)((B.g.ntot + 255) / 256)), dim3(256), 0, h->stream, a, B.g,
-                       (cplx<T>*)B.samp, h->nsamp);
+                       (cplx<T>*)B.samp, h->nsamp, xmask);
   }
   HIP_TRY(h, hipGetLastError());
   h->nsamp++;
@@ -1890,9 +1896,11 @@ int adjoint_double(iris_handle* h, int n, int nobox, int face_on, const void* re
     const TileMap tm{h->tx2, h->ty2, h->tx2 * h->ty2 * L.n2, h->tx2 * h->xcd_rows2};
     Yee2Samples<T> none{};
     if (h->bloch)
-      launch_yee2<T, true, false>(h, a2, (C*)h->ae2, (C*)h->ah2, nullptr, (const int*)L.chunks2, tm, L.n2, none);
+      launch_yee2<T, true, false>(h, a2, (C*)h->ae2, (C*)h->ah2, nullptr, (const int*)L.chunks2, tm, L.n2, none, -1,
+                                  -1);
     else
-      launch_yee2<T, false, false>(h, a2, (C*)h->ae2, (C*)h->ah2, nullptr, (const int*)L.chunks2, tm, L.n2, none);
+      launch_yee2<T, false, false>(h, a2, (C*)h->ae2, (C*)h->ah2, nullptr, (const int*)L.chunks2, tm, L.n2, none, -1,
+                                   -1);
   }
   const StepArgs<T> a3 = adj_args<T>(h, h->ae2, h->ah2, h->ae3, h->ah3, h->ad2, h->ab2, h->ad, h->ab);
   adj_sources<T>(h, a3, n - 1, nobox, face_on);
@@ -2479,10 +2487,12 @@ int batch_tables2(iris_batch* b) {
         if (pp == 0) {
           S.a[0] = S.a[1] = step_args_on<T>(h, h->ge, h->gh, h->ge, h->gh, h->dn, h->bn, h->dn, h->bn);
           S.zmask = h->zmask;
+          S.xmask = nullptr;
         } else {
           S.a[0] = step_args_on<T>(h, h->ke, h->kh, h->ke, h->kh, h->kd, h->kb, h->kd, h->kb);
           S.a[1] = step_args_on<T>(h, h->fe, h->fh, h->fe, h->fh, h->d, h->b, h->d, h->b);
           S.zmask = nullptr;
+          S.xmask = h->zmask;  // with xm: the x faces of the two-step planes are stored by their kernels
         }
       }
     }
@@ -2553,12 +2563,12 @@ int batch_create2(iris_batch* b) {
 // one DFT sample of every two-step handle of the batch (table half `half`,
 // state a[par]), then each handle's ring bookkeeping as dft_sample does it
 template <typename T>
-int batch_sample2(iris_batch* b, int half, int par, int n) {
+int batch_sample2(iris_batch* b, int half, int par, int n, int xm) {
   iris_handle* h0 = b->own[0];
   if (b->nent2)
     hipLaunchKernelGGL(k_dft_sample_batch<T>, dim3((unsigned)((b->npts2 + 255) / 256)), dim3(256), 0, b->stream,
                        (const BatchSamp<T>*)b->ent2 + (size_t)half * b->nent2, (const int64_t*)b->ent2_start,
-                       b->nent2, par, h0->nsamp);
+                       b->nent2, par, h0->nsamp, xm);
   for (iris_handle* h : b->own) {
     if (h->nsamp == 0) h->samp_n0 = n;
     h->nsamp++;
@@ -2575,19 +2585,19 @@ int batch_sample2(iris_batch* b, int half, int par, int n) {
 
 template <typename T, bool PH, bool PML>
 void launch_yee2_batch(const iris_batch* b, int W, unsigned nblk, const Batch2Inst<T>* I, const int* wg, int n,
-                       int slot) {
+                       int slot, int slot2) {
   if constexpr (sizeof(T) == 4 && !PML) {
     if (W == 128) {
-      hipLaunchKernelGGL((k_yee2p_batch<PH>), dim3(nblk), dim3(64, kP2R), 0, b->stream, I, wg, n, slot);
+      hipLaunchKernelGGL((k_yee2p_batch<PH>), dim3(nblk), dim3(64, kP2R), 0, b->stream, I, wg, n, slot, slot2);
       return;
     }
   }
   if (W == 32)
     hipLaunchKernelGGL((k_yee2_batch<T, PH, PML, 32>), dim3(nblk), dim3(kT2X, Yee2Shape<T, PML, 32>::R), 0,
-                       b->stream, I, wg, n, slot);
+                       b->stream, I, wg, n, slot, slot2);
   else
     hipLaunchKernelGGL((k_yee2_batch<T, PH, PML, 64>), dim3(nblk), dim3(kT2X, Yee2Shape<T, PML, 64>::R), 0,
-                       b->stream, I, wg, n, slot);
+                       b->stream, I, wg, n, slot, slot2);
 }
 
 // steps it_first..it_last of the two-step handles: pairs of steps while at
@@ -2614,6 +2624,9 @@ int batch_run2(iris_batch* b, int it_first, int it_last) {
     const bool s1 = it >= p.to_fourier && (it % p.sample_fourier) == 0;
     const bool s2 = it + 1 >= p.to_fourier && ((it + 1) % p.sample_fourier) == 0;
     const int slot = s1 ? b->own[0]->nsamp : -1;
+    // the x faces of step n+2 by the two-step kernels (as launch_double)
+    const int ns2 = b->own[0]->nsamp + (s1 ? 1 : 0);
+    const int slot2 = (s2 && ns2 < dft_slots<T>()) ? ns2 : -1;
     int rc;
     rest(0, it);
     for (int bl = 0; bl < 16; ++bl) {
@@ -2625,18 +2638,18 @@ int batch_run2(iris_batch* b, int it_first, int it_last) {
         const int W = (!pm && (bl & 8)) ? 128 : ((bl & (pm ? 4 : 2)) ? 32 : 64);
         const unsigned g = (unsigned)b->y2nwg[bl][pm];
         if (bl & 1) {
-          if (pm) launch_yee2_batch<T, true, true>(b, W, g, I, w, n, slot);
-          else launch_yee2_batch<T, true, false>(b, W, g, I, w, n, slot);
+          if (pm) launch_yee2_batch<T, true, true>(b, W, g, I, w, n, slot, slot2);
+          else launch_yee2_batch<T, true, false>(b, W, g, I, w, n, slot, slot2);
         } else {
-          if (pm) launch_yee2_batch<T, false, true>(b, W, g, I, w, n, slot);
-          else launch_yee2_batch<T, false, false>(b, W, g, I, w, n, slot);
+          if (pm) launch_yee2_batch<T, false, true>(b, W, g, I, w, n, slot, slot2);
+          else launch_yee2_batch<T, false, false>(b, W, g, I, w, n, slot, slot2);
         }
       }
     }
     rest(1, it + 1);
     if (hipGetLastError() != hipSuccess) return bfail(b, IRIS_EDEVICE, "batched two-step launch failed");
-    if (s1 && (rc = batch_sample2<T>(b, 0, 0, it + 1 - p.to_fourier))) return rc;
-    if (s2 && (rc = batch_sample2<T>(b, 1, pp, it + 2 - p.to_fourier))) return rc;
+    if (s1 && (rc = batch_sample2<T>(b, 0, 0, it + 1 - p.to_fourier, 0))) return rc;
+    if (s2 && (rc = batch_sample2<T>(b, 1, pp, it + 2 - p.to_fourier, slot2 >= 0 ? 1 : 0))) return rc;
     pp ^= 1;
     return 0;
   };
@@ -2848,7 +2861,7 @@ int batch_step_impl(iris_batch* b, int it_first, int it_last) {
     if (it >= h0->p.to_fourier && (it % h0->p.sample_fourier) == 0 && b->nent) {
       // the step just taken is in the buffers of parity `par`'s source side
       hipLaunchKernelGGL(k_dft_sample_batch<T>, dim3((unsigned)((b->npts + 255) / 256)), dim3(256), 0, b->stream,
-                         (const BatchSamp<T>*)b->ent, (const int64_t*)b->ent_start, b->nent, par, h0->nsamp);
+                         (const BatchSamp<T>*)b->ent, (const int64_t*)b->ent_start, b->nent, par, h0->nsamp, 0);
       for (iris_handle* h : b->one) {
         if (h->nsamp == 0) h->samp_n0 = it + 1 - h->p.to_fourier;
         h->nsamp++;

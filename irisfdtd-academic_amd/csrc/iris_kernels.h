@@ -46,13 +46,22 @@ __device__ inline void box_point(const BoxGeom& g, int64_t i, int* comp, int* x,
   }
 }
 
+// whether point index i of box g lies on an x face (the first pair block)
+__device__ inline bool on_xface(const BoxGeom& g, int64_t i) {
+  return g.npts[0] && i < g.start[0] + g.npts[0];
+}
+
 // copy the box-face samples of this step into slot `slot` of the buffer
+// (xmask set: not the x-face points on the planes z with xmask[z] != 0,
+// which the two-step kernels have stored)
 template <typename T>
-__global__ __launch_bounds__(256) void k_dft_sample(StepArgs<T> a, BoxGeom g, cplx<T>* samp, int slot) {
+__global__ __launch_bounds__(256) void k_dft_sample(StepArgs<T> a, BoxGeom g, cplx<T>* samp, int slot,
+                                                   const unsigned char* __restrict__ xmask) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= g.ntot) return;
   int c, x, y, z;
   box_point(g, i, &c, &x, &y, &z);
+  if (xmask && xmask[z] && on_xface(g, i)) return;
   const int64_t idx = (int64_t)(x + a.xoff) + a.sx * y + a.sxy * z;
   samp[(int64_t)slot * g.ntot + i] = (c < 3) ? a.e[idx + c * a.cs] : a.h[idx + (c - 3) * a.cs];
 }
@@ -67,11 +76,12 @@ template <typename T> struct BatchSamp {
   BoxGeom g;
   cplx<T>* samp;
   const unsigned char* zmask;  // when set: skip the planes z with zmask[z] != 0
+  const unsigned char* xmask;  // with xm: skip the x-face points of the planes z with xmask[z] != 0
 };
 template <typename T>
 __global__ __launch_bounds__(256) void k_dft_sample_batch(const BatchSamp<T>* __restrict__ ent,
                                                           const int64_t* __restrict__ start, int nent, int par,
-                                                          int slot) {
+                                                          int slot, int xm) {
   const int64_t b0 = (int64_t)blockIdx.x * blockDim.x;
   if (b0 >= start[nent]) return;
   int lo = 0, hi = nent;
@@ -87,6 +97,7 @@ __global__ __launch_bounds__(256) void k_dft_sample_batch(const BatchSamp<T>* __
   int c, x, y, z;
   box_point(E.g, r, &c, &x, &y, &z);
   if (E.zmask && E.zmask[z]) return;
+  if (xm && E.xmask && E.xmask[z] && on_xface(E.g, r)) return;
   const int64_t idx = (int64_t)(x + a.xoff) + a.sx * y + a.sxy * z;
   E.samp[(int64_t)slot * E.g.ntot + r] = (c < 3) ? a.e[idx + c * a.cs] : a.h[idx + (c - 3) * a.cs];
 }

@@ -69,6 +69,8 @@ __device__ inline unsigned face_bits(const BoxGeom& g, int x, int y) {
   if (g.npts[2] && inx && iny) f |= 16u;
   return f;
 }
+// the x-face bits of both boxes in a face_bits2 set
+constexpr unsigned kXFaceBits = 3u | (3u << 5);
 // the bits of both boxes of a sampling set (5 a box)
 template <typename T> struct Yee2Samples;
 template <typename T>
@@ -109,6 +111,23 @@ __device__ inline void sample_cell(const BoxGeom& g, C* __restrict__ samp, unsig
     if (z == g.bzi) { samp[q] = v[0]; samp[q + 1] = v[1]; samp[q + 2] = v[2]; }
     if (z == g.bzf) {
       const int64_t r = q + g.npts[2] / 2;
+      samp[r] = v[0]; samp[r + 1] = v[1]; samp[r + 2] = v[2];
+    }
+  }
+}
+
+// The x faces alone (f: the cell's face bits for g): the points whose
+// gather from the fields costs a cache line each (consecutive points of an x
+// face are rows apart).  The two-step kernels store their step n+2 values
+// there themselves; the gather of that step skips them on their planes.
+template <typename C>
+__device__ inline void sample_xface(const BoxGeom& g, C* __restrict__ samp, unsigned f, int y, int z, int base,
+                                    const C v[3]) {
+  if ((f & 3u) && z >= g.bzi && z <= g.bzf) {
+    const int64_t q = g.start[0] + 6 * ((int64_t)(y - g.byi) + (int64_t)g.nyb * (z - g.bzi)) + base;
+    if (f & 1u) { samp[q] = v[0]; samp[q + 1] = v[1]; samp[q + 2] = v[2]; }
+    if (f & 2u) {
+      const int64_t r = q + g.npts[0] / 2;
       samp[r] = v[0]; samp[r + 1] = v[1]; samp[r + 2] = v[2];
     }
   }
@@ -179,8 +198,10 @@ template <typename C> __device__ inline void store3_nt(C* f, int64_t i, int64_t 
   st_nt(f + i + 2 * cs, v[2]);
 }
 
-// Sampling of step n+1 by k_yee2: up to two boxes; samp[b] already points at
-// the ring slot of that step, or is null when the step is not sampled.
+// Sampling by the two-step kernels: up to two boxes; samp[b] is the box's
+// ring (slot 0), null when the box has no points.  The kernels take the ring
+// slot of step n+1 (every face of their own cells; < 0: not sampled) and of
+// step n+2 (x faces only; < 0: left to the gather).
 template <typename T> struct Yee2Samples {
   BoxGeom g[2];
   cplx<T>* samp[2];
@@ -206,7 +227,7 @@ template <typename T> struct Yee2Samples {
 template <typename T, bool PH, bool PML, int kT2W>
 __device__ __forceinline__ void yee2_tile(const StepArgs<T>& a, cplx<T>* __restrict__ eb, cplx<T>* __restrict__ hb,
                                           cplx<T>* __restrict__ db, const int* __restrict__ chunks, int bx, int by,
-                                          int bz, const Yee2Samples<T>& smp, int slot, int xbase) {
+                                          int bz, const Yee2Samples<T>& smp, int slot, int slot2, int xbase) {
   using C = cplx<T>;
   constexpr int kT2R = Yee2Shape<T, PML, kT2W>::RY, kT2OX = Yee2Shape<T, PML, kT2W>::OX,
                 kT2OY = Yee2Shape<T, PML, kT2W>::OY;
@@ -370,6 +391,11 @@ __device__ __forceinline__ void yee2_tile(const StepArgs<T>& a, cplx<T>* __restr
           a.dn[auxi(z - 1, 1)] = d2n[1];
           a.dn[auxi(z - 1, 2)] = d2n[2];
         }
+        if (slot2 >= 0 && (fbits & kXFaceBits))
+          for (int b = 0; b < smp.nbox; ++b)
+            if (smp.samp[b])
+              sample_xface(smp.g[b], smp.samp[b] + (int64_t)slot2 * smp.g[b].ntot, (fbits >> (5 * b)) & 31u, ys,
+                           z - 1, 0, e2n);
       }
       if (z >= z0 + 2) {
         const int64_t ih = col + sxy * (z - 2);
@@ -382,6 +408,11 @@ __device__ __forceinline__ void yee2_tile(const StepArgs<T>& a, cplx<T>* __restr
         }
         if (xs == nx) store3_phase(a.hn, ih - nx, cs, a.fxm, h2n);
         if (ys == ny) store3_phase(a.hn, ih - a.sx * ny, cs, a.fym, h2n);
+        if (slot2 >= 0 && (fbits & kXFaceBits))
+          for (int b = 0; b < smp.nbox; ++b)
+            if (smp.samp[b])
+              sample_xface(smp.g[b], smp.samp[b] + (int64_t)slot2 * smp.g[b].ntot, (fbits >> (5 * b)) & 31u, ys,
+                           z - 2, 3, h2n);
       }
       // step n+1 planes read by the single-step passes of the neighbouring chunks
       if (exp_below && z == z0) {
@@ -425,10 +456,10 @@ __device__ __forceinline__ void yee2_tile(const StepArgs<T>& a, cplx<T>* __restr
 template <typename T, bool PH, bool PML = false, int W = 64>
 __global__ __launch_bounds__((kT2X * Yee2Shape<T, PML, W>::R), (Yee2Shape<T, PML, W>::WAVES_PER_EU)) void k_yee2(
     StepArgs<T> a, cplx<T>* __restrict__ eb, cplx<T>* __restrict__ hb, cplx<T>* __restrict__ db,
-    const int* __restrict__ chunks, TileMap tm, Yee2Samples<T> smp) {
+    const int* __restrict__ chunks, TileMap tm, Yee2Samples<T> smp, int slot, int slot2) {
   int bx, by, bz;
   tile_of(tm, blockIdx.x, &bx, &by, &bz);
-  yee2_tile<T, PH, PML, W>(a, eb, hb, db, chunks, bx, by, bz, smp, 0, tm.x0);
+  yee2_tile<T, PH, PML, W>(a, eb, hb, db, chunks, bx, by, bz, smp, slot, slot2, tm.x0);
 }
 
 // Batched two-step pass: the chunks of many grids in one launch.  Entry i of
@@ -437,7 +468,8 @@ __global__ __launch_bounds__((kT2X * Yee2Shape<T, PML, W>::R), (Yee2Shape<T, PML
 // wg_start[i] .. wg_start[i+1]-1 are its tiles, a multiple of the XCD count
 // so that the tile order's XCD dealing holds within every grid (the padding
 // workgroups return at once).  The DFT samples of step n+1 go to ring slot
-// `slot` of the grid's boxes (no sample when slot < 0).
+// `slot` of the grid's boxes (no sample when slot < 0), the x faces of step
+// n+2 to slot2 (none when slot2 < 0).
 template <typename T> struct Batch2Inst {
   StepArgs<T> a;
   cplx<T>* eb;
@@ -451,7 +483,7 @@ template <typename T> struct Batch2Inst {
 
 template <typename T, bool PH, bool PML, int W>
 __global__ __launch_bounds__((kT2X * Yee2Shape<T, PML, W>::R), (Yee2Shape<T, PML, W>::WAVES_PER_EU)) void k_yee2_batch(
-    const Batch2Inst<T>* __restrict__ inst, const int* __restrict__ wg_start, int ninst, int slot) {
+    const Batch2Inst<T>* __restrict__ inst, const int* __restrict__ wg_start, int ninst, int slot, int slot2) {
   const int blk = blockIdx.x;
   const int i = batch_instance(wg_start, ninst, blk);
   const Batch2Inst<T>& I = inst[i];
@@ -459,7 +491,7 @@ __global__ __launch_bounds__((kT2X * Yee2Shape<T, PML, W>::R), (Yee2Shape<T, PML
   if (L >= I.tm.nblk) return;  // padding (no barrier has been reached)
   int bx, by, bz;
   tile_of(I.tm, L, &bx, &by, &bz);
-  yee2_tile<T, PH, PML, W>(I.a, I.eb, I.hb, I.db, I.chunks, bx, by, bz, I.smp, slot, I.tm.x0);
+  yee2_tile<T, PH, PML, W>(I.a, I.eb, I.hb, I.db, I.chunks, bx, by, bz, I.smp, slot, slot2, I.tm.x0);
 }
 
 // DFT samples of one step from a buffer that holds that step only on the

@@ -121,7 +121,8 @@ __device__ __forceinline__ void p_e_update(const StepArgs<float>& a, int64_t col
 template <bool PH>
 __device__ __forceinline__ void yee2p_tile(const StepArgs<float>& a, float2* __restrict__ eb, float2* __restrict__ hb,
                                            const int* __restrict__ chunks, int bx, int by, int bz,
-                                           const Yee2Samples<float>& smp, int slot, int x0, int ox, int sl, int ns) {
+                                           const Yee2Samples<float>& smp, int slot, int slot2, int x0, int ox, int sl,
+                                           int ns) {
   using C = float2;
   constexpr int NL = kP2R * 64;  // LDS entries per plane and component: rows x lanes of a row
   __shared__ P2 s1[2][2][NL];    // E1 comps 3, 1 of planes z (z & 1) and z-1
@@ -252,6 +253,14 @@ __device__ __forceinline__ void yee2p_tile(const StepArgs<float>& a, float2* __r
         store(a.en, ie, e2n[0]);
         store(a.en, ie + cs, e2n[1]);
         store(a.en, ie + 2 * cs, e2n[2]);
+        if (slot2 >= 0 && (fbits & (kXFaceBits | (kXFaceBits << 10))))
+          for (int b = 0; b < smp.nbox; ++b) {
+            if (!smp.samp[b]) continue;
+            C* const sp = smp.samp[b] + (int64_t)slot2 * smp.g[b].ntot;
+            const C va[3] = {e2n[0].a, e2n[1].a, e2n[2].a}, vb[3] = {e2n[0].b, e2n[1].b, e2n[2].b};
+            sample_xface(smp.g[b], sp, (fbits >> (5 * b)) & 31u, ys, z - 1, 0, va);
+            sample_xface(smp.g[b], sp, (fbits >> (10 + 5 * b)) & 31u, ys, z - 1, 0, vb);
+          }
       }
       if (z >= z0 + 2) {
         const int64_t ih = col + sxy * (z - 2);
@@ -268,6 +277,14 @@ __device__ __forceinline__ void yee2p_tile(const StepArgs<float>& a, float2* __r
           pst(a.hn, ig + cs, pmul(a.fym, h2n[1]));
           pst(a.hn, ig + 2 * cs, pmul(a.fym, h2n[2]));
         }
+        if (slot2 >= 0 && (fbits & (kXFaceBits | (kXFaceBits << 10))))
+          for (int b = 0; b < smp.nbox; ++b) {
+            if (!smp.samp[b]) continue;
+            C* const sp = smp.samp[b] + (int64_t)slot2 * smp.g[b].ntot;
+            const C va[3] = {h2n[0].a, h2n[1].a, h2n[2].a}, vb[3] = {h2n[0].b, h2n[1].b, h2n[2].b};
+            sample_xface(smp.g[b], sp, (fbits >> (5 * b)) & 31u, ys, z - 2, 3, va);
+            sample_xface(smp.g[b], sp, (fbits >> (10 + 5 * b)) & 31u, ys, z - 2, 3, vb);
+          }
       }
       // step n+1 planes read by the single-step passes of the neighbouring chunks
       if (exp_below && z == z0) {
@@ -331,7 +348,8 @@ __device__ __forceinline__ void yee2p_tile(const StepArgs<float>& a, float2* __r
 template <bool PH>
 __global__ __launch_bounds__(64 * kP2R, 3) void k_yee2p(StepArgs<float> a, float2* __restrict__ eb,
                                                         float2* __restrict__ hb, const int* __restrict__ chunks,
-                                                        P2Part p0, P2Part p1, int npad0, Yee2Samples<float> smp) {
+                                                        P2Part p0, P2Part p1, int npad0, Yee2Samples<float> smp,
+                                                        int slot, int slot2) {
   int L = blockIdx.x;
   const bool second = L >= npad0;
   if (!second && L >= p0.tm.nblk) return;  // padding (no barrier has been reached)
@@ -339,13 +357,14 @@ __global__ __launch_bounds__(64 * kP2R, 3) void k_yee2p(StepArgs<float> a, float
   const P2Part& P = second ? p1 : p0;
   int bx, by, bz;
   tile_of(P.tm, L, &bx, &by, &bz);
-  yee2p_tile<PH>(a, eb, hb, chunks, bx, by, bz, smp, 0, P.tm.x0, P.ox, P.sl, P.ns);
+  yee2p_tile<PH>(a, eb, hb, chunks, bx, by, bz, smp, slot, slot2, P.tm.x0, P.ox, P.sl, P.ns);
 }
 
 // the batched form (Batch2Inst tables, as k_yee2_batch; one entry per part)
 template <bool PH>
 __global__ __launch_bounds__(64 * kP2R, 3) void k_yee2p_batch(const Batch2Inst<float>* __restrict__ inst,
-                                                              const int* __restrict__ wg_start, int ninst, int slot) {
+                                                              const int* __restrict__ wg_start, int ninst, int slot,
+                                                              int slot2) {
   const int blk = blockIdx.x;
   const int i = batch_instance(wg_start, ninst, blk);
   const Batch2Inst<float>& I = inst[i];
@@ -353,7 +372,7 @@ __global__ __launch_bounds__(64 * kP2R, 3) void k_yee2p_batch(const Batch2Inst<f
   if (L >= I.tm.nblk) return;  // padding (no barrier has been reached)
   int bx, by, bz;
   tile_of(I.tm, L, &bx, &by, &bz);
-  yee2p_tile<PH>(I.a, I.eb, I.hb, I.chunks, bx, by, bz, I.smp, slot, I.tm.x0, I.ox, I.sl, I.ns);
+  yee2p_tile<PH>(I.a, I.eb, I.hb, I.chunks, bx, by, bz, I.smp, slot, slot2, I.tm.x0, I.ox, I.sl, I.ns);
 }
 
 }  // namespace iris
