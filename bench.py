@@ -355,12 +355,33 @@ def run_angles(precision=4, steps=ANGLE_STEPS, n=ANGLES_TIMED):
         eng.sync()
     # both sides include each engine's creation (the gradient's is inside flux())
     t_fwd = 0.0
+    j_first = None
     for th in thetas:
         t0 = time.perf_counter()
         with Engine(oblique(s, th), precision=precision) as eng:
             eng.run_to(p.nt)
-            eng.flux(1)
+            j = eng.flux(1)
         t_fwd += time.perf_counter() - t0
+        if j_first is None:
+            j_first = j
+    # the first angle's forward T/R against the same run with FP64 fields
+    # (whose T/R match the reference's FP64 program to 1e-10 on the goldens),
+    # on the wavelengths whose flux has arrived (J5+J6 above 1e-3 of its largest)
+    tr_err = None
+    if precision == 4:
+        with Engine(oblique(s, thetas[0]), precision=8) as eng:
+            eng.run_to(p.nt)
+            j64 = eng.flux(1)
+        den64 = j64[:, 4] + j64[:, 5]
+        live = np.abs(den64) > 1e-3 * np.abs(den64).max()
+        den32 = j_first[:, 4] + j_first[:, 5]
+        dR = np.abs(j_first[:, 4] / den32 - j64[:, 4] / den64)[live]
+        dT = np.abs(j_first[:, 5] / den32 - j64[:, 5] / den64)[live]
+        tr_err = {"angle_deg": round(float(thetas[0]), 3), "wavelengths": int(live.sum()),
+                  "max_abs_dR": float(dR.max()) if live.any() else None,
+                  "max_abs_dT": float(dT.max()) if live.any() else None}
+        log(f"angles: FP32 against FP64 T/R at {thetas[0]:.1f} deg: max |dR| {tr_err['max_abs_dR']:.2e}, "
+            f"max |dT| {tr_err['max_abs_dT']:.2e} over {tr_err['wavelengths']} wavelengths")
     leaf = torch.tensor(eps, requires_grad=True)
     t_grad, total, ck = 0.0, 0.0, None
     for th in thetas:
@@ -379,7 +400,8 @@ def run_angles(precision=4, steps=ANGLE_STEPS, n=ANGLES_TIMED):
            "recorded_share": round(share, 4), "record_bytes_per_angle": rec_bytes,
            "fields": "fp32" if precision == 4 else "fp64",
            "forward_s": round(t_fwd, 4), "gradient_s": round(t_grad, 4), "ratio": round(t_grad / t_fwd, 3),
-           "objective": total, "dT_deps_max_abs": float(np.abs(g).max())}
+           "objective": total, "dT_deps_max_abs": float(np.abs(g).max()),
+           "fp32_vs_fp64_tr": tr_err}
     log(f"angles: forwards {t_fwd:.2f} s, gradients {t_grad:.2f} s, ratio {t_grad / t_fwd:.2f}")
     return out
 
