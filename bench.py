@@ -380,8 +380,9 @@ def run_angles(precision=4, steps=ANGLE_STEPS, n=ANGLES_TIMED):
         tr_err = {"angle_deg": round(float(thetas[0]), 3), "wavelengths": int(live.sum()),
                   "max_abs_dR": float(dR.max()) if live.any() else None,
                   "max_abs_dT": float(dT.max()) if live.any() else None}
-        log(f"angles: FP32 against FP64 T/R at {thetas[0]:.1f} deg: max |dR| {tr_err['max_abs_dR']:.2e}, "
-            f"max |dT| {tr_err['max_abs_dT']:.2e} over {tr_err['wavelengths']} wavelengths")
+        if live.any():
+            log(f"angles: FP32 against FP64 T/R at {thetas[0]:.1f} deg: max |dR| {tr_err['max_abs_dR']:.2e}, "
+                f"max |dT| {tr_err['max_abs_dT']:.2e} over {tr_err['wavelengths']} wavelengths")
     leaf = torch.tensor(eps, requires_grad=True)
     t_grad, total, ck = 0.0, 0.0, None
     for th in thetas:
