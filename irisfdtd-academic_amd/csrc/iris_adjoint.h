@@ -470,8 +470,9 @@ __global__ __launch_bounds__(kFTX * (TY + 1), 1) void k_adj_gen(
   int tbx, tby, tbz;
   tile_of(tm, blockIdx.x, &tbx, &tby, &tbz);
   const int x0 = tbx * kFTX + 1, y0 = tby * TY + 1;
-  const int z0 = chunks[tbz];
-  const int z1 = min(z0 + a.zchunk - 1, nz);
+  const int cw = chunks[tbz];
+  const int z0 = cw & kChunkZMask, zl = (cw >> kChunkLenShift) & kChunkLenMask;
+  const int z1 = min(z0 + (zl ? zl : a.zchunk) - 1, nz);
   const bool halo = wy == TY;
   const int xa = x0 + lane, ya = y0 + wy;
   bool va = xa <= nx + 1 && ya <= ny + 1 && !(xa == nx + 1 && ya == ny + 1);

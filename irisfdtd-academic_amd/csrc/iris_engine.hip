@@ -1922,8 +1922,12 @@ int adjoint_double(iris_handle* h, int n, int nobox, int face_on, const void* re
 // hi-1..hf and off the TF/SF planes; with face_on (bit 2j+s: face s of pair j
 // of box nobox carries weight) also off the planes holding weighted face
 // points (the z-faces, or the box's whole z range when an x or y face is
-// weighted).  The other chunks are lean (planes z0-1..z1+1 plain, z1 < nz)
-// or general.
+// weighted).  Planes are classified one by one: runs of planes that may be
+// two-step outputs become near-equal two-step chunks of up to zc2 planes;
+// the planes between take single steps in chunks of up to zc planes that
+// carry their own lengths, lean (planes z0-1..z1+1 plain, z1 < nz) or
+// general, so the general passes (k_yee_fused<T, true>, k_adj_gen) cover
+// only the planes that need them.
 template <typename T>
 int build_lists(iris_handle* h, ListSet& out, int nobox, int face_on) {
   const iris_params& p = h->p;
@@ -1950,50 +1954,84 @@ int build_lists(iris_handle* h, ListSet& out, int nobox, int face_on) {
     return true;
   };
   const bool shape = h->two_step && two_step_fits(p, t2geom<T>(h->t2w, false));
-  const int zc = h->zchunk;
-  std::vector<char> dbl;
+  // output planes of two-step chunks: every plane z-2 .. z+2 eligible
+  std::vector<char> two((size_t)p.nz + 2, 0);
   int ndbl = 0;
-  for (int z0 = 1; z0 <= p.nz; z0 += zc) {
-    const int z1 = std::min(z0 + zc - 1, p.nz);
-    bool ok = shape && z1 + 2 <= p.nz && z1 - z0 + 1 == zc;
-    for (int z = z0 - 2; ok && z <= z1 + 2; ++z) ok = eligible(z);
-    dbl.push_back(ok ? 1 : 0);
-    ndbl += ok;
+  for (int z = 3; shape && z + 2 <= p.nz; ++z) {
+    bool ok = true;
+    for (int q = z - 2; ok && q <= z + 2; ++q) ok = eligible(q);
+    two[z] = ok;
+  }
+  // runs of fewer than kMinRun2 such planes take single steps
+  constexpr int kMinRun2 = 4;
+  for (int z = 1; z <= p.nz; ++z) {
+    if (!two[z]) continue;
+    int e = z;
+    while (e + 1 <= p.nz && two[e + 1]) ++e;
+    if (e - z + 1 < kMinRun2)
+      for (int q = z; q <= e; ++q) two[q] = 0;
+    else
+      ++ndbl;
+    z = e;
   }
   if (ndbl == 0) return 0;
+  // n near-equal pieces of at most len planes over a .. b
+  auto cut = [](int a, int b, int len, auto&& emit) {
+    const int n = (b - a + 1 + len - 1) / len, base = (b - a + 1) / n, extra = (b - a + 1) % n;
+    for (int k = 0, c0 = a; k < n; ++k) {
+      const int l = base + (k < extra ? 1 : 0);
+      emit(c0, l);
+      c0 += l;
+    }
+  };
   const int zc2 = h->zc2;
+  // single-step chunks carry their length in the chunk word (at most
+  // kChunkLenMask planes)
+  const int zc = std::min(h->zchunk, kChunkLenMask);
   std::vector<int> c2, rlean, rgen;
   std::vector<unsigned char> zm((size_t)p.nz + 2, 0);
-  const int nch = (int)dbl.size();
-  for (int k = 0; k < nch;) {
-    const int z0 = 1 + k * zc;
-    if (!dbl[k]) {
-      const int z1 = std::min(z0 + zc - 1, p.nz);
-      bool lean = z1 < p.nz;
-      for (int z = z0 - 1; lean && z <= z1 + 1; ++z) lean = is_bulk_plane(z);
-      (lean ? rlean : rgen).push_back(z0);
-      ++k;
-      continue;
+  for (int z = 1; z <= p.nz;) {
+    int e = z;
+    while (e + 1 <= p.nz && two[e + 1] == two[z]) ++e;
+    if (two[z]) {
+      // a run of two-step planes: near-equal chunks of up to zc2 planes,
+      // the first exporting below, the last above
+      std::vector<int> run;
+      cut(z, e, zc2, [&](int c0, int l) { run.push_back(c0 | ((l - 1) << kChunkLenShift)); });
+      run.front() |= kChunkExportBelow;
+      run.back() |= kChunkExportAbove;
+      c2.insert(c2.end(), run.begin(), run.end());
+      for (int q = z; q <= e; ++q) zm[q] = 1;
+    } else {
+      // single-step planes: lean where the planes a chunk reads or writes
+      // are all bulk (and it is not the top plane), general elsewhere;
+      // lean stretches shorter than kMinRun2 planes go with the general
+      auto lean_ok = [&](int q) { return q < p.nz && is_bulk_plane(q - 1) && is_bulk_plane(q) && is_bulk_plane(q + 1); };
+      std::vector<char> ln((size_t)(e - z + 1));
+      for (int q = z; q <= e; ++q) ln[q - z] = lean_ok(q);
+      for (int q = z; q <= e;) {
+        int f = q;
+        while (f + 1 <= e && ln[f + 1 - z] == ln[q - z]) ++f;
+        if (ln[q - z] && f - q + 1 < kMinRun2)
+          for (int r = q; r <= f; ++r) ln[r - z] = 0;
+        q = f + 1;
+      }
+      for (int q = z; q <= e;) {
+        int f = q;
+        while (f + 1 <= e && ln[f + 1 - z] == ln[q - z]) ++f;
+        auto& dst = ln[q - z] ? rlean : rgen;
+        cut(q, f, zc, [&](int c0, int l) { dst.push_back(c0 | (l << kChunkLenShift)); });
+        q = f + 1;
+      }
     }
-    int k1 = k;
-    while (k1 + 1 < nch && dbl[k1 + 1]) ++k1;
-    const int r0 = z0, r1 = 1 + (k1 + 1) * zc - 1;
-    for (int c0 = r0; c0 <= r1; c0 += zc2) {
-      const int len = std::min(zc2, r1 - c0 + 1);
-      int v = c0 | ((len - 1) << kChunkLenShift);
-      if (c0 == r0) v |= kChunkExportBelow;
-      if (c0 + len - 1 == r1) v |= kChunkExportAbove;
-      c2.push_back(v);
-    }
-    for (int z = r0; z <= r1; ++z) zm[z] = 1;
-    k = k1 + 1;
+    z = e + 1;
   }
   std::vector<int> rest(rlean);
   rest.insert(rest.end(), rgen.begin(), rgen.end());
   int rc;
   if (!out.chunks2) {
-    // sized for the most chunks any face mask can give
-    const size_t cap = (size_t)nch + 1;
+    // sized for the most chunks any face mask can give (one a plane)
+    const size_t cap = (size_t)p.nz + 1;
     if ((rc = adj_alloc(h, (void**)&out.chunks2, sizeof(int) * cap)) ||
         (rc = adj_alloc(h, (void**)&out.rest, sizeof(int) * cap)) || (rc = adj_alloc(h, (void**)&out.zmask, zm.size())))
       return rc;

@@ -397,8 +397,9 @@ __device__ inline void e_item(const StepArgs<T>& a, int xs, int ys, int zs, cplx
 // XCDs.  Tiles past the last whole round of runs, and run = 0, keep the
 // identity order t = L.
 // chunks[] word of the two-step pass: first plane (bits 0-19), planes - 1
-// (bits 20-27), export flags (bits 28-29).  A single-step chunk is its first
-// plane; it spans a.zchunk planes (up to nz).
+// (bits 20-27), export flags (bits 28-29).  A single-step chunk word is its
+// first plane (bits 0-19) and its planes (bits 20-27; 0: a.zchunk planes),
+// up to nz.
 constexpr int kChunkZMask = (1 << 20) - 1;
 constexpr int kChunkLenShift = 20, kChunkLenMask = 0xff;
 constexpr int kChunkExportBelow = 1 << 28;
@@ -450,8 +451,9 @@ __device__ __forceinline__ void yee_fused_tile(const StepArgs<T>& a, const TfsfC
   int tbx, tby, tbz;
   tile_of(tm, blk, &tbx, &tby, &tbz);
   const int x0 = tbx * kFTX + 1, y0 = tby * kFTY + 1;
-  const int z0 = chunks[tbz];
-  const int z1 = min(z0 + a.zchunk - 1, nz);
+  const int cw = chunks[tbz];
+  const int z0 = cw & kChunkZMask, zl = (cw >> kChunkLenShift) & kChunkLenMask;
+  const int z1 = min(z0 + (zl ? zl : a.zchunk) - 1, nz);
   const bool halo = wy == kFTY;
   // item A: own cell, or a Bloch image inside a partial tile, or (halo wave)
   // the +y neighbour row; item B (halo wave lanes < kFTY): +x neighbour column
