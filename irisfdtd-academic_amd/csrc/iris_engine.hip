@@ -197,6 +197,7 @@ struct iris_handle {
   // columns from x0n + 1
   int p2ox = 0, p2sl = 0, p2ns = 0, p2oxn = 0, p2sln = 0, p2nsn = 0, tx2n = 0, ty2n = 0, x0n = 0;
   int nt2 = 0;              // two-step pass: streaming stores of its outputs
+  int fold_persist = 1;     // FP32 fold: persistent workgroups when nof <= 128 (0: one workgroup per tile block)
   PassConfig pass;          // the layout the lists were built for
   int64_t pass_gen = 0;     // bumped by every rebuild of the pass lists
   // single steps of the two-step layout's other chunks when they come in
@@ -764,9 +765,19 @@ int dft_flush(iris_handle* h) {
       // each staged group of phases)
       int64_t tiles = 0;
       for (int q = 0; q < 3; ++q) tiles += (B.g.npts[q] + 15) / 16;
-      hipLaunchKernelGGL((k_dft_fold_mfma<dft_slots<T>(), kFoldWaves, kFoldGroups>),
-                         dim3((unsigned)((tiles + kFoldWaves - 1) / kFoldWaves)), dim3(64 * kFoldWaves), 0, h->stream,
-                         B.g, (const float2*)B.samp, nk, (const double2*)h->ph, nof, B.emg);
+      const int64_t nwg = (tiles + kFoldWaves - 1) / kFoldWaves;
+      if (h->fold_persist && (nof + 15) / 16 <= kFoldGroups) {
+        // all wavelength groups staged at once: one persistent workgroup a
+        // CU (its 128 KB of LDS allow no second)
+        const unsigned grid = (unsigned)std::min<int64_t>(nwg, (int64_t)h->ncu);
+        hipLaunchKernelGGL((k_dft_fold_mfma_p<dft_slots<T>(), kFoldWaves, kFoldGroups>), dim3(grid),
+                           dim3(64 * kFoldWaves), 0, h->stream, B.g, (const float2*)B.samp, nk,
+                           (const double2*)h->ph, nof, B.emg, tiles);
+      } else {
+        hipLaunchKernelGGL((k_dft_fold_mfma<dft_slots<T>(), kFoldWaves, kFoldGroups>), dim3((unsigned)nwg),
+                           dim3(64 * kFoldWaves), 0, h->stream, B.g, (const float2*)B.samp, nk,
+                           (const double2*)h->ph, nof, B.emg);
+      }
     }
   }
   HIP_TRY(h, hipGetLastError());
@@ -3029,6 +3040,8 @@ int iris_create(const iris_params* p, const double* eps_inv, const double* fpzmn
     const char* genv = std::getenv("IRIS_GRAPH");
     const char* tool = std::getenv("ROCP_TOOL_LIBRARIES");
     h->graph_on = genv ? genv[0] != '0' : !(tool && tool[0]);
+    const char* fp = std::getenv("IRIS_FOLD_PERSIST");
+    if (fp && *fp) h->fold_persist = std::max(0, std::atoi(fp));
   }
   (void)hipGetDevice(&h->device);
   if (hipDeviceGetAttribute(&h->ncu, hipDeviceAttributeMultiprocessorCount, h->device) != hipSuccess || h->ncu < 1)

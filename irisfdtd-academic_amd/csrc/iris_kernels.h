@@ -303,6 +303,99 @@ __global__ __launch_bounds__(64 * WAVES) void k_dft_fold_mfma(BoxGeom g, const f
   }
 }
 
+// The same fold with every wavelength group staged at once (nof <= 16 GMAX)
+// and persistent workgroups: a workgroup stages the phases once, then each
+// of its waves takes point tiles wave + WAVES * (block + grid * i) on its
+// own, with no barrier after the staging, so one wave's sample and
+// accumulator loads overlap the other waves' matrix products across tiles as
+// well as across groups.  Every tile's operations, and so its results, are
+// those of k_dft_fold_mfma.
+template <int KMAX, int WAVES, int GMAX>
+__global__ __launch_bounds__(64 * WAVES) void k_dft_fold_mfma_p(BoxGeom g, const float2* __restrict__ samp, int nk,
+                                                         const double2* __restrict__ ph, int nof,
+                                                         double2* __restrict__ emg, int64_t ntiles) {
+  static_assert((KMAX & (KMAX - 1)) == 0, "KMAX is a power of two");
+  __shared__ double2 sph[GMAX][KMAX][16];
+  // the wave index in a scalar register: the tile walk stays scalar
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int col = lane & 15, grp = lane >> 4;
+  const int ng = (nof + 15) / 16;  // <= GMAX (host check)
+  for (int t = threadIdx.x; t < ng * KMAX * 16; t += blockDim.x) {
+    const int gi = t / (KMAX * 16), k = (t >> 4) & (KMAX - 1), u = t & 15;
+    const int c = gi * 16 + u;
+    sph[gi][k][u] = (c < nof) ? ph[k * nof + c] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  for (int64_t tile = (int64_t)blockIdx.x * WAVES + wave; tile < ntiles; tile += (int64_t)gridDim.x * WAVES) {
+    // pair block j and the tile of 16 point-components in it
+    int64_t tl = tile;
+    int j = 0;
+    for (; j < 2; ++j) {
+      const int64_t tq = (g.npts[j] + 15) / 16;
+      if (tl < tq) break;
+      tl -= tq;
+    }
+    const int64_t np = g.npts[j];
+    const int64_t p0 = tl * 16;
+    const bool colok = p0 + col < np;
+    const int64_t ic = colok ? g.start[j] + p0 + col : 0;
+    float sv[KMAX / 2];
+    // slots k = 2q + (grp >> 1), walked by a pointer (no per-slot offsets held)
+    const float2* sp = samp + ic + (int64_t)(grp >> 1) * g.ntot;
+    const int64_t sstep = 2 * g.ntot;
+#pragma unroll
+    for (int q = 0; q < KMAX / 2; ++q) {
+      const int k = 2 * q + (grp >> 1);
+      float v = 0.f;
+      if (colok && k < nk) {
+        const float2 f = *sp;
+        v = (grp & 1) ? f.y : f.x;
+      }
+      sv[q] = v;
+      sp += sstep;
+    }
+    double2* const cbase = emg + g.start[j] * (int64_t)nof + p0 + col;
+    auto load = [&](int c0, double2 v[4]) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int w = c0 + 4 * r + grp;
+        v[r] = (colok && w < nof) ? cbase[(int64_t)w * np] : make_double2(0.0, 0.0);
+      }
+    };
+    int gi = wave % ng;
+    double2 nx[4];
+    load(gi * 16, nx);
+    for (int s = 0; s < ng; ++s) {
+      const int c0 = gi * 16;
+      iris_d4 cr, ci;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        cr[r] = nx[r].x;
+        ci[r] = nx[r].y;
+      }
+      const int gnext = gi + 1 == ng ? 0 : gi + 1;
+      if (s + 1 < ng) load(gnext * 16, nx);
+#pragma unroll
+      for (int q = 0; q < KMAX / 2; ++q) {
+        const double2 P = sph[gi][2 * q + (grp >> 1)][col];
+        const double br = (grp & 1) ? -P.y : P.x;
+        const double bi = (grp & 1) ? P.x : P.y;
+        float sf = sv[q];
+        asm volatile("" : "+v"(sf));  // keeps the sample in FP32 between groups (converted here, per use)
+        const double sq = (double)sf;
+        cr = __builtin_amdgcn_mfma_f64_16x16x4f64(br, sq, cr, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f64_16x16x4f64(bi, sq, ci, 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int w = c0 + 4 * r + grp;
+        if (colok && w < nof) cbase[(int64_t)w * np] = make_double2(cr[r], ci[r]);
+      }
+      gi = gnext;
+    }
+  }
+}
+
 // One field component on one plane: out[a + na*b] = f(x, y, z) with the
 // fixed coordinate `index` on `axis` (1: x, 2: y, 3: z) and the two free
 // coordinates a0 + a, b0 + b in (x, y, z) order.

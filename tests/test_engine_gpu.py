@@ -553,6 +553,27 @@ def test_fp32_dft_many_wavelengths_against_fp64(tmp_path, nof):
     assert np.abs(out[8][0]).max() > 0
 
 
+@pytest.mark.parametrize("nof", [5, 37, 128])
+def test_fp32_fold_persistent_is_bitwise_per_tile(tmp_path, monkeypatch, nof):
+    """The persistent matrix-core fold (every wavelength group staged once,
+    waves walking their own point tiles) gives the per-tile fold's
+    accumulators bit for bit: one partial group, a partial last group, and
+    all 8 groups staged; 150 sampled steps end in a partial fold."""
+    _announce(f"fp32 fold persistent nof={nof}")
+    s = load_input(write_input(tmp_path, inputs.render(
+        ixmax=70, iymax=21, izmax=96, pmlzi=10, pmlzf=86, hi=44, dh=10, dx=8, eps_bbox=2.25, tfsf_o=20,
+        box=(3, 66, 2, 20, 15, 80), nof=nof, dlambda=1500.0 / nof, lno=450.0, nt=150, no_downloads=150)))
+    out = {}
+    for v in ("0", "1"):
+        monkeypatch.setenv("IRIS_FOLD_PERSIST", v)
+        with Engine(s, precision=4) as eng:
+            eng.run_to(s.params.nt)
+            out[v] = [eng.flux(1)] + [eng.dft(1, j) for j in (1, 2, 3)]
+    for k, (a, b) in enumerate(zip(out["0"], out["1"])):
+        assert np.array_equal(a, b), k
+    assert np.abs(out["0"][0]).max() > 0
+
+
 def test_fp32_tr_with_two_step_pass(tmp_path):
     """FP32 fields (two-step pass engaged) against FP64 fields on a periodic
     slit array: T and R within 1e-5."""
