@@ -96,6 +96,9 @@ template <typename T> struct StepArgs {
   C fxm, fym, fzm;  // exp(-i ak)
   int zchunk;       // planes per thread in the z-marching passes
   int nt_out = 0;   // two-step pass: stream its E and H of step n+2 past the L2 (1) or not (0)
+  // two cells a lane: planes rec_z0..rec_z1 of each chunk's own planes also
+  // get H of step n+1 into the second buffers (a gradient run's records)
+  int rec_z0 = 1, rec_z1 = 0;
   // offset of component c of UPML slab k at in-slab index pidx
   __device__ int64_t aux(int k, int c, int64_t pidx) const {
     return (k ? pb1 + c * ps1 : c * ps0) + pidx;

@@ -96,12 +96,15 @@ struct PassLists {
   const unsigned char* zmask = nullptr;  // 1 on the planes of two-step chunks
   const RestGroup* groups = nullptr;      // rest chunks by length (null: one group of zchunk planes)
   int ngroups = 0;
+  int rz0 = 1, rz1 = 0;                   // record planes whose H of step n+1 the two-step pass exports
 };
 
 // device copy of the lists of one two-step classification
 struct ListSet {
   bool on = false;
   int key = -1;
+  int gen = -1;          // pass_gen of the layout the lists were made for
+  int rz0 = 1, rz1 = 0;  // record planes inside two-step chunks (their H of step n+1 exported)
   int* chunks2 = nullptr;
   int n2 = 0;
   int* rest = nullptr;
@@ -110,6 +113,7 @@ struct ListSet {
   PassLists view() const {
     PassLists L;
     L.chunks2 = chunks2; L.n2 = n2; L.rest = rest; L.nlean = nlean; L.ngen = ngen; L.zmask = zmask;
+    L.rz0 = rz0; L.rz1 = rz1;
     return L;
   }
 };
@@ -648,7 +652,9 @@ int launch_double(iris_handle* h, int it, const PassLists& L, bool dft) {
   }
   if (L.n2) {
     ProfScope ps(h, IRIS_K_YEE2);
-    const StepArgs<T> a2 = step_args_on<T>(h, h->fe, h->fh, h->ke, h->kh, h->d, h->b, h->dn, h->bn);
+    StepArgs<T> a2 = step_args_on<T>(h, h->fe, h->fh, h->ke, h->kh, h->d, h->b, h->dn, h->bn);
+    a2.rec_z0 = L.rz0;
+    a2.rec_z1 = L.rz1;
     const TileMap tm{h->tx2, h->ty2, h->tx2 * h->ty2 * L.n2, h->tx2 * h->xcd_rows2};
     if (h->bloch)
       launch_yee2<T, true, false>(h, a2, (C*)h->ge, (C*)h->gh, nullptr, L.chunks2, tm, L.n2, smp, slot, slot2);
@@ -1930,7 +1936,9 @@ int adjoint_double(iris_handle* h, int n, int nobox, int face_on, const void* re
 
 // Two-step lists of the gradient runs.  Two-step chunks need every plane
 // z0-2 .. z1+2 between the UPML slabs, off the structure and record planes
-// hi-1..hf and off the TF/SF planes; with face_on (bit 2j+s: face s of pair j
+// hi-1..hf (except in the record lists with two cells a lane, rec: that
+// kernel exports H of step n+1 on those planes into the second buffers,
+// where the records are taken) and off the TF/SF planes; with face_on (bit 2j+s: face s of pair j
 // of box nobox carries weight) also off the planes holding weighted face
 // points (the z-faces, or the box's whole z range when an x or y face is
 // weighted).  Planes are classified one by one: runs of planes that may be
@@ -1940,18 +1948,24 @@ int adjoint_double(iris_handle* h, int n, int nobox, int face_on, const void* re
 // general, so the general passes (k_yee_fused<T, true>, k_adj_gen) cover
 // only the planes that need them.
 template <typename T>
-int build_lists(iris_handle* h, ListSet& out, int nobox, int face_on) {
+int build_lists(iris_handle* h, ListSet& out, int nobox, int face_on, bool rec = false) {
   const iris_params& p = h->p;
   const int key = nobox * 64 + face_on;
-  if (out.key == key) return 0;
+  if (out.key == key && out.gen == h->pass_gen) return 0;
   out.key = key;
+  out.gen = h->pass_gen;
   out.on = false;
   out.n2 = out.nlean = out.ngen = 0;
+  // record lists with two cells a lane: the record planes may be two-step
+  // planes, whose kernel then exports their H of step n+1
+  const bool rec_two = rec && sizeof(T) == 4 && h->two_step && h->t2w == 128;
+  out.rz0 = rec_two ? p.hi - 1 : 1;
+  out.rz1 = rec_two ? p.hf : 0;
   const BoxGeom& g = h->box[nobox - 1].g;
   const bool side = (face_on & 0xf) != 0;
   auto eligible = [&](int z) {
     if (z < 1 || z <= p.pmlzi || z >= p.pmlzf) return false;
-    if (z >= p.hi - 1 && z <= p.hf) return false;
+    if (!rec_two && z >= p.hi - 1 && z <= p.hf) return false;
     if (p.em_source && (z == p.iz_min - 1 || z == p.iz_min)) return false;
     if (side && z >= g.bzi && z <= g.bzf) return false;
     if (((face_on >> 4) & 1) && z == g.bzi) return false;
@@ -2075,6 +2089,7 @@ int run_tail(iris_handle* h, int it_first, int it_last) {
   };
   int rc = rec(h->fh, it - 1);
   if (rc) return rc;
+  if (h->rl.key >= 0 && (rc = build_lists<T>(h, h->rl, 1, 0, true))) return rc;  // a rebuilt layout since
   const PassLists L = h->rl.view();
   while (h->rl.on && it_last - it + 1 >= 3) {
     if ((rc = launch_double<T>(h, it, L, true))) return rc;
@@ -2178,6 +2193,7 @@ int adjoint_impl(iris_handle* h, int nobox, const double* weights, double* grad_
   };
   // the recorded tail needs no recomputation
   if (h->tail_n && (rc = sweep(h->tail_t0, nt, h->tail_rec))) return rc;
+  if (h->rl.key >= 0 && (rc = build_lists<T>(h, h->rl, 1, 0, true))) return rc;  // a rebuilt layout since
   const PassLists RL = h->rl.view();
   for (int k = nhead - 1; k >= 0; --k) {
     const int s0 = k * seg, s1 = std::min(h->tail_t0, s0 + seg);
@@ -3153,7 +3169,7 @@ int iris_checkpoint(iris_handle* h, int32_t segment, int64_t record_bytes) {
   h->tail_n = (int)nrec;
   h->ck_seg = segment;
   if (h->two_step) {
-    const int rc = h->prec == 8 ? build_lists<double>(h, h->rl, 1, 0) : build_lists<float>(h, h->rl, 1, 0);
+    const int rc = h->prec == 8 ? build_lists<double>(h, h->rl, 1, 0, true) : build_lists<float>(h, h->rl, 1, 0, true);
     if (rc) return rc;
   }
   return 0;

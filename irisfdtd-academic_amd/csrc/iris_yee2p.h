@@ -292,8 +292,11 @@ __device__ __forceinline__ void yee2p_tile(const StepArgs<float>& a, float2* __r
         pst(eb, iz + cs, e1[1]);
         pst(eb, iz + 2 * cs, e1[2]);
       }
-      if (exp_below && z - 1 == z0) {
-        const int64_t ih = col + sxy * z0;
+      // H of step n+1 for the single steps of the chunk below, and on the
+      // record planes of a gradient run's recording
+      if ((exp_below && z - 1 == z0) ||
+          (z - 1 >= z0 && z - 1 <= z1 && z - 1 >= a.rec_z0 && z - 1 <= a.rec_z1)) {
+        const int64_t ih = col + sxy * (z - 1);
         pst(hb, ih, h1n[0]);
         pst(hb, ih + cs, h1n[1]);
         pst(hb, ih + 2 * cs, h1n[2]);
