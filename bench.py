@@ -716,9 +716,9 @@ def kernel_class(name):
 def precision_of(name):
     """Field precision from the kernel's template arguments (None for the
     kernels that have none, such as the phase table; k_yee2p holds FP32
-    fields only, and k_dft_fold_mfma folds FP32 samples)."""
+    fields only, and k_dft_fold_mfma and k_dft_fold_mfma_p fold FP32 samples)."""
     tmpl = name.split("(")[0]
-    if "<float" in tmpl or "k_yee2p<" in tmpl or "k_dft_fold_mfma<" in tmpl:
+    if "<float" in tmpl or "k_yee2p<" in tmpl or "k_dft_fold_mfma<" in tmpl or "k_dft_fold_mfma_p<" in tmpl:
         return 4
     if "<double" in tmpl:
         return 8
