@@ -2014,6 +2014,7 @@ int build_lists(iris_handle* h, ListSet& out, int nobox, int face_on, bool rec =
   // kChunkLenMask planes)
   const int zc = std::min(h->zchunk, kChunkLenMask);
   std::vector<int> c2, rlean, rgen;
+  std::vector<std::pair<int, int>> slean, sgen;  // single-step stretches (first, last plane)
   std::vector<unsigned char> zm((size_t)p.nz + 2, 0);
   for (int z = 1; z <= p.nz;) {
     int e = z;
@@ -2044,13 +2045,32 @@ int build_lists(iris_handle* h, ListSet& out, int nobox, int face_on, bool rec =
       for (int q = z; q <= e;) {
         int f = q;
         while (f + 1 <= e && ln[f + 1 - z] == ln[q - z]) ++f;
-        auto& dst = ln[q - z] ? rlean : rgen;
-        cut(q, f, zc, [&](int c0, int l) { dst.push_back(c0 | (l << kChunkLenShift)); });
+        (ln[q - z] ? slean : sgen).push_back({q, f});
         q = f + 1;
       }
     }
     z = e + 1;
   }
+  // single-step stretches in chunks of up to zc planes.  The adjoint's
+  // general chunks (k_adj_gen) take shorter ones (down to 4 planes) where
+  // zc-plane chunks would give fewer than 8 workgroups per CU: they cover a
+  // few thin slabs (UPML, structure), which in zc-plane chunks leave most
+  // CUs idle (on 256 x 256 x 1024, 84 planes in 5 chunks of 128 tiles: 460
+  // against 369 us a step in 6-plane chunks).  The forward's general pass
+  // measured no faster in shorter chunks (177 against 188 us there).
+  auto emit = [&](const std::vector<std::pair<int, int>>& segs, std::vector<int>& dst, bool fill) {
+    const int64_t tiles = (int64_t)h->grd_fused.x * h->grd_fused.y, want = 8LL * h->ncu;
+    auto wgs = [&](int l) {
+      int64_t n = 0;
+      for (const auto& sg : segs) n += (sg.second - sg.first + l) / l;
+      return n * tiles;
+    };
+    int L = zc;
+    while (fill && L > 4 && wgs(L) < want) --L;
+    for (const auto& sg : segs) cut(sg.first, sg.second, L, [&](int c0, int l) { dst.push_back(c0 | (l << kChunkLenShift)); });
+  };
+  emit(slean, rlean, false);
+  emit(sgen, rgen, !rec);
   std::vector<int> rest(rlean);
   rest.insert(rest.end(), rgen.begin(), rgen.end());
   int rc;
