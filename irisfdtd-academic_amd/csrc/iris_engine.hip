@@ -107,12 +107,15 @@ struct ListSet {
   int rz0 = 1, rz1 = 0;  // record planes inside two-step chunks (their H of step n+1 exported)
   int* chunks2 = nullptr;
   int n2 = 0;
+  int* chunks2p = nullptr;  // two-step chunks of the UPML variant (record lists only)
+  int n2p = 0;
   int* rest = nullptr;
   int nlean = 0, ngen = 0;
   unsigned char* zmask = nullptr;
   PassLists view() const {
     PassLists L;
     L.chunks2 = chunks2; L.n2 = n2; L.rest = rest; L.nlean = nlean; L.ngen = ngen; L.zmask = zmask;
+    L.chunks2p = chunks2p; L.n2p = n2p;
     L.rz0 = rz0; L.rz1 = rz1;
     return L;
   }
@@ -1408,10 +1411,13 @@ int build_passes(iris_handle* h, const PassConfig& c) {
       kind[z] = plain ? 1 : 2;
       ++nout;
     }
-    // The UPML variant by default only where its launch gives every CU a
-    // workgroup: its slabs are a few dozen planes, and on small grids the
-    // few workgroups of its launch took longer than the single steps they
-    // replace (measured on 64 gratings of 32..128 x 32..128 x 768 cells).
+    // The UPML variant by default only where its launch gives at least every
+    // other CU a workgroup: its slabs are a few dozen planes, and on small
+    // grids the few workgroups of its launch took longer than the single
+    // steps they replace (measured on 64 gratings of 32..128 x 32..128 x 768
+    // cells).  With FP32 fields and a Bloch phase on n x n x 768 grids it
+    // measured 7% faster a step at n = 192 (140 workgroups) and 224 (176),
+    // 6% on 256 x 256 x 1024 (234), within 1% at n = 128 and 160 (70, 96).
     // c.upml = 0: never, 1: wherever it applies, -1: by that rule.
     bool upml_two = c.upml != 0;
     if (c.upml < 0) {
@@ -1424,7 +1430,7 @@ int build_passes(iris_handle* h, const PassConfig& c) {
         z = e;
       }
       const int64_t tiles = (int64_t)g2p.tx(p.nx) * g2p.ty(p.ny);
-      if (tiles * nch < h->ncu) upml_two = false;
+      if (2 * tiles * nch < h->ncu) upml_two = false;
     }
     if (!upml_two)
       for (int z = 1; z <= p.nz; ++z)
@@ -1956,7 +1962,7 @@ int build_lists(iris_handle* h, ListSet& out, int nobox, int face_on, bool rec =
   out.key = key;
   out.gen = h->pass_gen;
   out.on = false;
-  out.n2 = out.nlean = out.ngen = 0;
+  out.n2 = out.n2p = out.nlean = out.ngen = 0;
   // record lists with two cells a lane: the record planes may be two-step
   // planes, whose kernel then exports their H of step n+1
   const bool rec_two = rec && sizeof(T) == 4 && h->two_step && h->t2w == 128;
@@ -1988,12 +1994,25 @@ int build_lists(iris_handle* h, ListSet& out, int nobox, int face_on, bool rec =
     for (int q = z - 2; ok && q <= z + 2; ++q) ok = eligible(q);
     two[z] = ok;
   }
+  // record lists: the UPML slabs in the UPML variant of the two-step pass
+  // where the handle's own lists take it (its output planes as there: no
+  // TF/SF plane in z-1 .. z+2, some UPML plane among them), marked 2
+  if (rec && h->two_step && h->n2p > 0 && two_step_fits(p, t2geom<T>(h->t2wp, true)))
+    for (int z = 2; z <= p.nz - 2; ++z) {
+      if (two[z]) continue;
+      bool ok = true, pml = false;
+      for (int q = z - 1; q <= z + 2; ++q) {
+        ok = ok && !(p.em_source && (q == p.iz_min - 1 || q == p.iz_min));
+        pml = pml || q <= p.pmlzi || q >= p.pmlzf;
+      }
+      if (ok && pml) two[z] = 2;
+    }
   // runs of fewer than kMinRun2 such planes take single steps
   constexpr int kMinRun2 = 4;
   for (int z = 1; z <= p.nz; ++z) {
     if (!two[z]) continue;
     int e = z;
-    while (e + 1 <= p.nz && two[e + 1]) ++e;
+    while (e + 1 <= p.nz && two[e + 1] == two[z]) ++e;
     if (e - z + 1 < kMinRun2)
       for (int q = z; q <= e; ++q) two[q] = 0;
     else
@@ -2014,20 +2033,22 @@ int build_lists(iris_handle* h, ListSet& out, int nobox, int face_on, bool rec =
   // single-step chunks carry their length in the chunk word (at most
   // kChunkLenMask planes)
   const int zc = std::min(h->zchunk, kChunkLenMask);
-  std::vector<int> c2, rlean, rgen;
+  const int zc2p = std::min(2 * h->zchunk, kChunkLenMask + 1);
+  std::vector<int> c2, c2p, rlean, rgen;
   std::vector<std::pair<int, int>> slean, sgen;  // single-step stretches (first, last plane)
   std::vector<unsigned char> zm((size_t)p.nz + 2, 0);
   for (int z = 1; z <= p.nz;) {
     int e = z;
     while (e + 1 <= p.nz && two[e + 1] == two[z]) ++e;
     if (two[z]) {
-      // a run of two-step planes: near-equal chunks of up to zc2 planes,
-      // the first exporting below, the last above
+      // a run of two-step planes: near-equal chunks of up to zc2 planes
+      // (zc2p in the UPML variant), the first exporting below, the last above
       std::vector<int> run;
-      cut(z, e, zc2, [&](int c0, int l) { run.push_back(c0 | ((l - 1) << kChunkLenShift)); });
+      cut(z, e, two[z] == 2 ? zc2p : zc2, [&](int c0, int l) { run.push_back(c0 | ((l - 1) << kChunkLenShift)); });
       run.front() |= kChunkExportBelow;
       run.back() |= kChunkExportAbove;
-      c2.insert(c2.end(), run.begin(), run.end());
+      auto& dst = two[z] == 2 ? c2p : c2;
+      dst.insert(dst.end(), run.begin(), run.end());
       for (int q = z; q <= e; ++q) zm[q] = 1;
     } else {
       // single-step planes: lean where the planes a chunk reads or writes
@@ -2079,15 +2100,20 @@ int build_lists(iris_handle* h, ListSet& out, int nobox, int face_on, bool rec =
     // sized for the most chunks any face mask can give (one a plane)
     const size_t cap = (size_t)p.nz + 1;
     if ((rc = adj_alloc(h, (void**)&out.chunks2, sizeof(int) * cap)) ||
+        (rc = adj_alloc(h, (void**)&out.chunks2p, sizeof(int) * cap)) ||
         (rc = adj_alloc(h, (void**)&out.rest, sizeof(int) * cap)) || (rc = adj_alloc(h, (void**)&out.zmask, zm.size())))
       return rc;
   }
-  HIP_TRY(h, hipMemcpyAsync(out.chunks2, c2.data(), sizeof(int) * c2.size(), hipMemcpyHostToDevice, h->stream));
+  if (!c2.empty())
+    HIP_TRY(h, hipMemcpyAsync(out.chunks2, c2.data(), sizeof(int) * c2.size(), hipMemcpyHostToDevice, h->stream));
+  if (!c2p.empty())
+    HIP_TRY(h, hipMemcpyAsync(out.chunks2p, c2p.data(), sizeof(int) * c2p.size(), hipMemcpyHostToDevice, h->stream));
   if (!rest.empty())
     HIP_TRY(h, hipMemcpyAsync(out.rest, rest.data(), sizeof(int) * rest.size(), hipMemcpyHostToDevice, h->stream));
   HIP_TRY(h, hipMemcpyAsync(out.zmask, zm.data(), zm.size(), hipMemcpyHostToDevice, h->stream));
   HIP_TRY(h, hipStreamSynchronize(h->stream));
   out.n2 = (int)c2.size();
+  out.n2p = (int)c2p.size();
   out.nlean = (int)rlean.size();
   out.ngen = (int)rgen.size();
   out.on = true;
