@@ -136,7 +136,7 @@ struct PassConfig {
   int xcd_rows2 = -1; // the same for the two-step pass (-1: 2 with two cells a lane, else 4)
   int w2 = 0;         // two-step tile width: 64, 32 or 128 (two cells a lane, FP32 fields); 0: by the grid
   int w2p = 0;        // the same for its UPML variant (0: 32)
-  int nt2 = -1;       // two-step pass streams its outputs past the L2: 0 no, 1 yes, -1 up to 160 x 160 cells a plane, with two cells a lane or FP64 fields
+  int nt2 = -1;       // two-step pass streams its outputs past the L2: 0 no, 1 yes, -1 up to 160 x 160 cells a plane or with two cells a lane
   int p2ox = 0;       // two cells a lane: output cells a row of the main tiles (124, 60, 28; 0: by the grid)
 };
 
@@ -1284,7 +1284,7 @@ int build_passes(iris_handle* h, const PassConfig& c) {
   h->xcd_rows2 = c.xcd_rows2 >= 0 ? c.xcd_rows2 : 4;  // resolved below once the tile width is known
   // Streaming stores of the two-step outputs: measured 6-10% faster at
   // 128 x 128 x 768 and, with FP32 fields one cell a lane, 3-5% slower at
-  // 512 x 512 x 1024 (same box); on for two cells a lane and FP64 below
+  // 512 x 512 x 1024 (same box); on for two cells a lane below
   h->nt2 = c.nt2 >= 0 ? c.nt2 : (p.nx <= 160 && p.ny <= 160 ? 1 : 0);
   const size_t fbytes = h->csize * 3 * (size_t)h->cs;
   const size_t abytes = h->csize * 3 * (size_t)(h->ps0 + h->ps1);
@@ -1369,10 +1369,10 @@ int build_passes(iris_handle* h, const PassConfig& c) {
     if (W != 128 && !two_step_fits(p, t2geom<T>(W, false)) && two_step_fits(p, t2geom<T>(96 - W, false)))
       W = 96 - W;
     h->t2w = W;
-    // streaming stores at 512 x 512 x 1024 with two cells a lane: 4-6%
-    // faster (whole steps, eight runs of each interleaved in one process);
-    // with FP64 fields: 4% faster per launch (three runs of each)
-    if (c.nt2 < 0 && (W == 128 || sizeof(T) == 8)) h->nt2 = 1;
+    // streaming stores at 512 x 512 x 1024 with two cells a lane: within
+    // noise, ~2% faster per launch (four runs of each in ABBA order; with
+    // FP64 fields ~2% slower, so those keep the rule above)
+    if (c.nt2 < 0 && W == 128) h->nt2 = 1;
     // tile rows per XCD run: 2 with two cells a lane (measured 3% faster
     // than 4 at 512 x 512 x 1024 over four interleaved runs each)
     if (c.xcd_rows2 < 0) h->xcd_rows2 = W == 128 ? 2 : 4;
