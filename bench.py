@@ -45,6 +45,8 @@ fastest; --no-autotune keeps the default layout.
   python bench.py --pmc-probe [--fp64] [--pmc-steps N]     workload of the counter runs
   python bench.py --bw-probe                               streaming bandwidth of plain kernels
   python bench.py --report-stats CSV [--command CMD] [--steps-fp32 N] [--steps-fp64 N]
+  python bench.py --report-phases TRACE_CSV [--split-at REGEX]   kernel totals before/after a launch
+  python bench.py --report-sq PMC_DIR [--kernels REGEX]          mean SQ counters per dispatch
   python bench.py --report-pmc FETCH_DIR WRITE_DIR --pmc-steps N --out-dir DIR
                   [--command-fetch CMD --command-write CMD]
 
@@ -766,10 +768,65 @@ def _read_counter(d, counter):
     return rows
 
 
+def report_phases(path, split_at):
+    """Per-kernel totals of a rocprofv3 kernel trace (kernel_trace.csv), in
+    two phases split at the first launch whose kernel name matches the
+    regular expression split_at (kernel end minus start, summed)."""
+    rows = sorted(csv.DictReader(open(path)), key=lambda r: int(r["Start_Timestamp"]))
+    starts = [int(r["Start_Timestamp"]) for r in rows if re.search(split_at, r["Kernel_Name"])]
+    if not starts:
+        raise SystemExit(f"no kernel matches {split_at!r}")
+    t0 = starts[0]
+    phases = [{}, {}]
+    for r in rows:
+        k = 1 if int(r["Start_Timestamp"]) >= t0 else 0
+        e = phases[k].setdefault(short(r["Kernel_Name"]), [0, 0.0])
+        e[0] += 1
+        e[1] += (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6
+    out = []
+    for k, title in ((0, f"Before the first `{split_at}` launch"), (1, f"From the first `{split_at}` launch")):
+        tot = sum(v[1] for v in phases[k].values())
+        out += [f"## {title} (kernel time {tot:.0f} ms)", "", "| kernel | calls | total (ms) |", "|---|---:|---:|"]
+        for n, v in sorted(phases[k].items(), key=lambda kv: -kv[1][1]):
+            out.append(f"| `{n}` | {v[0]} | {v[1]:.1f} |")
+        out.append("")
+    print("\n".join(out))
+
+
+SQ_COUNTERS = ("SQ_WAVES", "SQ_WAVE_CYCLES", "SQ_WAIT_ANY", "SQ_WAIT_INST_ANY", "SQ_ACTIVE_INST_VALU",
+               "SQ_INSTS_VALU", "SQ_ACTIVE_INST_LDS")
+
+
+def report_sq(d, pattern):
+    """Mean SQ counters per dispatch of the kernels whose name matches the
+    regular expression `pattern`, from a rocprofv3 --pmc run, with the
+    wait and VALU fractions of the waves' cycles."""
+    files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        raise SystemExit(f"no counter_collection.csv under {d}")
+    acc = {}
+    for f in files:
+        for r in csv.DictReader(open(f)):
+            if re.search(pattern, r["Kernel_Name"]) and r["Counter_Name"] in SQ_COUNTERS:
+                acc.setdefault(short(r["Kernel_Name"]), {}).setdefault(r["Counter_Name"], []).append(
+                    float(r["Counter_Value"]))
+    out = ["| kernel | " + " | ".join(SQ_COUNTERS) + " |", "|---|" + "---:|" * len(SQ_COUNTERS)]
+    ratios = []
+    for n, c in sorted(acc.items()):
+        m = {k: sum(v) / len(v) for k, v in c.items()}
+        out.append(f"| `{n}` | " + " | ".join(f"{m.get(k, 0):.0f}" for k in SQ_COUNTERS) + " |")
+        if m.get("SQ_WAVE_CYCLES") and m.get("SQ_WAVES"):
+            wc = m["SQ_WAVE_CYCLES"]
+            ratios.append(f"`{n}`: waves wait (any) {100 * m.get('SQ_WAIT_ANY', 0) / wc:.1f}% of their cycles, "
+                          f"wait to issue {100 * m.get('SQ_WAIT_INST_ANY', 0) / wc:.1f}%, VALU active "
+                          f"{100 * m.get('SQ_ACTIVE_INST_VALU', 0) / wc:.1f}%, "
+                          f"{m.get('SQ_INSTS_VALU', 0) / m['SQ_WAVES']:.0f} VALU instructions a wave")
+    print("\n".join(out + [""] + ratios))
+
+
 def report_pmc(fetch_dir, write_dir, steps, out_dir, cmd_fetch, cmd_write):
-    # launches of a class: the dispatches of its most frequent kernel (the
-    # two-step pass of FP32 fields over nx = 512 is two kernels a launch:
-    # 124-cell tiles of two cells a lane and the strip of 16 columns beside)
+    # launches of a class: the dispatches of its most frequent kernel (a
+    # class whose launch is several kernels counts the most frequent one)
     res, per_kernel = {}, {}
     for counter, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
         rows = _read_counter(d, counter)
@@ -858,7 +915,17 @@ def main():
     ap.add_argument("--out-dir", default=None)
     ap.add_argument("--command-fetch", default="")
     ap.add_argument("--command-write", default="")
+    ap.add_argument("--report-phases", metavar="TRACE_CSV")
+    ap.add_argument("--split-at", default=r"_batch<.*|k_yee2p_batch")
+    ap.add_argument("--report-sq", metavar="PMC_DIR")
+    ap.add_argument("--kernels", default=r"k_yee2p<|k_yee2<float, false, true")
     a = ap.parse_args()
+    if a.report_phases:
+        report_phases(a.report_phases, a.split_at)
+        return
+    if a.report_sq:
+        report_sq(a.report_sq, a.kernels)
+        return
     if a.report_stats:
         report_stats(a.report_stats, a.command, a.steps_fp32, a.steps_fp64)
         return
