@@ -2122,8 +2122,9 @@ int build_lists(iris_handle* h, ListSet& out, int nobox, int face_on, bool rec =
 
 // Steps it_first..it_last of the recorded tail (all after step tail_t0): H of
 // every step goes to its record slot; two steps per pass on the record
-// lists (their single-step chunks hold the record planes, so H of the first
-// step is whole there in the second buffer), the DFT as in run_steps.
+// lists (the record planes are in their single-step chunks, or in two-step
+// chunks of two cells a lane that export their H of the first step, so that
+// H is whole there in the second buffer), the DFT as in run_steps.
 template <typename T>
 int run_tail(iris_handle* h, int it_first, int it_last) {
   const iris_params& p = h->p;
